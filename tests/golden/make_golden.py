@@ -1,0 +1,276 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ by running the compiled Osprey-DPD
+reference (oracle/_ref/dpd, built from /root/reference/src by oracle/ref.mk).
+
+Run in the development container only (the reference binary and sources do not
+exist on the GPU machine):
+
+    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py
+
+Every case is a small box.  The reference first builds a random state and relaxes it
+for WARM steps (``State random``), then restarts from the 6-digit restart snapshot it
+wrote (``State restart``, src/InitialStateRestart.cpp:96-139, src/RestartState.cpp:250-330)
+and advances it:
+
+* ``nf1``  one step with ``Command ToggleDPDBeadThermostat 1`` (gamma = 0, so the
+           dissipative and random forces vanish: src/CNTCell.cpp:318-346);
+* ``th1``  one step with the thermostat on (serial LCG stream, src/CNTCell.cpp:2454-2465);
+* ``th10`` ten steps with the thermostat on, plus the ten ``dmpchs`` history rows
+           (time, kT, P, ... : src/Monitor.cpp:1750-1823).
+
+Both sides parse the same text snapshot, so their inputs are bit-identical.  The
+fixtures keep the reference's own 6-significant-digit values
+(src/RestartState.cpp:134-156).  Files are gzip'd text: one bead per line,
+``polyId polyType beadId beadType x y z ux uy uz vx vy vz fx fy fz``.
+
+``rng.txt`` holds the first 1000 draws of the reference LCG for seed |RNGSeed|, logged
+by ``Command GenerateRNGUniform`` (src/mcGenerateRNGUniformImpl.cpp:44-62).
+"""
+import gzip
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref", "dpd")
+
+WARM = 200
+
+HEADER = """dpd
+
+Title	" {title} "
+Date    02/12/26
+Comment	" golden fixture "
+
+"""
+
+TAIL = """Box         {nx} {ny} {nz}       1  1  1
+Density		3
+Temp        1
+RNGSeed		{seed}
+Lambda		0.5
+Step		{dt}
+Time		{time}
+SamplePeriod     1
+AnalysisPeriod	 {time}
+DensityPeriod    {time}
+DisplayPeriod    {time}
+RestartPeriod    {time}
+Grid		1  1  1
+"""
+
+# name -> (box, seed, dt, body text between State and Box)
+CASES = {
+    "water6": ((6, 6, 6), -4321, 0.02, """Bead  W
+      0.5
+      25
+      4.5
+
+Polymer	Water    1.0   " (W) "
+
+"""),
+    "binary6": ((6, 6, 6), -977, 0.02, """Bead  W
+      0.5
+      25
+      4.5
+
+Bead  O
+      0.5
+      80  25
+      4.5 4.5
+
+Polymer	Water    0.5   " (W) "
+Polymer	Oil      0.5   " (O) "
+
+"""),
+    "chains7": ((7, 7, 7), -31337, 0.02, """Bead  W
+      0.5
+      25
+      4.5
+
+Bead  S
+      0.5
+      25 10
+      4.5 4.5
+
+Bead  P
+      0.5
+      25 25 25
+      4.5 4.5 4.5
+
+Bond    S P  128  0.5
+Bond    P P  128  0.5
+
+Polymer	Water    0.984615   " (W) "
+Polymer	Chain    0.015385   " (S (3 (P P P S)) P P P) "
+
+"""),
+    "lipid8": ((8, 8, 8), -8086, 0.02, """Bead  H
+      0.5
+      25
+      4.5
+
+Bead  T
+      0.5
+      35  25
+      4.5 4.5
+
+Bead  W
+      0.5
+      35  75  25
+      4.5 4.5 4.5
+
+Bond    H H  128  0.5
+Bond    H T  128  0.5
+Bond    T T  128  0.5
+
+BondPair  H T T  20.0  0.0
+BondPair  T T T  20.0  0.0
+
+Polymer	Water    0.90   " (W) "
+Polymer	Lipid    0.10   " (H H (* (T T T T)) H T T T T) "
+
+"""),
+    "lipidphi8": ((8, 8, 8), -5150, 0.02, """Bead  H
+      0.5
+      25
+      4.5
+
+Bead  T
+      0.5
+      35  25
+      4.5 4.5
+
+Bead  W
+      0.5
+      35  75  25
+      4.5 4.5 4.5
+
+Bond    H H  128  0.5
+Bond    H T  128  0.5
+Bond    T T  128  0.5
+
+BondPair  H H T  10.0  60.0
+BondPair  H T T  15.0  30.0
+BondPair  T T T  20.0  0.0
+
+Polymer	Water    0.90   " (W) "
+Polymer	Lipid    0.10   " (H H (* (T T T T)) H T T T T) "
+
+"""),
+}
+
+
+def dmpci(name, state, time, extra=""):
+    box, seed, dt, body = CASES[name]
+    txt = HEADER.format(title=name) + "State\t" + state + "\n\n" + body
+    txt += TAIL.format(nx=box[0], ny=box[1], nz=box[2], seed=seed, dt=dt, time=time)
+    return txt + extra
+
+
+def run(work, run_id, text):
+    with open(os.path.join(work, "dmpci." + run_id), "w") as fh:
+        fh.write(text)
+    r = subprocess.run([REF, run_id], cwd=work, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference run {run_id} failed: {r.stdout.decode()[-2000:]}")
+
+
+def bead_lines(path):
+    """Bead lines of a reference restart file (drops the flag line, the radius column and
+    the 'inclusive' tail)."""
+    out = []
+    with open(path) as fh:
+        fh.readline()
+        for line in fh:
+            tok = line.split()
+            if not tok or tok[0] == "inclusive":
+                break
+            out.append(" ".join(tok[:4] + tok[5:]))
+    return out
+
+
+def write_gz(path, lines):
+    with gzip.open(path, "wt") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def history(path):
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            tok = line.split()
+            if tok:
+                rows.append(tok)
+    return rows
+
+
+def count_summary(path):
+    keys = {"polymers": r"Number of polymers created\s+(\d+)",
+            "beads": r"Number of beads created\s+(\d+)",
+            "bonds": r"Number of bonds created\s+(\d+)",
+            "bondpairs": r"Number of bond pairs created\s+(\d+)"}
+    txt = open(path).read()
+    return {k: int(re.search(p, txt).group(1)) for k, p in keys.items()}
+
+
+def make_case(name, work):
+    out = os.path.join(HERE, name)
+    os.makedirs(out, exist_ok=True)
+    w = name + "w"
+    run(work, w, dmpci(name, "random", WARM))
+    restart = f"restart RunId {w} StateId {WARM}"
+    write_gz(os.path.join(out, "in.dat.gz"), bead_lines(os.path.join(work, f"dmpcrs.{w}.con.{WARM}.dat")))
+    run(work, name + "n", dmpci(name, restart, 1, "Command ToggleDPDBeadThermostat 1\n"))
+    write_gz(os.path.join(out, "nf1.dat.gz"), bead_lines(os.path.join(work, f"dmpcrs.{name}n.con.1.dat")))
+    run(work, name + "t", dmpci(name, restart, 1))
+    write_gz(os.path.join(out, "th1.dat.gz"), bead_lines(os.path.join(work, f"dmpcrs.{name}t.con.1.dat")))
+    run(work, name + "u", dmpci(name, restart, 10))
+    write_gz(os.path.join(out, "th10.dat.gz"), bead_lines(os.path.join(work, f"dmpcrs.{name}u.con.10.dat")))
+    hist = history(os.path.join(work, f"dmpchs.{name}u"))
+    with open(os.path.join(out, "th10_history.txt"), "w") as fh:
+        fh.write("# time kT P ... (reference dmpchs rows, thermostat on)\n")
+        for row in hist:
+            fh.write(" ".join(row) + "\n")
+    with open(os.path.join(out, "case.dmpci"), "w") as fh:
+        fh.write(dmpci(name, "restart RunId golden StateId 0", 10))
+    with open(os.path.join(out, "counts.json"), "w") as fh:
+        json.dump(count_summary(os.path.join(work, f"dmpcis.{name}u")), fh)
+
+
+def make_rng(work):
+    # a restart-based run draws nothing before step 1, so the logged draws are the
+    # first draws of the stream seeded with |RNGSeed|
+    run(work, "rngw", dmpci("water6", "random", WARM))
+    run(work, "rng", dmpci("water6", f"restart RunId rngw StateId {WARM}", 1,
+                           "Command GenerateRNGUniform 1 1000\n"))
+    txt = open(os.path.join(work, "dmpcls.rng")).read()
+    i = txt.index("GenerateRNGSequence\n1000 Uniform\n")
+    vals = txt[i:].split("\n")[2:1002]
+    with open(os.path.join(HERE, "rng.txt"), "w") as fh:
+        fh.write(f"# seed {abs(CASES['water6'][1])}: first 1000 reference LCG uniforms (6 digits)\n")
+        fh.write("\n".join(vals) + "\n")
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("build the reference first: make -f oracle/ref.mk -j8")
+    work = tempfile.mkdtemp(prefix="dpd_golden_")
+    try:
+        make_rng(work)
+        names = sys.argv[1:] or list(CASES)
+        for name in names:
+            make_case(name, work)
+            print("wrote", name)
+    finally:
+        shutil.rmtree(work)
+
+
+if __name__ == "__main__":
+    main()
