@@ -1,0 +1,145 @@
+/*
+ * dpd_engine.h -- C ABI of the MI355X DPD step engine (libospreydpd_amd.so).
+ *
+ * The engine replaces the body of the reference's per-step loop:
+ *   CSimBox::Evolve (src/SimBox.cpp:977-1120) = UpdatePos over all cells
+ *   (src/CNTCell.cpp:1598), UpdateForce (src/CNTCell.cpp:877), the bond and
+ *   bond-pair loops (src/SimBox.cpp:1382-1439 -> src/Bond.cpp:111,
+ *   src/BondPair.cpp:138) and UpdateMom (src/CNTCell.cpp:2280),
+ * with hand-written HIP kernels for gfx950.  The reference has no plugin or FFI
+ * interface: the seam is the call of Evolve() from CSimBox::Run
+ * (src/SimBox.cpp:1320); constants are pushed into the CCNTCell statics by
+ * CSimState (src/SimState.cpp:69-81, :112), which dpd_create mirrors.
+ *
+ * Conventions: plain C types only; every call returns DPD_OK (0) or an error code
+ * and leaves a message for dpd_last_error(); no exceptions cross the ABI.  One
+ * engine per host thread; each engine owns its device buffers and one HIP stream.
+ * Host arrays are copied on upload and never aliased.  Bead arrays are in "upload
+ * order" (the order of the arrays given to dpd_upload_beads); topology refers to
+ * beads by that 0-based index.  The bead `id` (the reference's CAbstractBead id)
+ * keys the pair random stream only.
+ */
+#ifndef DPD_ENGINE_H
+#define DPD_ENGINE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference reports these conditions through bool returns and
+ * CLog* messages: src/LogMessage.cpp:33-37, src/CNTCell.cpp:2628-2693) */
+#define DPD_OK              0
+#define DPD_ERR_PARAM       1   /* invalid argument or parameter                      */
+#define DPD_ERR_OOM         2   /* device allocation failed                           */
+#define DPD_ERR_HIP         3   /* HIP runtime error                                  */
+#define DPD_ERR_NAN         4   /* a force or velocity became NaN/Inf                 */
+#define DPD_ERR_OUT_OF_BOX  5   /* uploaded bead outside [0, L)                       */
+#define DPD_ERR_STATE       6   /* call out of order (e.g. step before upload)        */
+#define DPD_ERR_IO          7   /* file or parse error (host layer)                   */
+#define DPD_ERR_UNSUPPORTED 8   /* configuration the engine does not handle           */
+
+typedef struct dpd_engine dpd_engine;
+
+/* Box and integration constants.  Mirrors CCNTCell::SetSimBoxLengths
+ * (src/CNTCell.cpp:143-158), SetTimeStepConstants (:180-203), SetRNGSeed
+ * (:134-137) and SetDPDBeadIntArrays (src/SimState.cpp:112). */
+typedef struct dpd_params {
+    int32_t nx, ny, nz;         /* CNT cells per axis (>= 3)                         */
+    double wx, wy, wz;          /* cell widths (>= 1: the cutoff is 1)               */
+    double dt;                  /* Step                                              */
+    double lambda;              /* Lambda (velocity prediction, 0..1)                */
+    double kT;                  /* Temp                                              */
+    uint64_t seed;              /* |RNGSeed|: key of the pair random stream          */
+    int32_t ntypes;             /* bead types (<= DPD_MAX_TYPES)                     */
+    const double* a;            /* ntypes*ntypes conservative a_ij, row-major        */
+    const double* gamma;        /* ntypes*ntypes dissipative gamma_ij                */
+    int32_t device;             /* HIP device ordinal                                */
+    int32_t pair_kernel;        /* DPD_PAIR_AUTO / _BRICK / _FULLSHELL               */
+} dpd_params;
+
+#define DPD_MAX_TYPES 16
+
+#define DPD_PAIR_AUTO      0    /* tuned choice                                      */
+#define DPD_PAIR_BRICK     1    /* LDS cell-brick half-shell kernel                  */
+#define DPD_PAIR_FULLSHELL 2    /* thread-per-bead 27-cell kernel (cross-check)      */
+
+/* FP64 observables of the last completed step (CMonitor::CalculateSingleBeadData,
+ * src/Monitor.cpp:790-1041).  stress = (sum over pairs and bonds of r (x) F) / V,
+ * component [3a+b] = sum dx_b F_a; bond-angle forces add no stress, as in the
+ * reference. */
+typedef struct dpd_obs {
+    int64_t step;               /* simulation time of the sample                     */
+    double kT;                  /* sum v^2 / (3N)                                    */
+    double pressure;            /* 0.3333333333/V * (sum v^2 + trace sum r(x)F)     */
+    double stress[9];
+    double sum_v2[3];
+    double momentum[3];         /* sum v (unit mass)                                 */
+    double pe_pair;             /* sum 1/2 a (1-r)^2                                 */
+    double pe_bond;             /* sum 1/2 k (L-l0)^2                                */
+    double pe_angle;            /* CBondPair::AddPotentialEnergy                     */
+    int64_t jumps;              /* bead moves of more than one cell in the last call */
+    int64_t nbeads;
+} dpd_obs;
+
+/* flags of dpd_step / dpd_compute_forces */
+#define DPD_STEP_OBSERVE   1u   /* accumulate FP64 observables at the last step      */
+#define DPD_NO_THERMOSTAT  2u   /* dpd_compute_forces: drop dissipative + random     */
+
+/* toggles of dpd_toggle (command handlers of src/CNTCell.cpp:223-346) */
+#define DPD_TOGGLE_THERMOSTAT     1   /* ToggleDPDBeadThermostat: on=0 zeroes gamma  */
+#define DPD_TOGGLE_CONSERVATIVE   2   /* ToggleDPDBeadConservativeForces              */
+
+/* Create an engine on p->device.  Replaces the CCNTCell static set-up. */
+int  dpd_create(const dpd_params* p, dpd_engine** out);
+void dpd_destroy(dpd_engine* e);
+const char* dpd_last_error(const dpd_engine* e);
+
+/* Beads (FP64, [3n] xyz-interleaved).  x must lie in [0, L); unpbc may be NULL
+ * (then unwrapped = wrapped); f may be NULL (zero), as for a fresh state.  Replaces
+ * CSimBox::AssignBeadstoCells (src/SimBox.cpp:715-762) and the bead fields the
+ * restart reader fills (src/RestartState.cpp:312-330).  step0 is the simulation
+ * time of the uploaded state. */
+int dpd_upload_beads(dpd_engine* e, int64_t n, const int32_t* id, const int32_t* type,
+                     const double* x, const double* unpbc, const double* v, const double* f,
+                     int64_t step0);
+
+/* Bonds: head/tail bead indices (CBond m_pHead/m_pTail, src/InputData.cpp:1499-1500),
+ * spring constant and unstretched length (src/Bond.cpp:111-162).
+ * Bond pairs: indices of the first and second bond (CBondPair m_pFirst/m_pSecond,
+ * stored in reverse chain order, src/InputData.cpp:1155-1156), modulus and phi0 in
+ * radians (src/BondPairType.cpp:85).  Either count may be 0. */
+int dpd_upload_topology(dpd_engine* e, int64_t nbonds, const int32_t* head, const int32_t* tail,
+                        const double* k, const double* l0, int64_t npairs,
+                        const int32_t* first_bond, const int32_t* second_bond,
+                        const double* modulus, const double* phi0);
+
+/* Advance nsteps steps (one CSimBox::Evolve each).  Inputs and outputs stay in
+ * device memory; the call returns once the steps are enqueued and checked. */
+int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags);
+
+/* One force evaluation at the uploaded positions, treating the uploaded velocities
+ * as the predicted velocities, with the pair stream of simulation time `step`.
+ * f_out [3n] in upload order; virial[9] and pe[3] may be NULL. */
+int dpd_compute_forces(dpd_engine* e, int64_t step, uint32_t flags, double* f_out,
+                       double* virial, double* pe);
+
+int dpd_observables(dpd_engine* e, dpd_obs* out);
+
+/* Copy the state back in upload order (any pointer may be NULL). */
+int dpd_download(dpd_engine* e, double* x, double* unpbc, double* v, double* f);
+
+/* Command hooks (src/CNTCell.cpp:357-377, :434-456, :223-346) */
+int dpd_set_interactions(dpd_engine* e, const double* a, const double* gamma);
+int dpd_set_timestep(dpd_engine* e, double dt);
+int dpd_toggle(dpd_engine* e, int32_t what, int32_t on);
+
+/* Timing of the last dpd_step call's kernels (ms, from HIP events on the
+ * engine's stream): [0] whole call, [1] pair kernel total, [2] other kernels.
+ * Returns the number of pair-kernel launches timed. */
+int64_t dpd_last_timing(dpd_engine* e, double* ms3);
+int64_t dpd_get_step(const dpd_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -305,12 +305,14 @@ static void pair_force(const pair_ctx* c, int t1, int t2, int id1, int id2, cons
     double wr2 = wr * wr;
     double conForce = c->a[t1 * c->nt + t2] * wr;
     double rdotv = (dx[0] * dv[0] + dx[1] * dv[1] + dx[2] * dv[2]) / dr;
-    double gammap = c->g[t1 * c->nt + t2] * wr2;
+    /* thermostat: 1 = dissipative + random (reference), 0 = neither (gamma = 0, as
+     * ToggleDPDBeadThermostat), 2 = dissipative only (closed-form checks) */
+    double gammap = c->thermostat ? c->g[t1 * c->nt + t2] * wr2 : 0.0;
     double dissForce = -gammap * rdotv;
     double U;
     if (c->rng_mode == ORC_RNG_LCG) U = orc_lcg_uniform(c->lcg);
     else U = orc_pair_uniform(c->seed, c->step, id1, id2);
-    double randForce = c->thermostat ? sqrt(gammap) * c->invrootdt * (0.5 - U) : 0.0;
+    double randForce = c->thermostat == 2 ? 0.0 : sqrt(gammap) * c->invrootdt * (0.5 - U);
     double nf[3];
     nf[0] = (conForce + dissForce + randForce) * dx[0] / dr;
     nf[1] = (conForce + dissForce + randForce) * dx[1] / dr;

@@ -88,7 +88,8 @@ void     orc_set_gamma(orc_sys* s, const double* g);   /* thermostat toggles */
 void     orc_set_a(orc_sys* s, const double* a);
 
 /* One force evaluation at given positions and (predicted) velocities, Philox stream
- * keyed by `step`.  Cell-list half shell; forces, virial (9, sum r_ij (x) F_ij incl.
+ * keyed by `step`.  thermostat: 1 = dissipative + random, 0 = neither, 2 = dissipative
+ * only.  Cell-list half shell; forces, virial (9, sum r_ij (x) F_ij incl.
  * bonds, no angles) and potential energies out.  Used for kernel parity. */
 void orc_forces(const orc_params* p, int32_t n, const int32_t* id, const int32_t* type,
                 const double* x, const double* unpbc, const double* v,

@@ -178,7 +178,8 @@ class System:
 
 
 def forces(box, ids, types, x, v, step, thermostat=True, unpbc=None, topo=None):
-    """One FP64 force evaluation (Philox pair stream).  Returns f[n,3], virial[9], pe[3]."""
+    """One FP64 force evaluation (Philox pair stream).  Returns f[n,3], virial[9], pe[3].
+    thermostat: True/1 = dissipative + random, False/0 = neither, 2 = dissipative only."""
     L = lib()
     n = len(ids)
     ids = np.ascontiguousarray(ids, dtype=np.int32)
@@ -191,7 +192,7 @@ def forces(box, ids, types, x, v, step, thermostat=True, unpbc=None, topo=None):
     ts = t.struct()
     f = np.zeros((n, 3)); vir = np.zeros(9); pe = np.zeros(3)
     L.orc_forces(C.byref(p), n, _p(ids), _p(types), _p(x), _p(u), _p(v), C.byref(ts), int(step),
-                 int(bool(thermostat)), _p(f), _p(vir), _p(pe))
+                 int(thermostat), _p(f), _p(vir), _p(pe))
     return f, vir, pe
 
 
@@ -205,7 +206,7 @@ def pair_forces_brute(box, ids, types, x, v, step, thermostat=True):
     p = box.params()
     f = np.zeros((n, 3)); vir = np.zeros(9)
     L.orc_pair_forces_brute(C.byref(p), n, _p(ids), _p(types), _p(x), _p(v), int(step),
-                            int(bool(thermostat)), _p(f), _p(vir))
+                            int(thermostat), _p(f), _p(vir))
     return f, vir
 
 

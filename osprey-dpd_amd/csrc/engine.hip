@@ -1,0 +1,995 @@
+// engine.hip -- the DPD step engine for MI355X (gfx950): kernels of one box and the
+// C ABI of include/dpd_engine.h.
+//
+// One step (the reference's CSimBox::Evolve, src/SimBox.cpp:977-1120) runs as:
+//   k_advance   UpdateMom of the previous step fused with UpdatePos of this one
+//               (src/CNTCell.cpp:2280-2310, :1704-1757), bonded forces of the previous
+//               step (src/Bond.cpp:111-162, src/BondPair.cpp:138-286) added first;
+//               computes each bead's new cell key and its rank in that cell
+//   k_brick_scan / k_cell_start   counting-sort prefix sums (bricks, then cells)
+//   k_permute   scatter into the new cell-sorted SoA buffers
+//   pair kernel DPD pair forces (src/CNTCell.cpp:1144-1555)
+// and a last k_finish closes the final step (UpdateMom + bonded + FP64 observables).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dpd_engine.h"
+#include "engine_common.hpp"
+#include "pair_kernels.hpp"
+
+using namespace dpd;
+
+namespace {
+
+enum AdvanceMode { kRebin = 0, kBegin = 1, kEndBegin = 2 };
+
+// FP64 observable accumulator layout
+enum ObsSlot {
+    O_V2 = 0,      // 3
+    O_MOM = 3,     // 3
+    O_VIR = 6,     // 9
+    O_PEP = 15,
+    O_PEB = 16,
+    O_PEA = 17,
+    O_N = 18
+};
+
+struct Topo {
+    // CSR by bead index
+    const int* bond_ptr;      // [n+1]
+    const int4* bond_ent;     // (other, is_head, bond type row, 0)
+    const double2* bond_par;  // (k, l0) per entry
+    const int* ang_ptr;       // [n+1]
+    const int2* ang_ent;      // (angle, role)
+    const int4* ang_bead;     // (a, b, c, 0) per angle
+    const double4* ang_par;   // (k, phi0, cos phi0, sin phi0)
+    int has_bonds, has_angles;
+};
+
+struct Buffers {
+    float4* pos;   // offset xyz, type bits
+    float4* vel;   // velocity xyz, id bits
+    float4* fold;  // previous force
+    int4* aux;     // image xyz, bead index
+    int* key;      // cell key
+};
+
+__device__ inline void unwrapped_pos(int slot, const Buffers& S, const BoxDev& B, double u[3])
+{
+    int c[3];
+    key_cell(S.key[slot], B, c);
+    const float4 p = S.pos[slot];
+    const int4 a = S.aux[slot];
+    u[0] = (double)(c[0] + a.x * B.n[0]) * B.wd[0] + (double)p.x;
+    u[1] = (double)(c[1] + a.y * B.n[1]) * B.wd[1] + (double)p.y;
+    u[2] = (double)(c[2] + a.z * B.n[2]) * B.wd[2] + (double)p.z;
+}
+
+// CBondPair::AddForce (src/BondPair.cpp:138-286) for one angle, FP64.  first = bond
+// (b -> c), second = bond (a -> b); returns the force on the bead of `role`
+// (0: c = tail of first, 1: b = shared bead, 2: a = head of second).
+__device__ inline void angle_force(const double ua[3], const double ub[3], const double uc[3],
+                                   const double4 par, int role, double f[3], double* pe)
+{
+    double first[3], second[3];
+    for (int d = 0; d < 3; ++d) { first[d] = ub[d] - uc[d]; second[d] = ua[d] - ub[d]; }
+    const double l1 = sqrt(first[0] * first[0] + first[1] * first[1] + first[2] * first[2]);
+    const double l2 = sqrt(second[0] * second[0] + second[1] * second[1] + second[2] * second[2]);
+    f[0] = f[1] = f[2] = 0.0;
+    const double magProduct = l1 * l2;
+    if (!(magProduct > 0.0001)) { if (pe) *pe = 0.0; return; }
+    const double k = par.x, phi0 = par.y, cosPhi0 = par.z, sinPhi0 = par.w;
+    const double b1Dotb2 = first[0] * second[0] + first[1] * second[1] + first[2] * second[2];
+    const double b1b2Overb1Sq = b1Dotb2 / (l1 * l1);
+    const double b1b2Overb2Sq = b1Dotb2 / (l2 * l2);
+    const double cosPhiSq = b1b2Overb1Sq * b1b2Overb2Sq;
+    double forceMag;
+    if (fabs(b1Dotb2) > 0.000001) {
+        const double prefactor = sqrt(1.0 / cosPhiSq - 1.0);
+        forceMag = (phi0 > 0.0) ? k * (cosPhi0 - sinPhi0 / prefactor) / magProduct : k / magProduct;
+    } else {
+        forceMag = k / magProduct;
+    }
+    double ft[3], fh[3];
+    for (int d = 0; d < 3; ++d) {
+        ft[d] = forceMag * (b1b2Overb1Sq * first[d] - second[d]);
+        fh[d] = forceMag * (first[d] - b1b2Overb2Sq * second[d]);
+    }
+    for (int d = 0; d < 3; ++d) f[d] = role == 0 ? ft[d] : (role == 2 ? fh[d] : -ft[d] - fh[d]);
+    if (pe) {
+        // CBondPair::AddPotentialEnergy (src/BondPair.cpp:320-378)
+        const double cps = (b1Dotb2 * b1Dotb2) / (magProduct * magProduct);
+        const double sinPhi = sqrt(fmax(0.0, 1.0 - cps));
+        double e;
+        if (b1Dotb2 > 0.000001) e = phi0 > 0.0 ? k * (1.0 - sqrt(cps) * cosPhi0 - sinPhi * sinPhi0) : k * (1.0 - sqrt(cps));
+        else if (b1Dotb2 < -0.000001) e = phi0 > 0.0 ? k * (1.0 + sqrt(cps) * cosPhi0 - sinPhi * sinPhi0) : k * (1.0 + sqrt(cps));
+        else e = k * (1.0 - sinPhi0);
+        *pe = e;
+    }
+}
+
+// Bonded force on the bead in `slot` (index q) in FP64, plus (OBS) its share of the
+// bond virial (head bead only, as CBond::AddForce adds stress to the head) and the
+// bonded potential energies.
+__device__ inline void bonded_force(int slot, int q, const Buffers& S, const int* slot_of,
+                                    const Topo& T, const BoxDev& B, double f[3], double vir[9],
+                                    double* pe_b, double* pe_a, bool obs)
+{
+    f[0] = f[1] = f[2] = 0.0;
+    double us[3];
+    bool have = false;
+    if (T.has_bonds) {
+        const int e0 = T.bond_ptr[q], e1 = T.bond_ptr[q + 1];
+        if (e0 < e1) { unwrapped_pos(slot, S, B, us); have = true; }
+        for (int e = e0; e < e1; ++e) {
+            const int4 ent = T.bond_ent[e];
+            const double2 par = T.bond_par[e];
+            double uo[3];
+            unwrapped_pos(slot_of[ent.x], S, B, uo);
+            double d[3] = {us[0] - uo[0], us[1] - uo[1], us[2] - uo[2]};
+            const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            const double s = par.x * (par.y - L) / L;
+            const double fb[3] = {s * d[0], s * d[1], s * d[2]};
+            f[0] += fb[0]; f[1] += fb[1]; f[2] += fb[2];
+            if (obs && ent.y) {
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) vir[3 * a + b] += d[b] * fb[a];
+                *pe_b += 0.5 * par.x * (L - par.y) * (L - par.y);
+            }
+        }
+    }
+    if (T.has_angles) {
+        const int e0 = T.ang_ptr[q], e1 = T.ang_ptr[q + 1];
+        for (int e = e0; e < e1; ++e) {
+            const int2 ent = T.ang_ent[e];
+            const int4 abc = T.ang_bead[ent.x];
+            double ua[3], ub[3], uc[3], fa[3];
+            unwrapped_pos(slot_of[abc.x], S, B, ua);
+            unwrapped_pos(slot_of[abc.y], S, B, ub);
+            unwrapped_pos(slot_of[abc.z], S, B, uc);
+            double e_ang = 0.0;
+            angle_force(ua, ub, uc, T.ang_par[ent.x], ent.y, fa, (obs && ent.y == 1) ? &e_ang : nullptr);
+            f[0] += fa[0]; f[1] += fa[1]; f[2] += fa[2];
+            if (obs && ent.y == 1) *pe_a += e_ang;
+        }
+    }
+    (void)have;
+}
+
+// block reduction of FP64 observables into acc[] with one atomic per slot per block
+template <int N>
+__device__ inline void block_accumulate(double (&v)[N], double* acc)
+{
+    __shared__ double red[N][4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double s = wave_sum(v[i]);
+        if (lane == 0) red[i][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
+        atomicAdd(acc + threadIdx.x, s);
+    }
+}
+
+// ---------------------------------------------------------------- kernels ------
+
+// Close the previous step (kEndBegin) and start the next one: new cell key and rank.
+template <int MODE, bool OBS>
+__global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc,
+                                                 const int* __restrict__ slot_of, Topo T, BoxDev B,
+                                                 int* __restrict__ cnt, int* __restrict__ bcnt,
+                                                 int* __restrict__ nkey, int* __restrict__ rank,
+                                                 int* __restrict__ flags, double* __restrict__ acc)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double ob[O_N];
+    if (OBS)
+        for (int i = 0; i < O_N; ++i) ob[i] = 0.0;
+    if (s < n) {
+        const float4 p = S.pos[s];
+        float4 v = S.vel[s];
+        float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
+        const int4 ax = S.aux[s];
+        if (MODE == kEndBegin) {
+            double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
+            bonded_force(s, ax.w, S, slot_of, T, B, fb, vir, &peb, &pea, OBS);
+            f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
+            const float4 fo = S.fold[s];
+            v.x = v.x + B.hdt * (f.x + fo.x);
+            v.y = v.y + B.hdt * (f.y + fo.y);
+            v.z = v.z + B.hdt * (f.z + fo.z);
+            S.vel[s] = v;
+            frc[s] = f;
+            if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
+                atomicAdd(flags + 0, 1);
+        }
+        int c[3];
+        key_cell(S.key[s], B, c);
+        const float o[3] = {p.x, p.y, p.z};
+        const int img[3] = {ax.x, ax.y, ax.z};
+        float d[3] = {0.f, 0.f, 0.f};
+        if (MODE != kRebin) {
+            d[0] = B.dt * v.x + B.hdt2 * f.x;
+            d[1] = B.dt * v.y + B.hdt2 * f.y;
+            d[2] = B.dt * v.z + B.hdt2 * f.z;
+        }
+        Moved m;
+        advance(o, c, img, d, B, m);
+        const int k = cell_key(m.c[0], m.c[1], m.c[2], B);
+        nkey[s] = k;
+        rank[s] = atomicAdd(cnt + k, 1);
+        atomicAdd(bcnt + (k >> 6), 1);
+        if (m.jump) atomicAdd(flags + 1, 1);
+        if (m.bad) atomicAdd(flags + 0, 1);
+    }
+    if (OBS) block_accumulate(ob, acc);
+}
+
+// exclusive scan of brick counts (one workgroup), clears the counts
+__global__ __launch_bounds__(1024) void k_brick_scan(int nbricks, int* __restrict__ bcnt, int* __restrict__ bstart)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (nbricks + 1023) / 1024;
+    const int b0 = t * per, b1 = min(nbricks, b0 + per);
+    int s = 0;
+    for (int b = b0; b < b1; ++b) s += bcnt[b];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int add = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int b = b0; b < b1; ++b) {
+        const int c = bcnt[b];
+        bstart[b] = run;
+        run += c;
+        bcnt[b] = 0;
+    }
+}
+
+// cell starts: one wavefront per brick scans its 64 cell counts; clears the counts
+__global__ __launch_bounds__(256) void k_cell_start(int nbricks, int n, const int* __restrict__ bstart,
+                                                    int* __restrict__ cnt, int* __restrict__ cstart)
+{
+    const int brick = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (brick >= nbricks) return;
+    const int k = brick * 64 + lane;
+    const int c = cnt[k];
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    cstart[k] = bstart[brick] + incl - c;
+    cnt[k] = 0;
+    if (brick == nbricks - 1 && lane == 63) cstart[nbricks * 64] = n;
+}
+
+// scatter into the new sorted buffers (recomputes the same advance as k_advance)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, const float4* __restrict__ frc,
+                                                 float4* __restrict__ vp, BoxDev B,
+                                                 const int* __restrict__ nkey, const int* __restrict__ rank,
+                                                 const int* __restrict__ cstart, int* __restrict__ slot_of,
+                                                 float4* __restrict__ frc_out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = S.pos[s];
+    const float4 v = S.vel[s];
+    const float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
+    const int4 ax = S.aux[s];
+    int c[3];
+    key_cell(S.key[s], B, c);
+    const float o[3] = {p.x, p.y, p.z};
+    const int img[3] = {ax.x, ax.y, ax.z};
+    float d[3] = {0.f, 0.f, 0.f};
+    if (MODE != kRebin) {
+        d[0] = B.dt * v.x + B.hdt2 * f.x;
+        d[1] = B.dt * v.y + B.hdt2 * f.y;
+        d[2] = B.dt * v.z + B.hdt2 * f.z;
+    }
+    Moved m;
+    advance(o, c, img, d, B, m);
+    const int k = nkey[s];
+    const int dst = cstart[k] + rank[s];
+    D.pos[dst] = make_float4(m.o[0], m.o[1], m.o[2], p.w);
+    D.vel[dst] = v;
+    D.fold[dst] = f;
+    D.aux[dst] = make_int4(m.img[0], m.img[1], m.img[2], ax.w);
+    D.key[dst] = k;
+    if (MODE == kRebin) {
+        vp[dst] = v;
+        frc_out[dst] = f;
+    } else {
+        vp[dst] = make_float4(v.x + B.lamdt * f.x, v.y + B.lamdt * f.y, v.z + B.lamdt * f.z, v.w);
+    }
+    slot_of[ax.w] = dst;
+}
+
+// Close the last step: bonded forces, UpdateMom (UPDATE_V) and FP64 observables.
+template <bool UPDATE_V, bool OBS>
+__global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __restrict__ frc,
+                                                const int* __restrict__ slot_of, Topo T, BoxDev B,
+                                                int* __restrict__ flags, double* __restrict__ acc)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double ob[O_N];
+    if (OBS)
+        for (int i = 0; i < O_N; ++i) ob[i] = 0.0;
+    if (s < n) {
+        float4 f = frc[s];
+        const int4 ax = S.aux[s];
+        double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
+        bonded_force(s, ax.w, S, slot_of, T, B, fb, vir, &peb, &pea, OBS);
+        f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
+        frc[s] = f;
+        float4 v = S.vel[s];
+        if (UPDATE_V) {
+            const float4 fo = S.fold[s];
+            v.x = v.x + B.hdt * (f.x + fo.x);
+            v.y = v.y + B.hdt * (f.y + fo.y);
+            v.z = v.z + B.hdt * (f.z + fo.z);
+            S.vel[s] = v;
+        }
+        if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
+            atomicAdd(flags + 0, 1);
+        if (OBS) {
+            ob[O_V2 + 0] = (double)v.x * v.x;
+            ob[O_V2 + 1] = (double)v.y * v.y;
+            ob[O_V2 + 2] = (double)v.z * v.z;
+            ob[O_MOM + 0] = v.x;
+            ob[O_MOM + 1] = v.y;
+            ob[O_MOM + 2] = v.z;
+            for (int i = 0; i < 9; ++i) ob[O_VIR + i] = vir[i];
+            ob[O_PEB] = peb;
+            ob[O_PEA] = pea;
+        }
+    }
+    if (OBS) block_accumulate(ob, acc);
+}
+
+// state back to upload order in FP64
+__global__ __launch_bounds__(256) void k_gather(int n, Buffers S, const float4* __restrict__ frc, BoxDev B,
+                                                double* __restrict__ x, double* __restrict__ u,
+                                                double* __restrict__ v, double* __restrict__ f)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int c[3];
+    key_cell(S.key[s], B, c);
+    const float4 p = S.pos[s];
+    const int4 a = S.aux[s];
+    const float4 vv = S.vel[s];
+    const float4 ff = frc[s];
+    const int q = a.w;
+    const double o[3] = {p.x, p.y, p.z};
+    const int im[3] = {a.x, a.y, a.z};
+    for (int d = 0; d < 3; ++d) {
+        const double xa = (double)c[d] * B.wd[d] + o[d];
+        x[3 * q + d] = xa;
+        u[3 * q + d] = xa + (double)im[d] * (double)B.n[d] * B.wd[d];
+    }
+    v[3 * q + 0] = vv.x; v[3 * q + 1] = vv.y; v[3 * q + 2] = vv.z;
+    f[3 * q + 0] = ff.x; f[3 * q + 1] = ff.y; f[3 * q + 2] = ff.z;
+}
+
+__global__ void k_zero_doubles(double* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_zero_ints(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side ---
+
+struct dpd_engine {
+    dpd_params p{};
+    std::vector<double> a_h, g_h, a_backup, g_backup;
+    bool thermostat_on = true, conservative_on = true;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    BoxDev B{};
+    double dt = 0, lambda = 0, kT = 0;
+    uint64_t seed = 0;
+    int nbricks = 0, nkeys = 0;
+    int64_t n = 0;
+    int64_t step = 0;
+    int cur = 0;
+    int pair_kernel = DPD_PAIR_FULLSHELL;
+    // device buffers
+    float4 *pos[2] = {}, *vel[2] = {}, *fold[2] = {};
+    int4* aux[2] = {};
+    int* key[2] = {};
+    float4 *frc = nullptr, *vp = nullptr;
+    int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
+    int *cnt = nullptr, *bcnt = nullptr, *bstart = nullptr, *cstart = nullptr;
+    int* flags = nullptr;
+    double* acc = nullptr;
+    float* table = nullptr;
+    double* gather = nullptr;   // 12n doubles for downloads
+    // topology
+    int *bond_ptr = nullptr, *ang_ptr = nullptr;
+    int4* bond_ent = nullptr;
+    double2* bond_par = nullptr;
+    int2* ang_ent = nullptr;
+    int4* ang_bead = nullptr;
+    double4* ang_par = nullptr;
+    int has_bonds = 0, has_angles = 0;
+    int64_t nbonds = 0, nangles = 0;
+    // observables
+    dpd_obs obs{};
+    bool obs_valid = false;
+    int64_t jumps = 0;
+    // timing
+    hipEvent_t ev[4] = {};
+    std::vector<hipEvent_t> pev;
+    double t_total = 0, t_pair = 0;
+    int64_t n_pair = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(dpd_engine* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t _st = (expr);                                                           \
+        if (_st != hipSuccess)                                                             \
+            return fail(e, DPD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_st)); \
+    } while (0)
+
+template <class T>
+int dalloc(dpd_engine* e, T** p, size_t count)
+{
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (count == 0) count = 1;
+    hipError_t st = hipMalloc((void**)p, sizeof(T) * count);
+    if (st != hipSuccess) return fail(e, DPD_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(st));
+    return DPD_OK;
+}
+
+void set_step_constants(dpd_engine* e)
+{
+    // CCNTCell::SetTimeStepConstants / ChangeTimeStepConstants (src/CNTCell.cpp:180-203, :357-377)
+    e->B.dt = (float)e->dt;
+    e->B.hdt = (float)(0.5 * e->dt);
+    e->B.hdt2 = (float)(0.5 * e->dt * e->dt);
+    e->B.lamdt = (float)(e->lambda * e->dt);
+    e->B.invrootdt = (float)std::sqrt(24.0 * e->kT / e->dt);
+}
+
+int upload_table(dpd_engine* e)
+{
+    const int nt = e->p.ntypes;
+    std::vector<float> t(3 * nt * nt);
+    for (int i = 0; i < nt * nt; ++i) {
+        const double a = e->conservative_on ? e->a_h[i] : 0.0;
+        const double g = e->thermostat_on ? e->g_h[i] : 0.0;
+        t[i] = (float)a;
+        t[nt * nt + i] = (float)g;
+        t[2 * nt * nt + i] = (float)std::sqrt(g);
+    }
+    HIPCHK(hipMemcpyAsync(e->table, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DPD_OK;
+}
+
+Buffers bufs(dpd_engine* e, int i)
+{
+    return Buffers{e->pos[i], e->vel[i], e->fold[i], e->aux[i], e->key[i]};
+}
+
+Topo topo(dpd_engine* e)
+{
+    return Topo{e->bond_ptr, e->bond_ent, e->bond_par, e->ang_ptr, e->ang_ent, e->ang_bead, e->ang_par,
+                e->has_bonds, e->has_angles};
+}
+
+void set_rng_step(dpd_engine* e, int64_t step)
+{
+    e->B.step0 = (uint32_t)(uint64_t)step;
+    e->B.step1 = (uint32_t)((uint64_t)step >> 32);
+}
+
+inline int grid(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// counting sort of buffer `src` into buffer `src^1`; MODE decides the move
+template <int MODE>
+int sort_step(dpd_engine* e, bool obs)
+{
+    const int n = (int)e->n;
+    const int src = e->cur, dst = e->cur ^ 1;
+    Buffers S = bufs(e, src), D = bufs(e, dst);
+    Topo T = topo(e);
+    if (obs)
+        hipLaunchKernelGGL((k_advance<MODE, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
+                           e->slot_of, T, e->B, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->acc);
+    else
+        hipLaunchKernelGGL((k_advance<MODE, false>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
+                           e->slot_of, T, e->B, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->acc);
+    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, e->stream, e->nbricks, e->bcnt, e->bstart);
+    hipLaunchKernelGGL(k_cell_start, dim3(grid(e->nbricks, 4)), dim3(256), 0, e->stream, e->nbricks, n,
+                       e->bstart, e->cnt, e->cstart);
+    hipLaunchKernelGGL((k_permute<MODE>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, D, e->frc, e->vp,
+                       e->B, e->nkey, e->rank, e->cstart, e->slot_of, e->frc);
+    e->cur = dst;
+    HIPCHK(hipGetLastError());
+    return DPD_OK;
+}
+
+int launch_pair(dpd_engine* e, bool obs)
+{
+    PairArgs a;
+    a.n = (int)e->n;
+    a.pos = e->pos[e->cur];
+    a.vp = e->vp;
+    a.key = e->key[e->cur];
+    a.cstart = e->cstart;
+    a.table = e->table;
+    a.frc = e->frc;
+    a.acc = e->acc + O_VIR;
+    a.nbricks = e->nbricks;
+    return launch_pair_kernel(e->pair_kernel, a, e->B, obs, e->stream) == hipSuccess
+               ? DPD_OK
+               : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
+}
+
+int read_flags(dpd_engine* e)
+{
+    int h[4];
+    HIPCHK(hipMemcpyAsync(h, e->flags, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->jumps += h[1];
+    if (h[0]) return fail(e, DPD_ERR_NAN, "non-finite force, velocity or position detected");
+    return DPD_OK;
+}
+
+int zero_acc(dpd_engine* e)
+{
+    hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, e->stream, e->acc, 32);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, e->stream, e->flags, 4);
+    HIPCHK(hipGetLastError());
+    return DPD_OK;
+}
+
+int finish_obs(dpd_engine* e, bool update_v)
+{
+    (void)update_v;
+    double h[32];
+    HIPCHK(hipMemcpyAsync(h, e->acc, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    dpd_obs& o = e->obs;
+    std::memset(&o, 0, sizeof(o));
+    o.step = e->step;
+    o.nbeads = e->n;
+    const double V = (double)e->p.nx * e->p.wx * (double)e->p.ny * e->p.wy * (double)e->p.nz * e->p.wz;
+    for (int d = 0; d < 3; ++d) { o.sum_v2[d] = h[O_V2 + d]; o.momentum[d] = h[O_MOM + d]; }
+    for (int i = 0; i < 9; ++i) o.stress[i] = h[O_VIR + i] / V;
+    const double v2 = o.sum_v2[0] + o.sum_v2[1] + o.sum_v2[2];
+    o.kT = v2 / (3.0 * (double)e->n);
+    // src/Monitor.cpp:981-982 keeps the 10-digit constant
+    o.pressure = 0.3333333333 * (1.0 / V) * (v2 + h[O_VIR + 0] + h[O_VIR + 4] + h[O_VIR + 8]);
+    o.pe_pair = h[O_PEP];
+    o.pe_bond = h[O_PEB];
+    o.pe_angle = h[O_PEA];
+    o.jumps = e->jumps;
+    e->obs_valid = true;
+    return DPD_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI ------
+
+extern "C" {
+
+const char* dpd_last_error(const dpd_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int dpd_create(const dpd_params* p, dpd_engine** out)
+{
+    if (!out) return DPD_ERR_PARAM;
+    *out = nullptr;
+    if (!p) return DPD_ERR_PARAM;
+    dpd_engine* e = new dpd_engine();
+    *out = e;
+    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(e, DPD_ERR_PARAM, "need at least 3 cells per axis");
+    if (p->wx < 1.0 || p->wy < 1.0 || p->wz < 1.0) return fail(e, DPD_ERR_PARAM, "cell widths must be >= 1");
+    if (p->ntypes < 1 || p->ntypes > DPD_MAX_TYPES) return fail(e, DPD_ERR_PARAM, "ntypes out of range");
+    if (!(p->dt > 0.0) || p->lambda < 0.0 || p->lambda > 1.0 || p->kT < 0.0)
+        return fail(e, DPD_ERR_PARAM, "invalid dt, lambda or kT");
+    if (!p->a || !p->gamma) return fail(e, DPD_ERR_PARAM, "null interaction matrices");
+    e->p = *p;
+    const int nt = p->ntypes;
+    e->a_h.assign(p->a, p->a + nt * nt);
+    e->g_h.assign(p->gamma, p->gamma + nt * nt);
+    for (int i = 0; i < nt; ++i)
+        for (int j = 0; j < nt; ++j)
+            if (e->a_h[i * nt + j] != e->a_h[j * nt + i] || e->g_h[i * nt + j] != e->g_h[j * nt + i] ||
+                e->g_h[i * nt + j] < 0.0)
+                return fail(e, DPD_ERR_PARAM, "interaction matrices must be symmetric, gamma >= 0");
+    e->p.a = e->a_h.data();
+    e->p.gamma = e->g_h.data();
+    e->device = p->device;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (auto& ev : e->ev) HIPCHK(hipEventCreate(&ev));
+    e->dt = p->dt; e->lambda = p->lambda; e->kT = p->kT; e->seed = p->seed;
+    BoxDev& B = e->B;
+    B.n[0] = p->nx; B.n[1] = p->ny; B.n[2] = p->nz;
+    for (int d = 0; d < 3; ++d) B.nb[d] = (B.n[d] + kBrick - 1) / kBrick;
+    B.wd[0] = p->wx; B.wd[1] = p->wy; B.wd[2] = p->wz;
+    for (int d = 0; d < 3; ++d) B.w[d] = (float)B.wd[d];
+    B.ntypes = nt;
+    B.seed0 = (uint32_t)p->seed;
+    B.seed1 = (uint32_t)(p->seed >> 32);
+    set_step_constants(e);
+    e->nbricks = B.nb[0] * B.nb[1] * B.nb[2];
+    e->nkeys = e->nbricks * kBrickCells;
+    e->pair_kernel = (p->pair_kernel == DPD_PAIR_AUTO) ? default_pair_kernel() : p->pair_kernel;
+    if (e->pair_kernel != DPD_PAIR_BRICK && e->pair_kernel != DPD_PAIR_FULLSHELL)
+        return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
+    int rc;
+    if ((rc = dalloc(e, &e->cnt, e->nkeys)) || (rc = dalloc(e, &e->cstart, e->nkeys + 1)) ||
+        (rc = dalloc(e, &e->bcnt, e->nbricks)) || (rc = dalloc(e, &e->bstart, e->nbricks)) ||
+        (rc = dalloc(e, &e->flags, 4)) || (rc = dalloc(e, &e->acc, 32)) || (rc = dalloc(e, &e->table, 3 * nt * nt)))
+        return rc;
+    HIPCHK(hipMemsetAsync(e->cnt, 0, sizeof(int) * e->nkeys, e->stream));
+    HIPCHK(hipMemsetAsync(e->bcnt, 0, sizeof(int) * e->nbricks, e->stream));
+    HIPCHK(hipMemsetAsync(e->flags, 0, sizeof(int) * 4, e->stream));
+    HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(double) * 32, e->stream));
+    if ((rc = upload_table(e))) return rc;
+    e->err.clear();
+    return DPD_OK;
+}
+
+void dpd_destroy(dpd_engine* e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->aux[0], e->aux[1],
+                    e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
+                    e->bstart, e->cstart, e->flags, e->acc, e->table, e->gather, e->bond_ptr, e->ang_ptr,
+                    e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (auto& ev : e->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : e->pev) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int dpd_upload_beads(dpd_engine* e, int64_t n, const int32_t* id, const int32_t* type, const double* x,
+                     const double* unpbc, const double* v, const double* f, int64_t step0)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (n < 1 || n > (int64_t)1 << 30) return fail(e, DPD_ERR_PARAM, "bead count out of range");
+    if (!id || !type || !x || !v) return fail(e, DPD_ERR_PARAM, "null bead array");
+    HIPCHK(hipSetDevice(e->device));
+    const BoxDev& B = e->B;
+    const double L[3] = {B.n[0] * B.wd[0], B.n[1] * B.wd[1], B.n[2] * B.wd[2]};
+    std::vector<float4> hp(n), hv(n), hf(n);
+    std::vector<int4> ha(n);
+    std::vector<int> hk(n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (type[i] < 0 || type[i] >= e->p.ntypes) return fail(e, DPD_ERR_PARAM, "bead type out of range");
+        int c[3], im[3];
+        float o[3];
+        for (int d = 0; d < 3; ++d) {
+            const double xd = x[3 * i + d];
+            if (!(xd >= 0.0 && xd < L[d]))
+                return fail(e, DPD_ERR_OUT_OF_BOX, "bead " + std::to_string(i) + " outside the box");
+            int ci = (int)std::floor(xd / B.wd[d]);
+            ci = std::min(std::max(ci, 0), B.n[d] - 1);
+            double od = xd - ci * B.wd[d];
+            float of = (float)od;
+            if (of >= B.w[d]) of = B.w[d] * 0.99999994f;
+            if (of < 0.f) of = 0.f;
+            c[d] = ci;
+            o[d] = of;
+            const double ud = unpbc ? unpbc[3 * i + d] : xd;
+            im[d] = (int)std::lround((ud - xd) / L[d]);
+        }
+        float tb;
+        int32_t ty = type[i];
+        std::memcpy(&tb, &ty, 4);
+        float ib;
+        int32_t idv = id[i];
+        std::memcpy(&ib, &idv, 4);
+        hp[i] = make_float4(o[0], o[1], o[2], tb);
+        hv[i] = make_float4((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2], ib);
+        hf[i] = f ? make_float4((float)f[3 * i], (float)f[3 * i + 1], (float)f[3 * i + 2], 0.f)
+                  : make_float4(0.f, 0.f, 0.f, 0.f);
+        ha[i] = make_int4(im[0], im[1], im[2], (int)i);
+        hk[i] = cell_key(c[0], c[1], c[2], B);
+    }
+    int rc;
+    if (n != e->n) {
+        for (int b = 0; b < 2; ++b)
+            if ((rc = dalloc(e, &e->pos[b], n)) || (rc = dalloc(e, &e->vel[b], n)) ||
+                (rc = dalloc(e, &e->fold[b], n)) || (rc = dalloc(e, &e->aux[b], n)) ||
+                (rc = dalloc(e, &e->key[b], n)))
+                return rc;
+        if ((rc = dalloc(e, &e->frc, n)) || (rc = dalloc(e, &e->vp, n)) || (rc = dalloc(e, &e->slot_of, n)) ||
+            (rc = dalloc(e, &e->nkey, n)) || (rc = dalloc(e, &e->rank, n)) || (rc = dalloc(e, &e->gather, 12 * n)))
+            return rc;
+        e->n = n;
+        // topology refers to bead indices: a new bead count invalidates it
+        e->has_bonds = e->has_angles = 0;
+    }
+    const int src = e->cur;
+    HIPCHK(hipMemcpyAsync(e->pos[src], hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->vel[src], hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->fold[src], hf.data(), sizeof(float4) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->aux[src], ha.data(), sizeof(int4) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->key[src], hk.data(), sizeof(int) * n, hipMemcpyHostToDevice, e->stream));
+    if ((rc = zero_acc(e))) return rc;
+    if ((rc = sort_step<kRebin>(e, false))) return rc;
+    e->step = step0;
+    e->jumps = 0;
+    e->obs_valid = false;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DPD_OK;
+}
+
+int dpd_upload_topology(dpd_engine* e, int64_t nbonds, const int32_t* head, const int32_t* tail, const double* k,
+                        const double* l0, int64_t npairs, const int32_t* first_bond, const int32_t* second_bond,
+                        const double* modulus, const double* phi0)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (e->n == 0) return fail(e, DPD_ERR_STATE, "upload beads before topology");
+    if (nbonds < 0 || npairs < 0) return fail(e, DPD_ERR_PARAM, "negative counts");
+    if (nbonds && (!head || !tail || !k || !l0)) return fail(e, DPD_ERR_PARAM, "null bond array");
+    if (npairs && (!first_bond || !second_bond || !modulus || !phi0)) return fail(e, DPD_ERR_PARAM, "null pair array");
+    HIPCHK(hipSetDevice(e->device));
+    const int64_t n = e->n;
+    for (int64_t b = 0; b < nbonds; ++b)
+        if (head[b] < 0 || head[b] >= n || tail[b] < 0 || tail[b] >= n || head[b] == tail[b])
+            return fail(e, DPD_ERR_PARAM, "bond bead index out of range");
+    for (int64_t q = 0; q < npairs; ++q) {
+        if (first_bond[q] < 0 || first_bond[q] >= nbonds || second_bond[q] < 0 || second_bond[q] >= nbonds)
+            return fail(e, DPD_ERR_PARAM, "bond pair index out of range");
+        if (tail[second_bond[q]] != head[first_bond[q]])
+            return fail(e, DPD_ERR_PARAM, "bond pair bonds do not share a bead");
+    }
+    // CSR by bead
+    std::vector<int> bp(n + 1, 0), ap(n + 1, 0);
+    for (int64_t b = 0; b < nbonds; ++b) { bp[head[b] + 1]++; bp[tail[b] + 1]++; }
+    std::vector<int4> abead(std::max<int64_t>(npairs, 1));
+    std::vector<double4> apar(std::max<int64_t>(npairs, 1));
+    for (int64_t q = 0; q < npairs; ++q) {
+        const int fb = first_bond[q], sb = second_bond[q];
+        const int a = head[sb], bm = head[fb], c = tail[fb];
+        abead[q] = make_int4(a, bm, c, 0);
+        apar[q] = make_double4(modulus[q], phi0[q], std::cos(phi0[q]), std::sin(phi0[q]));
+        ap[a + 1]++; ap[bm + 1]++; ap[c + 1]++;
+    }
+    for (int64_t i = 0; i < n; ++i) { bp[i + 1] += bp[i]; ap[i + 1] += ap[i]; }
+    std::vector<int4> bent(std::max<int>(bp[n], 1));
+    std::vector<double2> bpar(std::max<int>(bp[n], 1));
+    std::vector<int2> aent(std::max<int>(ap[n], 1));
+    {
+        std::vector<int> fill(bp.begin(), bp.end() - 1);
+        for (int64_t b = 0; b < nbonds; ++b) {
+            const int h = head[b], t = tail[b];
+            bent[fill[h]] = make_int4(t, 1, 0, 0); bpar[fill[h]++] = make_double2(k[b], l0[b]);
+            bent[fill[t]] = make_int4(h, 0, 0, 0); bpar[fill[t]++] = make_double2(k[b], l0[b]);
+        }
+        std::vector<int> afill(ap.begin(), ap.end() - 1);
+        for (int64_t q = 0; q < npairs; ++q) {
+            const int4 abc = abead[q];
+            aent[afill[abc.z]++] = make_int2((int)q, 0);
+            aent[afill[abc.y]++] = make_int2((int)q, 1);
+            aent[afill[abc.x]++] = make_int2((int)q, 2);
+        }
+    }
+    int rc;
+    if ((rc = dalloc(e, &e->bond_ptr, n + 1)) || (rc = dalloc(e, &e->ang_ptr, n + 1)) ||
+        (rc = dalloc(e, &e->bond_ent, bent.size())) || (rc = dalloc(e, &e->bond_par, bpar.size())) ||
+        (rc = dalloc(e, &e->ang_ent, aent.size())) || (rc = dalloc(e, &e->ang_bead, abead.size())) ||
+        (rc = dalloc(e, &e->ang_par, apar.size())))
+        return rc;
+    HIPCHK(hipMemcpyAsync(e->bond_ptr, bp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->ang_ptr, ap.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->bond_ent, bent.data(), sizeof(int4) * bent.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->bond_par, bpar.data(), sizeof(double2) * bpar.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->ang_ent, aent.data(), sizeof(int2) * aent.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->ang_bead, abead.data(), sizeof(int4) * abead.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->ang_par, apar.data(), sizeof(double4) * apar.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->has_bonds = nbonds > 0;
+    e->has_angles = npairs > 0;
+    e->nbonds = nbonds;
+    e->nangles = npairs;
+    return DPD_OK;
+}
+
+int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (e->n == 0) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (nsteps < 0) return fail(e, DPD_ERR_PARAM, "negative step count");
+    if (nsteps == 0) return DPD_OK;
+    HIPCHK(hipSetDevice(e->device));
+    const bool obs = flags & DPD_STEP_OBSERVE;
+    const bool timing = flags & 4u;
+    int rc;
+    if ((rc = zero_acc(e))) return rc;
+    e->jumps = 0;
+    if (timing) {
+        while ((int64_t)e->pev.size() < 2 * nsteps) {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            e->pev.push_back(ev);
+        }
+        HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    }
+    for (int64_t t = 0; t < nsteps; ++t) {
+        e->step += 1;
+        set_rng_step(e, e->step);
+        rc = (t == 0) ? sort_step<kBegin>(e, false) : sort_step<kEndBegin>(e, false);
+        if (rc) return rc;
+        if (timing) HIPCHK(hipEventRecord(e->pev[2 * t], e->stream));
+        if ((rc = launch_pair(e, obs && t == nsteps - 1))) return rc;
+        if (timing) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
+    }
+    const int n = (int)e->n;
+    if (obs)
+        hipLaunchKernelGGL((k_finish<true, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur),
+                           e->frc, e->slot_of, topo(e), e->B, e->flags, e->acc);
+    else
+        hipLaunchKernelGGL((k_finish<true, false>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur),
+                           e->frc, e->slot_of, topo(e), e->B, e->flags, e->acc);
+    HIPCHK(hipGetLastError());
+    if (timing) HIPCHK(hipEventRecord(e->ev[1], e->stream));
+    if ((rc = read_flags(e))) return rc;
+    if (timing) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+        e->t_total = ms;
+        double tp = 0.0;
+        for (int64_t t = 0; t < nsteps; ++t) {
+            HIPCHK(hipEventElapsedTime(&ms, e->pev[2 * t], e->pev[2 * t + 1]));
+            tp += ms;
+        }
+        e->t_pair = tp;
+        e->n_pair = nsteps;
+    }
+    if (obs) return finish_obs(e, true);
+    e->obs_valid = false;
+    return DPD_OK;
+}
+
+int dpd_compute_forces(dpd_engine* e, int64_t step, uint32_t flags, double* f_out, double* virial, double* pe)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (e->n == 0) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    HIPCHK(hipSetDevice(e->device));
+    int rc;
+    const bool thermo_saved = e->thermostat_on;
+    if (flags & DPD_NO_THERMOSTAT) {
+        e->thermostat_on = false;
+        if ((rc = upload_table(e))) return rc;
+    }
+    if ((rc = zero_acc(e))) return rc;
+    set_rng_step(e, step);
+    // the uploaded state is sorted with vp = v: evaluate at it
+    if ((rc = launch_pair(e, true))) return rc;
+    const int n = (int)e->n;
+    hipLaunchKernelGGL((k_finish<false, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur),
+                       e->frc, e->slot_of, topo(e), e->B, e->flags, e->acc);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_flags(e))) return rc;
+    double h[32];
+    HIPCHK(hipMemcpyAsync(h, e->acc, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    if (f_out) {
+        double* g = e->gather;
+        hipLaunchKernelGGL(k_gather, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->frc, e->B,
+                           g, g + 3 * n, g + 6 * n, g + 9 * n);
+        HIPCHK(hipMemcpyAsync(f_out, g + 9 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (virial)
+        for (int i = 0; i < 9; ++i) virial[i] = h[O_VIR + i];
+    if (pe) { pe[0] = h[O_PEP]; pe[1] = h[O_PEB]; pe[2] = h[O_PEA]; }
+    if (flags & DPD_NO_THERMOSTAT) {
+        e->thermostat_on = thermo_saved;
+        if ((rc = upload_table(e))) return rc;
+    }
+    return DPD_OK;
+}
+
+int dpd_observables(dpd_engine* e, dpd_obs* out)
+{
+    if (!e || !out) return DPD_ERR_PARAM;
+    if (!e->obs_valid) return fail(e, DPD_ERR_STATE, "no observables: call dpd_step with DPD_STEP_OBSERVE");
+    *out = e->obs;
+    return DPD_OK;
+}
+
+int dpd_download(dpd_engine* e, double* x, double* unpbc, double* v, double* f)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (e->n == 0) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    HIPCHK(hipSetDevice(e->device));
+    const int n = (int)e->n;
+    double* g = e->gather;
+    hipLaunchKernelGGL(k_gather, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->frc, e->B, g,
+                       g + 3 * n, g + 6 * n, g + 9 * n);
+    HIPCHK(hipGetLastError());
+    const size_t b3 = sizeof(double) * 3 * n;
+    if (x) HIPCHK(hipMemcpyAsync(x, g, b3, hipMemcpyDeviceToHost, e->stream));
+    if (unpbc) HIPCHK(hipMemcpyAsync(unpbc, g + 3 * n, b3, hipMemcpyDeviceToHost, e->stream));
+    if (v) HIPCHK(hipMemcpyAsync(v, g + 6 * n, b3, hipMemcpyDeviceToHost, e->stream));
+    if (f) HIPCHK(hipMemcpyAsync(f, g + 9 * n, b3, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DPD_OK;
+}
+
+int dpd_set_interactions(dpd_engine* e, const double* a, const double* gamma)
+{
+    if (!e) return DPD_ERR_PARAM;
+    const int nt = e->p.ntypes;
+    if (a) e->a_h.assign(a, a + nt * nt);
+    if (gamma) e->g_h.assign(gamma, gamma + nt * nt);
+    return upload_table(e);
+}
+
+int dpd_set_timestep(dpd_engine* e, double dt)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (!(dt > 0.0)) return fail(e, DPD_ERR_PARAM, "dt must be positive");
+    e->dt = dt;
+    set_step_constants(e);
+    return DPD_OK;
+}
+
+int dpd_toggle(dpd_engine* e, int32_t what, int32_t on)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (what == DPD_TOGGLE_THERMOSTAT) e->thermostat_on = on != 0;
+    else if (what == DPD_TOGGLE_CONSERVATIVE) e->conservative_on = on != 0;
+    else return fail(e, DPD_ERR_PARAM, "unknown toggle");
+    return upload_table(e);
+}
+
+int64_t dpd_last_timing(dpd_engine* e, double* ms3)
+{
+    if (!e) return -1;
+    if (ms3) { ms3[0] = e->t_total; ms3[1] = e->t_pair; ms3[2] = e->t_total - e->t_pair; }
+    return e->n_pair;
+}
+
+int64_t dpd_get_step(const dpd_engine* e) { return e ? e->step : -1; }
+
+}  // extern "C"

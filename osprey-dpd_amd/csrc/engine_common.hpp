@@ -1,0 +1,155 @@
+// engine_common.hpp -- device-side building blocks shared by the DPD step kernels.
+//
+// Layout in HBM (one box): beads are sorted by a cell key every step.  The key orders
+// cells brick-major: the box is tiled into 4x4x4-cell bricks (partial bricks at the
+// far faces), key = brick * 64 + local cell (x fastest).  A brick's beads are one
+// contiguous slot range, which is what the LDS pair kernel stages.
+//
+// Positions are stored as FP32 offsets inside the bead's cell, o in [0, w), plus the
+// integer cell (from the key) and integer periodic image counts.  Differences of
+// positions are formed as (o_i - o_j) + (c_i - c_j) * w with exact integer cell
+// deltas, so FP32 keeps ~1e-7 absolute precision at any box size (absolute FP32
+// coordinates would lose 7.6e-6 at L = 64).  The absolute FP64 position c*w + o and
+// the unwrapped position (c + img*n)*w + o are formed only on download and in the
+// FP64 bonded terms.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dpd {
+
+constexpr int kBrick = 4;        // cells per brick edge
+constexpr int kBrickCells = 64;  // cells per brick
+constexpr int kMaxTypes = 16;
+
+// Constants of one box, passed by value (kernarg).  Mirrors the CCNTCell statics set by
+// SetSimBoxLengths / SetTimeStepConstants (src/CNTCell.cpp:143-203).
+struct BoxDev {
+    int n[3];          // cells per axis
+    int nb[3];         // bricks per axis
+    float w[3];        // cell widths
+    double wd[3];
+    float dt, hdt, hdt2, lamdt;   // dt, dt/2, dt^2/2, lambda*dt
+    float invrootdt;              // sqrt(24 kT / dt)
+    int ntypes;
+    uint32_t seed0, seed1;        // key of the pair stream
+    uint32_t step0, step1;        // simulation time of the force evaluation
+};
+
+__host__ __device__ inline int cell_key(int cx, int cy, int cz, const BoxDev& B)
+{
+    const int brick = (cx >> 2) + B.nb[0] * ((cy >> 2) + B.nb[1] * (cz >> 2));
+    return brick * kBrickCells + (((cz & 3) * 4 + (cy & 3)) * 4 + (cx & 3));
+}
+
+__host__ __device__ inline void key_cell(int key, const BoxDev& B, int c[3])
+{
+    const int brick = key >> 6, l = key & 63;
+    const int bx = brick % B.nb[0];
+    const int t = brick / B.nb[0];
+    const int by = t % B.nb[1];
+    const int bz = t / B.nb[1];
+    c[0] = bx * 4 + (l & 3);
+    c[1] = by * 4 + ((l >> 2) & 3);
+    c[2] = bz * 4 + (l >> 4);
+}
+
+__host__ __device__ inline int floordiv(int a, int n)
+{
+    return (a >= 0) ? a / n : -((-a + n - 1) / n);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).  Pair uniform: counter (min id, max id, step),
+// key = seed; U = (word0 >> 8) * 2^-24, exactly representable in FP32.
+__device__ inline uint32_t philox_word0(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                        uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    return c0;
+}
+
+__device__ inline float pair_uniform(uint32_t ida, uint32_t idb, const BoxDev& B)
+{
+    const uint32_t lo = ida < idb ? ida : idb;
+    const uint32_t hi = ida < idb ? idb : ida;
+    const uint32_t r = philox_word0(lo, hi, B.step0, B.step1, B.seed0, B.seed1);
+    return (float)(r >> 8) * 0x1p-24f;
+}
+
+// Type-pair table in device memory: [0, nt^2) a_ij, [nt^2, 2nt^2) gamma_ij,
+// [2nt^2, 3nt^2) sqrt(gamma_ij).
+struct PairTable {
+    const float* t;
+};
+
+// DPD pair force on bead i from bead j (src/CNTCell.cpp:1191-1225):
+//   F = [a w + (-gamma w^2 (r^.v)) + sqrt(gamma) w sqrt(24kT/dt) (0.5 - U)] r^,  w = 1 - r
+// returns the scalar F/r so that F_vec = s * dx.
+__device__ inline float pair_scalar(float dr2, float dxdv, float a, float sg, float g,
+                                    uint32_t idi, uint32_t idj, const BoxDev& B, float* pe)
+{
+    const float dr = sqrtf(dr2);
+    const float inv = 1.0f / dr;
+    const float wr = 1.0f - dr;
+    const float con = a * wr;
+    const float diss = -(g * wr * wr) * (dxdv * inv);
+    const float u = pair_uniform(idi, idj, B);
+    const float rnd = sg * wr * B.invrootdt * (0.5f - u);
+    if (pe) *pe = 0.5f * a * wr * wr;
+    return (con + diss + rnd) * inv;
+}
+
+// Move a bead by displacement d from offset o in cell c (image img): returns the new
+// offset in [0, w), cell and image; sets *jump when it crossed more than one cell.
+struct Moved {
+    float o[3];
+    int c[3];
+    int img[3];
+    int jump;
+    int bad;
+};
+
+__device__ inline void advance(const float o[3], const int c[3], const int img[3],
+                               const float d[3], const BoxDev& B, Moved& m)
+{
+    m.jump = 0;
+    m.bad = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float x = o[a] + d[a];
+        int s = 0;
+        if (!(fabsf(x) < 1.0e6f)) { m.bad = 1; x = 0.0f; }
+        s = (int)floorf(x / B.w[a]);
+        x = x - (float)s * B.w[a];
+        if (x < 0.0f) { x += B.w[a]; s -= 1; }
+        if (x >= B.w[a]) x = B.w[a] * 0.99999994f;
+        if (x < 0.0f) x = 0.0f;
+        m.jump |= (s > 1 || s < -1);
+        int cc = c[a] + s;
+        const int q = floordiv(cc, B.n[a]);
+        cc -= q * B.n[a];
+        m.o[a] = x;
+        m.c[a] = cc;
+        m.img[a] = img[a] + q;
+    }
+}
+
+// wave-level helpers (64 lanes)
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+}  // namespace dpd

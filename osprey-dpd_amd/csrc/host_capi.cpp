@@ -1,0 +1,511 @@
+// host_capi.cpp -- C ABI of the host layer (include/dpd_host.h): dmpci input, starting
+// states, restart snapshots and the CSimBox::Run-shaped driver over the engine.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "dmpci.hpp"
+#include "dpd_engine.h"
+#include "dpd_host.h"
+
+using dpd::host::InputData;
+using dpd::host::System;
+
+struct dpd_input {
+    InputData d;
+    System s;
+    std::vector<double> a, g;
+};
+
+namespace {
+
+void put_err(char* err, int32_t len, const std::string& msg)
+{
+    if (err && len > 0) {
+        std::strncpy(err, msg.c_str(), (size_t)len - 1);
+        err[len - 1] = 0;
+    }
+}
+
+int finish_input(dpd_input* in, dpd_input** out, char* err, int32_t errlen)
+{
+    std::string e;
+    if (!dpd::host::build_system(in->d, in->s, e)) {
+        put_err(err, errlen, e);
+        delete in;
+        return DPD_ERR_IO;
+    }
+    in->a = in->d.a;
+    in->g = in->d.g;
+    *out = in;
+    return DPD_OK;
+}
+
+double box_len(const InputData& d, int a)
+{
+    return a == 0 ? d.nx * d.wx : (a == 1 ? d.ny * d.wy : d.nz * d.wz);
+}
+
+// Maxwell velocities at kT: Gaussian components, zero total momentum, exact kT
+// (the reference rescales to the target temperature too, src/RandomBuilder.cpp:315-360)
+void maxwell(std::mt19937_64& rng, double kT, size_t n, double* v)
+{
+    std::normal_distribution<double> nd(0.0, std::sqrt(kT > 0 ? kT : 1.0));
+    double m[3] = {0, 0, 0};
+    for (size_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) { v[3 * i + d] = nd(rng); m[d] += v[3 * i + d]; }
+    double s2 = 0.0;
+    for (size_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) { v[3 * i + d] -= m[d] / (double)n; s2 += v[3 * i + d] * v[3 * i + d]; }
+    const double scale = (kT > 0 && s2 > 0) ? std::sqrt(3.0 * (double)n * kT / s2) : 0.0;
+    for (size_t i = 0; i < 3 * n; ++i) v[i] *= scale;
+}
+
+// parent bead of each polymer bead (the head of the bond whose tail it is), -1 for roots
+std::vector<int64_t> parents(const System& s)
+{
+    std::vector<int64_t> p(s.id.size(), -1);
+    for (size_t b = 0; b < s.bond_head.size(); ++b)
+        if (p[s.bond_tail[b]] < 0 && s.bond_head[b] < s.bond_tail[b]) p[s.bond_tail[b]] = s.bond_head[b];
+    return p;
+}
+
+void wrap(const InputData& d, size_t n, const double* u, double* x)
+{
+    for (size_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const double L = box_len(d, a);
+            double y = u[3 * i + a] - L * std::floor(u[3 * i + a] / L);
+            if (y >= L) y = 0.0;
+            x[3 * i + a] = y;
+        }
+}
+
+// random state: free beads uniform; polymer beads a random unit vector times the bond
+// length from their parent (a random walk along the polymer)
+void build_random(const dpd_input* in, std::mt19937_64& rng, double* u)
+{
+    const InputData& d = in->d;
+    const System& s = in->s;
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::normal_distribution<double> N(0.0, 1.0);
+    const std::vector<int64_t> par = parents(s);
+    std::vector<double> blen(s.id.size(), 0.5);
+    for (size_t b = 0; b < s.bond_head.size(); ++b)
+        if (par[s.bond_tail[b]] == s.bond_head[b]) blen[s.bond_tail[b]] = s.bond_l0[b] > 0 ? s.bond_l0[b] : 0.5;
+    for (size_t i = 0; i < s.id.size(); ++i) {
+        if (par[i] < 0) {
+            for (int a = 0; a < 3; ++a) u[3 * i + a] = U(rng) * box_len(d, a);
+        } else {
+            double r[3], nn = 0;
+            for (int a = 0; a < 3; ++a) { r[a] = N(rng); nn += r[a] * r[a]; }
+            nn = std::sqrt(nn);
+            for (int a = 0; a < 3; ++a) u[3 * i + a] = u[3 * par[i] + a] + blen[i] * r[a] / nn;
+        }
+    }
+}
+
+// lamella: the named polymer in two leaflets about the mid-plane normal to `Normal`
+// (src/Builder.cpp:842, isBilayer::Assemble), heads outwards, each bead placed by its
+// bond depth from the head; other beads uniform outside the slab.
+bool build_lamella(const dpd_input* in, std::mt19937_64& rng, double* u, std::string& err)
+{
+    const InputData& d = in->d;
+    const System& s = in->s;
+    std::vector<std::string> pol = d.state_values("Polymer", 1);
+    std::vector<std::string> nrm = d.state_values("Normal", 3);
+    std::vector<std::string> cen = d.state_values("Centre", 1);
+    std::vector<std::string> thk = d.state_values("Thickness", 1);
+    std::vector<std::string> upf = d.state_values("UpperFraction", 1);
+    if (pol.empty() || nrm.size() != 3 || cen.empty() || thk.empty()) {
+        err = "lamella state needs Polymer, Normal, Centre and Thickness";
+        return false;
+    }
+    int axis = -1;
+    for (int a = 0; a < 3; ++a)
+        if (std::atof(nrm[a].c_str()) != 0.0) axis = a;
+    int ptype = -1;
+    for (size_t p = 0; p < d.polymers.size(); ++p)
+        if (d.polymers[p].name == pol[0]) ptype = (int)p;
+    if (axis < 0 || ptype < 0) {
+        err = "lamella polymer or normal not found";
+        return false;
+    }
+    const double L = box_len(d, axis);
+    const double zc = std::atof(cen[0].c_str()) * L;
+    const double half = 0.5 * std::atof(thk[0].c_str());
+    const double upper = upf.empty() ? 0.5 : std::atof(upf[0].c_str());
+    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    // depth of each bead of the polymer template (bonds from bead 0)
+    const auto& tmpl = d.polymers[ptype];
+    std::vector<int> depth(tmpl.bead_type.size(), 0);
+    for (size_t b = 0; b < tmpl.bond_head.size(); ++b) depth[tmpl.bond_tail[b]] = depth[tmpl.bond_head[b]] + 1;
+    const int maxd = std::max(1, *std::max_element(depth.begin(), depth.end()));
+    const int64_t nlip = s.polymers_per_type[ptype];
+    const int64_t nup = (int64_t)std::llround(upper * (double)nlip);
+    // lipids on a jittered square grid per leaflet
+    auto grid_side = [](int64_t m) { return std::max<int64_t>(1, (int64_t)std::ceil(std::sqrt((double)m))); };
+    int64_t lip = 0;
+    for (size_t i = 0; i < s.id.size();) {
+        const int pt = s.poly_type[i];
+        size_t j = i;
+        while (j < s.id.size() && s.poly_id[j] == s.poly_id[i]) ++j;
+        if (pt == ptype) {
+            const bool up = lip < nup;
+            const int64_t k = up ? lip : lip - nup;
+            const int64_t m = up ? nup : nlip - nup;
+            const int64_t side = grid_side(m);
+            const double p1 = ((double)(k % side) + 0.5 + 0.3 * (U(rng) - 0.5)) * box_len(d, a1) / side;
+            const double p2 = ((double)(k / side) + 0.5 + 0.3 * (U(rng) - 0.5)) * box_len(d, a2) / side;
+            for (size_t q = i; q < j; ++q) {
+                const int dep = depth[q - i];
+                const double off = half * (1.0 - (double)dep / (double)maxd) * 0.95;
+                u[3 * q + axis] = up ? zc + off + 0.02 : zc - off - 0.02;
+                u[3 * q + a1] = p1 + 0.25 * (U(rng) - 0.5);
+                u[3 * q + a2] = p2 + 0.25 * (U(rng) - 0.5);
+            }
+            ++lip;
+        } else {
+            for (size_t q = i; q < j; ++q) {
+                // outside the slab: draw the normal coordinate from the complement
+                const double out = L - 2.0 * half;
+                double z = U(rng) * std::max(out, 1e-3);
+                z = zc + half + z;
+                u[3 * q + axis] = z;
+                u[3 * q + a1] = U(rng) * box_len(d, a1);
+                u[3 * q + a2] = U(rng) * box_len(d, a2);
+            }
+        }
+        i = j;
+    }
+    return true;
+}
+
+std::string fmt6(double v)
+{
+    std::ostringstream o;
+    o << v;   // default ostream precision (6 significant digits), as the reference
+    return o.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpd_input_parse(const char* text, dpd_input** out, char* err, int32_t errlen)
+{
+    if (!out || !text) return DPD_ERR_PARAM;
+    *out = nullptr;
+    dpd_input* in = new dpd_input();
+    std::string e;
+    if (!dpd::host::read_dmpci(text, in->d, e)) {
+        put_err(err, errlen, e);
+        delete in;
+        return DPD_ERR_IO;
+    }
+    return finish_input(in, out, err, errlen);
+}
+
+int dpd_input_read(const char* path, dpd_input** out, char* err, int32_t errlen)
+{
+    if (!out || !path) return DPD_ERR_PARAM;
+    *out = nullptr;
+    dpd_input* in = new dpd_input();
+    std::string e;
+    if (!dpd::host::read_dmpci_file(path, in->d, e)) {
+        put_err(err, errlen, e);
+        delete in;
+        return DPD_ERR_IO;
+    }
+    return finish_input(in, out, err, errlen);
+}
+
+void dpd_input_free(dpd_input* in) { delete in; }
+
+int dpd_input_counts(const dpd_input* in, dpd_counts* o)
+{
+    if (!in || !o) return DPD_ERR_PARAM;
+    const InputData& d = in->d;
+    std::memset(o, 0, sizeof(*o));
+    o->ntypes = d.ntypes();
+    o->nbeads = (int64_t)in->s.id.size();
+    o->nbonds = (int64_t)in->s.bond_head.size();
+    o->npairs = (int64_t)in->s.pair_first.size();
+    o->npolymers = in->s.npolymers;
+    o->npolymer_types = (int32_t)d.polymers.size();
+    o->nx = d.nx; o->ny = d.ny; o->nz = d.nz;
+    o->wx = d.wx; o->wy = d.wy; o->wz = d.wz;
+    o->density = d.density; o->kT = d.kT; o->lambda = d.lambda; o->dt = d.dt;
+    o->seed = d.seed;
+    o->total_time = d.total_time;
+    o->sample_period = d.sample_period;
+    o->restart_period = d.restart_period;
+    o->ncommands = (int32_t)d.commands.size();
+    o->state = d.state == "random" ? 0 : d.state == "restart" ? 1 : d.state == "lamella" ? 2 : 3;
+    o->restart_state = d.restart_state;
+    return DPD_OK;
+}
+
+int dpd_input_types(const dpd_input* in, char* names, int32_t len, double* a, double* gamma)
+{
+    if (!in) return DPD_ERR_PARAM;
+    std::string all;
+    for (size_t i = 0; i < in->d.bead_names.size(); ++i) all += (i ? "," : "") + in->d.bead_names[i];
+    put_err(names, len, all);
+    if (a) std::copy(in->a.begin(), in->a.end(), a);
+    if (gamma) std::copy(in->g.begin(), in->g.end(), gamma);
+    return DPD_OK;
+}
+
+int dpd_input_beads(const dpd_input* in, int32_t* id, int32_t* type, int32_t* poly_id, int32_t* poly_type)
+{
+    if (!in) return DPD_ERR_PARAM;
+    const System& s = in->s;
+    if (id) std::copy(s.id.begin(), s.id.end(), id);
+    if (type) std::copy(s.type.begin(), s.type.end(), type);
+    if (poly_id) std::copy(s.poly_id.begin(), s.poly_id.end(), poly_id);
+    if (poly_type) std::copy(s.poly_type.begin(), s.poly_type.end(), poly_type);
+    return DPD_OK;
+}
+
+int dpd_input_topology(const dpd_input* in, int32_t* head, int32_t* tail, double* k, double* l0, int32_t* first,
+                       int32_t* second, double* modulus, double* phi0)
+{
+    if (!in) return DPD_ERR_PARAM;
+    const System& s = in->s;
+    if (head) std::copy(s.bond_head.begin(), s.bond_head.end(), head);
+    if (tail) std::copy(s.bond_tail.begin(), s.bond_tail.end(), tail);
+    if (k) std::copy(s.bond_k.begin(), s.bond_k.end(), k);
+    if (l0) std::copy(s.bond_l0.begin(), s.bond_l0.end(), l0);
+    if (first) std::copy(s.pair_first.begin(), s.pair_first.end(), first);
+    if (second) std::copy(s.pair_second.begin(), s.pair_second.end(), second);
+    if (modulus) std::copy(s.pair_k.begin(), s.pair_k.end(), modulus);
+    if (phi0) std::copy(s.pair_phi0.begin(), s.pair_phi0.end(), phi0);
+    return DPD_OK;
+}
+
+int dpd_input_params(const dpd_input* in, int32_t device, dpd_params* p)
+{
+    if (!in || !p) return DPD_ERR_PARAM;
+    const InputData& d = in->d;
+    std::memset(p, 0, sizeof(*p));
+    p->nx = d.nx; p->ny = d.ny; p->nz = d.nz;
+    p->wx = d.wx; p->wy = d.wy; p->wz = d.wz;
+    p->dt = d.dt;
+    p->lambda = d.lambda;
+    p->kT = d.kT;
+    p->seed = (uint64_t)std::llabs(d.seed);   // CCNTCell::SetRNGSeed takes |seed|
+    p->ntypes = d.ntypes();
+    p->a = in->a.data();
+    p->gamma = in->g.data();
+    p->device = device;
+    p->pair_kernel = DPD_PAIR_AUTO;
+    return DPD_OK;
+}
+
+int dpd_input_build_state(const dpd_input* in, uint64_t seed, double* x, double* unpbc, double* v)
+{
+    if (!in || !x || !v) return DPD_ERR_PARAM;
+    const size_t n = in->s.id.size();
+    std::mt19937_64 rng(seed);
+    std::vector<double> u(3 * n);
+    std::string err;
+    if (in->d.state == "lamella") {
+        if (!build_lamella(in, rng, u.data(), err)) return DPD_ERR_UNSUPPORTED;
+    } else {
+        build_random(in, rng, u.data());
+    }
+    wrap(in->d, n, u.data(), x);
+    if (unpbc) std::copy(u.begin(), u.end(), unpbc);
+    maxwell(rng, in->d.kT, n, v);
+    return DPD_OK;
+}
+
+int dpd_input_command(const dpd_input* in, int32_t i, char* name, int32_t nlen, int64_t* time, char* args,
+                      int32_t alen)
+{
+    if (!in || i < 0 || i >= (int32_t)in->d.commands.size()) return DPD_ERR_PARAM;
+    const auto& c = in->d.commands[i];
+    put_err(name, nlen, c.name);
+    if (time) *time = c.time;
+    std::string a;
+    for (size_t k = 0; k < c.args.size(); ++k) a += (k ? " " : "") + c.args[k];
+    put_err(args, alen, a);
+    return DPD_OK;
+}
+
+int dpd_restart_read(const char* path, int64_t n, int32_t* poly_id, int32_t* poly_type, int32_t* id, int32_t* type,
+                     double* x, double* unpbc, double* v, double* f)
+{
+    std::ifstream in(path);
+    if (!in) return DPD_ERR_IO;
+    int w, g, sh;
+    if (!(in >> w >> g >> sh)) return DPD_ERR_IO;
+    for (int64_t i = 0; i < n; ++i) {
+        long pid, pt, bid, bt;
+        double r, c[12];
+        if (!(in >> pid >> pt >> bid >> bt >> r)) return DPD_ERR_IO;
+        for (double& q : c)
+            if (!(in >> q)) return DPD_ERR_IO;
+        if (poly_id) poly_id[i] = (int32_t)pid;
+        if (poly_type) poly_type[i] = (int32_t)pt;
+        if (id) id[i] = (int32_t)bid;
+        if (type) type[i] = (int32_t)bt;
+        for (int a = 0; a < 3; ++a) {
+            if (x) x[3 * i + a] = c[a];
+            if (unpbc) unpbc[3 * i + a] = c[3 + a];
+            if (v) v[3 * i + a] = c[6 + a];
+            if (f) f[3 * i + a] = c[9 + a];
+        }
+    }
+    return DPD_OK;
+}
+
+int dpd_restart_write(const char* path, int64_t n, const int32_t* poly_id, const int32_t* poly_type,
+                      const int32_t* id, const int32_t* type, const double* radius_by_type, const double* L3,
+                      const double* x, const double* unpbc, const double* v, const double* f)
+{
+    std::ofstream o(path);
+    if (!o) return DPD_ERR_IO;
+    o << "0 0 0\n";
+    for (int64_t i = 0; i < n; ++i) {
+        double xp[3];
+        for (int a = 0; a < 3; ++a) {
+            xp[a] = x[3 * i + a];
+            if (xp[a] > L3[a]) return DPD_ERR_OUT_OF_BOX;
+            if (xp[a] > L3[a] - 0.001 || xp[a] < 0.001) xp[a] = 0.001;   // CheckBeadWithinBox
+        }
+        o << poly_id[i] << " " << poly_type[i] << " " << id[i] << " " << type[i] << " "
+          << fmt6(radius_by_type ? radius_by_type[type[i]] : 0.5) << " ";
+        for (int a = 0; a < 3; ++a) o << fmt6(xp[a]) << " ";
+        for (int a = 0; a < 3; ++a) o << fmt6(unpbc[3 * i + a]) << " ";
+        for (int a = 0; a < 3; ++a) o << fmt6(v[3 * i + a]) << " ";
+        o << fmt6(f[3 * i]) << " " << fmt6(f[3 * i + 1]) << " " << fmt6(f[3 * i + 2]) << "\n";
+    }
+    return o ? DPD_OK : DPD_ERR_IO;
+}
+
+int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* stats, char* err, int32_t errlen)
+{
+    if (!dir || !run_id) return DPD_ERR_PARAM;
+    const std::string base = std::string(dir) + "/";
+    dpd_input* in = nullptr;
+    int rc = dpd_input_read((base + "dmpci." + run_id).c_str(), &in, err, errlen);
+    if (rc) return rc;
+    std::unique_ptr<dpd_input> hold(in);
+    const InputData& d = in->d;
+    const System& s = in->s;
+    const int64_t n = (int64_t)s.id.size();
+    std::vector<double> x(3 * n), u(3 * n), v(3 * n), f(3 * n, 0.0);
+    if (d.state == "restart") {
+        const std::string rp = base + "dmpcrs." + d.restart_run + ".con." + std::to_string(d.restart_state) + ".dat";
+        std::vector<int32_t> ids(n);
+        rc = dpd_restart_read(rp.c_str(), n, nullptr, nullptr, ids.data(), nullptr, x.data(), u.data(), v.data(),
+                              f.data());
+        if (rc) { put_err(err, errlen, "cannot read restart snapshot " + rp); return rc; }
+        for (int64_t i = 0; i < n; ++i)
+            if (ids[i] != s.id[i]) { put_err(err, errlen, "restart snapshot does not match the polymers"); return DPD_ERR_IO; }
+    } else if (d.state == "random" || d.state == "lamella") {
+        rc = dpd_input_build_state(in, (uint64_t)std::llabs(d.seed), x.data(), u.data(), v.data());
+        if (rc) { put_err(err, errlen, "cannot build the starting state"); return rc; }
+    } else {
+        put_err(err, errlen, "State " + d.state + " is not supported by this engine");
+        return DPD_ERR_UNSUPPORTED;
+    }
+    dpd_params p;
+    dpd_input_params(in, device, &p);
+    dpd_engine* e = nullptr;
+    rc = dpd_create(&p, &e);
+    auto fail = [&](int code) {
+        put_err(err, errlen, e ? dpd_last_error(e) : "engine creation failed");
+        if (e) dpd_destroy(e);
+        return code;
+    };
+    if (rc) return fail(rc);
+    if ((rc = dpd_upload_beads(e, n, s.id.data(), s.type.data(), x.data(), u.data(), v.data(), f.data(), 0)))
+        return fail(rc);
+    if (!s.bond_head.empty() || !s.pair_first.empty())
+        if ((rc = dpd_upload_topology(e, (int64_t)s.bond_head.size(), s.bond_head.data(), s.bond_tail.data(),
+                                      s.bond_k.data(), s.bond_l0.data(), (int64_t)s.pair_first.size(),
+                                      s.pair_first.data(), s.pair_second.data(), s.pair_k.data(),
+                                      s.pair_phi0.data())))
+            return fail(rc);
+    std::ofstream hist(base + "dmpchs." + run_id);
+    hist << std::setprecision(6);
+    const double L3[3] = {d.nx * d.wx, d.ny * d.wy, d.nz * d.wz};
+    dpd_run_stats st{};
+    st.nbeads = n;
+    size_t next_cmd = 0;
+    bool thermo_on = true, cons_on = true;
+    double dt = d.dt;
+    std::vector<double> a = in->a, g = in->g;
+    double seconds = 0.0;
+    int64_t t = 0;
+    while (t < d.total_time) {
+        // commands due before step t+1 (CSimBox::Run executes them before Evolve)
+        while (next_cmd < d.commands.size() && d.commands[next_cmd].time <= t + 1) {
+            const auto& c = d.commands[next_cmd++];
+            if (c.time < t + 1) continue;
+            if (c.name == "ToggleDPDBeadThermostat") { thermo_on = !thermo_on; rc = dpd_toggle(e, DPD_TOGGLE_THERMOSTAT, thermo_on); }
+            else if (c.name == "ToggleDPDBeadConservativeForces") { cons_on = !cons_on; rc = dpd_toggle(e, DPD_TOGGLE_CONSERVATIVE, cons_on); }
+            else if (c.name == "SetTimeStepSize" && !c.args.empty()) { dt = std::atof(c.args[0].c_str()); rc = dpd_set_timestep(e, dt); }
+            else if ((c.name == "SetDPDBeadConsInt" || c.name == "SetDPDBeadDissInt") && c.args.size() >= 3) {
+                const int i = d.type_of(c.args[0]), j = d.type_of(c.args[1]);
+                if (i >= 0 && j >= 0) {
+                    std::vector<double>& m = c.name == "SetDPDBeadConsInt" ? a : g;
+                    m[i * d.ntypes() + j] = m[j * d.ntypes() + i] = std::atof(c.args[2].c_str());
+                    rc = dpd_set_interactions(e, a.data(), g.data());
+                }
+            } else {
+                continue;   // commands outside the step engine are not executed
+            }
+            if (rc) return fail(rc);
+            st.commands_applied++;
+        }
+        int64_t stop = d.total_time;
+        if (next_cmd < d.commands.size()) stop = std::min(stop, d.commands[next_cmd].time - 1);
+        stop = std::min(stop, (t / d.sample_period + 1) * d.sample_period);
+        stop = std::min(stop, (t / d.restart_period + 1) * d.restart_period);
+        if (stop <= t) stop = t + 1;
+        const bool sample = stop % d.sample_period == 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((rc = dpd_step(e, stop - t, sample ? DPD_STEP_OBSERVE : 0))) return fail(rc);
+        seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        t = stop;
+        if (sample) {
+            dpd_obs o;
+            dpd_observables(e, &o);
+            hist << t << " " << o.kT << " " << o.pressure << "\n";
+            st.last_kT = o.kT;
+            st.last_pressure = o.pressure;
+            st.samples++;
+        }
+        if (t % d.restart_period == 0) {
+            if ((rc = dpd_download(e, x.data(), u.data(), v.data(), f.data()))) return fail(rc);
+            const std::string rp = base + "dmpcrs." + std::string(run_id) + ".con." + std::to_string(t) + ".dat";
+            if ((rc = dpd_restart_write(rp.c_str(), n, s.poly_id.data(), s.poly_type.data(), s.id.data(),
+                                        s.type.data(), d.bead_radius.data(), L3, x.data(), u.data(), v.data(),
+                                        f.data())))
+                return fail(rc);
+        }
+    }
+    st.steps = t;
+    st.seconds = seconds;
+    st.bead_steps_per_s = seconds > 0 ? (double)n * (double)t / seconds : 0.0;
+    if (stats) *stats = st;
+    dpd_destroy(e);
+    return DPD_OK;
+}
+
+}  // extern "C"

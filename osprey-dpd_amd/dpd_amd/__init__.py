@@ -1,0 +1,202 @@
+"""Python binding of the MI355X DPD engine (ctypes over include/dpd_engine.h).
+
+The HIP library lib/libospreydpd_amd.so is required: importing works without it, but
+every call that needs it raises ``EngineMissing``; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libospreydpd_amd.so")
+
+OK = 0
+ERR_NAMES = {1: "PARAM", 2: "OOM", 3: "HIP", 4: "NAN", 5: "OUT_OF_BOX", 6: "STATE", 7: "IO",
+             8: "UNSUPPORTED"}
+PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL = 0, 1, 2
+STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING = 1, 2, 4
+TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE = 1, 2
+
+
+class EngineMissing(RuntimeError):
+    pass
+
+
+class DpdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dpd error {ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("wx", C.c_double), ("wy", C.c_double), ("wz", C.c_double),
+                ("dt", C.c_double), ("lam", C.c_double), ("kT", C.c_double),
+                ("seed", C.c_uint64), ("ntypes", C.c_int32),
+                ("a", C.POINTER(C.c_double)), ("gamma", C.POINTER(C.c_double)),
+                ("device", C.c_int32), ("pair_kernel", C.c_int32)]
+
+
+class Obs(C.Structure):
+    _fields_ = [("step", C.c_int64), ("kT", C.c_double), ("pressure", C.c_double),
+                ("stress", C.c_double * 9), ("sum_v2", C.c_double * 3),
+                ("momentum", C.c_double * 3), ("pe_pair", C.c_double), ("pe_bond", C.c_double),
+                ("pe_angle", C.c_double), ("jumps", C.c_int64), ("nbeads", C.c_int64)]
+
+
+# exported symbols of include/dpd_engine.h: name -> (restype, argtypes)
+_P = C.c_void_p
+SIGNATURES = {
+    "dpd_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "dpd_destroy": (None, [_P]),
+    "dpd_last_error": (C.c_char_p, [_P]),
+    "dpd_upload_beads": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, _P, _P, C.c_int64]),
+    "dpd_upload_topology": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, _P, _P, _P, _P]),
+    "dpd_step": (C.c_int, [_P, C.c_int64, C.c_uint32]),
+    "dpd_compute_forces": (C.c_int, [_P, C.c_int64, C.c_uint32, _P, _P, _P]),
+    "dpd_observables": (C.c_int, [_P, C.POINTER(Obs)]),
+    "dpd_download": (C.c_int, [_P, _P, _P, _P, _P]),
+    "dpd_set_interactions": (C.c_int, [_P, _P, _P]),
+    "dpd_set_timestep": (C.c_int, [_P, C.c_double]),
+    "dpd_toggle": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "dpd_last_timing": (C.c_int64, [_P, _P]),
+    "dpd_get_step": (C.c_int64, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises EngineMissing if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineMissing(f"{LIB_PATH} not built: run __graft_entry__.build() or "
+                                f"make -C osprey-dpd_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Engine:
+    """One DPD box on one GPU."""
+
+    def __init__(self, cells, a, gamma, dt=0.02, lam=0.5, kT=1.0, seed=1, widths=(1.0, 1.0, 1.0),
+                 device=0, pair_kernel=PAIR_AUTO):
+        L = lib()
+        self._a = np.ascontiguousarray(a, dtype=np.float64)
+        self._g = np.ascontiguousarray(gamma, dtype=np.float64)
+        p = Params()
+        p.nx, p.ny, p.nz = (int(c) for c in cells)
+        p.wx, p.wy, p.wz = (float(w) for w in widths)
+        p.dt, p.lam, p.kT = dt, lam, kT
+        p.seed = int(seed)
+        p.ntypes = self._a.shape[0]
+        p.a = self._a.ctypes.data_as(C.POINTER(C.c_double))
+        p.gamma = self._g.ctypes.data_as(C.POINTER(C.c_double))
+        p.device = device
+        p.pair_kernel = pair_kernel
+        h = C.c_void_p()
+        rc = L.dpd_create(C.byref(p), C.byref(h))
+        self.h = h
+        if rc != OK:
+            msg = L.dpd_last_error(h).decode() if h else ""
+            self.close()
+            raise DpdError(rc, msg)
+        self.n = 0
+        self.cells = tuple(int(c) for c in cells)
+        self.widths = tuple(float(w) for w in widths)
+
+    def _check(self, rc):
+        if rc != OK:
+            raise DpdError(rc, lib().dpd_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dpd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, ids, types, x, v, f=None, unpbc=None, step0=0):
+        n = len(ids)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        x, v, f, u = _f64(x, (n, 3)), _f64(v, (n, 3)), _f64(f, (n, 3)), _f64(unpbc, (n, 3))
+        self._check(lib().dpd_upload_beads(self.h, n, _ptr(ids), _ptr(types), _ptr(x), _ptr(u),
+                                           _ptr(v), _ptr(f), int(step0)))
+        self.n = n
+
+    def upload_topology(self, head=(), tail=(), k=(), l0=(), first=(), second=(), modulus=(), phi0=()):
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        head, tail, first, second = i32(head), i32(tail), i32(first), i32(second)
+        k, l0, modulus, phi0 = _f64(k), _f64(l0), _f64(modulus), _f64(phi0)
+        self._check(lib().dpd_upload_topology(self.h, len(head), _ptr(head), _ptr(tail), _ptr(k),
+                                              _ptr(l0), len(first), _ptr(first), _ptr(second),
+                                              _ptr(modulus), _ptr(phi0)))
+
+    def step(self, nsteps=1, observe=False, timing=False):
+        flags = (STEP_OBSERVE if observe else 0) | (STEP_TIMING if timing else 0)
+        self._check(lib().dpd_step(self.h, int(nsteps), flags))
+
+    def compute_forces(self, step, thermostat=True):
+        f = np.zeros((self.n, 3))
+        vir = np.zeros(9)
+        pe = np.zeros(3)
+        flags = 0 if thermostat else NO_THERMOSTAT
+        self._check(lib().dpd_compute_forces(self.h, int(step), flags, _ptr(f), _ptr(vir), _ptr(pe)))
+        return f, vir, pe
+
+    def observables(self):
+        o = Obs()
+        self._check(lib().dpd_observables(self.h, C.byref(o)))
+        return {"step": o.step, "kT": o.kT, "pressure": o.pressure, "stress": np.array(o.stress[:]),
+                "sum_v2": np.array(o.sum_v2[:]), "momentum": np.array(o.momentum[:]),
+                "pe": np.array([o.pe_pair, o.pe_bond, o.pe_angle]), "jumps": o.jumps}
+
+    def download(self):
+        n = self.n
+        x, u, v, f = (np.empty((n, 3)) for _ in range(4))
+        self._check(lib().dpd_download(self.h, _ptr(x), _ptr(u), _ptr(v), _ptr(f)))
+        return x, u, v, f
+
+    def set_interactions(self, a=None, gamma=None):
+        a, g = _f64(a), _f64(gamma)
+        self._check(lib().dpd_set_interactions(self.h, _ptr(a), _ptr(g)))
+
+    def set_timestep(self, dt):
+        self._check(lib().dpd_set_timestep(self.h, float(dt)))
+
+    def toggle(self, what, on):
+        self._check(lib().dpd_toggle(self.h, int(what), int(bool(on))))
+
+    def last_timing(self):
+        ms = np.zeros(3)
+        n = lib().dpd_last_timing(self.h, _ptr(ms))
+        return int(n), ms
+
+    @property
+    def step_count(self):
+        return lib().dpd_get_step(self.h)

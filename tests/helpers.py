@@ -46,3 +46,63 @@ def six_digit_close(got, ref, rtol=5.01e-6, atol=1e-12):
 
 def rel_norm(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def random_state(cells, n, ntypes, seed, widths=(1.0, 1.0, 1.0), kT=1.0):
+    """Uniform positions in the box, Maxwell velocities (COM removed), random types."""
+    rng = np.random.default_rng(seed)
+    L = np.array(cells, dtype=float) * np.array(widths)
+    x = rng.uniform(0, 1, size=(n, 3)) * L
+    v = rng.normal(0, np.sqrt(kT), size=(n, 3))
+    v -= v.mean(axis=0)
+    t = rng.integers(0, ntypes, size=n).astype(np.int32)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    return ids, t, x, v
+
+
+def chain_state(cells, nchains, length, nsolvent, seed, bond=0.5):
+    """Linear chains as random walks (step `bond`) plus solvent; returns ids, types (0 =
+    solvent, 1 = chain end/sticker, 2 = chain interior), wrapped x, unwrapped u, v and the
+    topology: bonds (head = earlier bead), bond pairs (first = later bond) as in the
+    reference's polymer construction (src/InputData.cpp:1114-1164, :1469-1513)."""
+    rng = np.random.default_rng(seed)
+    L = np.array(cells, dtype=float)
+    u = []
+    types = []
+    head, tail, first, second = [], [], [], []
+    for c in range(nchains):
+        start = rng.uniform(0, 1, size=3) * L
+        base = len(u)
+        p = start.copy()
+        for k in range(length):
+            if k:
+                d = rng.normal(size=3)
+                p = p + bond * d / np.linalg.norm(d)
+            u.append(p.copy())
+            types.append(1 if k % 4 == 0 else 2)
+            if k:
+                head.append(base + k - 1)
+                tail.append(base + k)
+            if k >= 2:
+                # bonds A = (k-2 -> k-1) index b-1, B = (k-1 -> k) index b: first = B, second = A
+                b = len(head) - 1
+                first.append(b)
+                second.append(b - 1)
+    for s in range(nsolvent):
+        u.append(rng.uniform(0, 1, size=3) * L)
+        types.append(0)
+    u = np.array(u)
+    x = np.mod(u, L)
+    x[x >= L] = 0.0
+    n = len(u)
+    v = rng.normal(size=(n, 3))
+    v -= v.mean(axis=0)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    return ids, np.array(types, dtype=np.int32), x, u, v, (head, tail, first, second)
+
+
+def force_errors(f, ref, floor=1.0):
+    """Normwise relative error and per-bead max |dF_i| / max(|F_i|, floor)."""
+    norm = rel_norm(f, ref)
+    per = np.linalg.norm(f - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), floor)
+    return norm, float(per.max())

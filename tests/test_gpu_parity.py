@@ -1,0 +1,195 @@
+"""GPU parity: the HIP engine against the FP64 oracle (Philox pair stream) and the
+reference's golden snapshots.  Tolerances: forces within 1e-5 normwise relative
+(FP32 pair math), positions/velocities within 1e-5 relative after one step."""
+import sys
+
+import numpy as np
+import pytest
+
+import dpd_amd as D
+import oracle as O
+from helpers import (SEEDS, chain_state, force_errors, load_beads, random_state, rel_norm,
+                     restart_clamp)
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = [pytest.param(D.PAIR_FULLSHELL, id="fullshell"), pytest.param(D.PAIR_BRICK, id="brick")]
+BINARY_A = np.array([[25.0, 80.0], [80.0, 25.0]])
+BINARY_G = np.array([[4.5, 4.5], [4.5, 4.5]])
+C4_A = np.array([[25.0, 25.0, 25.0], [25.0, 10.0, 25.0], [25.0, 25.0, 25.0]])
+C4_G = np.full((3, 3), 4.5)
+FORCE_TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def engine(cells, a, g, kernel, seed=1234, dt=0.02, widths=(1.0, 1.0, 1.0)):
+    return D.Engine(cells, a, g, dt=dt, seed=seed, widths=widths, pair_kernel=kernel)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("cells,ntypes,thermo", [((10, 10, 10), 2, True), ((10, 10, 10), 2, False),
+                                                 ((20, 20, 20), 2, True), ((9, 11, 7), 2, True)])
+def test_forces_match_oracle(kernel, cells, ntypes, thermo):
+    say("forces", kernel, cells, thermo)
+    n = 3 * int(np.prod(cells))
+    ids, t, x, v = random_state(cells, n, ntypes, seed=7)
+    e = engine(cells, BINARY_A, BINARY_G, kernel)
+    e.upload(ids, t, x, v)
+    f, vir, pe = e.compute_forces(step=5, thermostat=thermo)
+    box = O.Box(cells, BINARY_A, BINARY_G, seed=1234)
+    fr, virr, per = O.forces(box, ids, t, x, v, step=5, thermostat=thermo)
+    norm, per_bead = force_errors(f, fr)
+    say(f"  normwise {norm:.2e} per-bead {per_bead:.2e}")
+    assert norm < FORCE_TOL
+    assert per_bead < 20 * FORCE_TOL
+    assert rel_norm(vir, virr) < FORCE_TOL
+    assert abs(pe[0] - per[0]) / per[0] < FORCE_TOL
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_forces_condensate_box_full_size(kernel):
+    """64^3 cells at rho = 3 (786,432 beads, three types with a sticker a = 10)."""
+    cells = (64, 64, 64)
+    n = 3 * 64 ** 3
+    say("condensate full size", kernel)
+    ids, t, x, v = random_state(cells, n, 3, seed=11)
+    e = engine(cells, C4_A, C4_G, kernel)
+    e.upload(ids, t, x, v)
+    f, vir, _ = e.compute_forces(step=17)
+    fr, virr, _ = O.forces(O.Box(cells, C4_A, C4_G, seed=1234), ids, t, x, v, step=17)
+    norm, per_bead = force_errors(f, fr)
+    say(f"  normwise {norm:.2e} per-bead {per_bead:.2e}")
+    assert norm < FORCE_TOL
+    assert rel_norm(vir, virr) < FORCE_TOL
+    # Newton's third law on the GPU sum
+    assert np.abs(f.sum(axis=0)).max() / np.abs(f).sum() < 1e-6
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_bonded_forces_match_oracle(kernel):
+    cells = (12, 12, 12)
+    ids, t, x, u, v, (h, tl, fi, se) = chain_state(cells, 60, 16, 4224, seed=3)
+    n = len(ids)
+    k = np.full(len(h), 128.0)
+    l0 = np.full(len(h), 0.5)
+    mod = np.where(np.arange(len(fi)) % 3 == 0, 15.0, 20.0)
+    phi0 = np.where(np.arange(len(fi)) % 3 == 0, np.radians(30.0), 0.0)
+    say("bonded", kernel, n)
+    e = engine(cells, C4_A, C4_G, kernel)
+    e.upload(ids, t, x, v, unpbc=u)
+    e.upload_topology(h, tl, k, l0, fi, se, mod, phi0)
+    f, vir, pe = e.compute_forces(step=2)
+    topo = O.Topo(h, tl, k, l0, fi, se, mod, phi0)
+    fr, virr, per = O.forces(O.Box(cells, C4_A, C4_G, seed=1234), ids, t, x, v, step=2, unpbc=u, topo=topo)
+    norm, per_bead = force_errors(f, fr)
+    say(f"  normwise {norm:.2e} per-bead {per_bead:.2e}")
+    assert norm < FORCE_TOL
+    assert rel_norm(vir, virr) < FORCE_TOL
+    assert np.allclose(pe[1:], per[1:], rtol=1e-9)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("thermo", [False, True])
+def test_one_step_matches_oracle(kernel, thermo):
+    """One full integration step (UpdatePos, forces, bonds, angles, UpdateMom)."""
+    cells = (12, 12, 12)
+    ids, t, x, u, v, (h, tl, fi, se) = chain_state(cells, 40, 16, 4544, seed=5)
+    k = np.full(len(h), 128.0)
+    l0 = np.full(len(h), 0.5)
+    mod = np.full(len(fi), 20.0)
+    phi0 = np.zeros(len(fi))
+    topo = O.Topo(h, tl, k, l0, fi, se, mod, phi0)
+    g = C4_G if thermo else np.zeros_like(C4_G)
+    box = O.Box(cells, C4_A, g, seed=99, rng_mode=O.RNG_PHILOX)
+    # start from a force-consistent state: f = F(x, v)
+    f0, _, _ = O.forces(box, ids, t, x, v, step=0, unpbc=u, topo=topo)
+    say("one step", kernel, thermo)
+    e = D.Engine(cells, C4_A, g, seed=99, pair_kernel=kernel)
+    e.upload(ids, t, x, v, f=f0, unpbc=u, step0=0)
+    e.upload_topology(h, tl, k, l0, fi, se, mod, phi0)
+    e.step(1, observe=True)
+    X, U, V, F = e.download()
+    s = O.System(box, ids, t, x, v, f0, unpbc=u, topo=topo)
+    s.run(1, step0=0)
+    xr, ur, vr, fr = s.state()
+    assert rel_norm(U, ur) < 1e-7
+    assert rel_norm(V, vr) < FORCE_TOL
+    norm, _ = force_errors(F, fr)
+    say(f"  force normwise {norm:.2e}")
+    assert norm < FORCE_TOL
+    dx = (X - xr + 0.5 * np.array(cells)) % np.array(cells) - 0.5 * np.array(cells)
+    assert np.abs(dx).max() < 1e-5
+    ob = e.observables()
+    obr = s.observables()
+    assert abs(ob["kT"] - obr["kT"]) / obr["kT"] < FORCE_TOL
+    assert abs(ob["pressure"] - obr["pressure"]) / obr["pressure"] < FORCE_TOL
+    assert rel_norm(ob["stress"], obr["stress"]) < FORCE_TOL
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("case", ["water6", "binary6"])
+def test_golden_noise_free_step(kernel, case):
+    """One noise-free step from the reference's snapshot vs the reference's own result."""
+    a = np.array([[25.0]]) if case == "water6" else BINARY_A
+    d = load_beads(case, "in")
+    ref = load_beads(case, "nf1")
+    say("golden", kernel, case)
+    e = D.Engine((6, 6, 6), a, np.zeros_like(a), seed=SEEDS[case], pair_kernel=kernel)
+    e.upload(d["id"], d["type"], d["x"], d["v"], f=d["f"], unpbc=d["u"])
+    e.step(1)
+    X, U, V, F = e.download()
+    tol = 2e-5
+    assert np.all(np.abs(V - ref["v"]) <= tol * np.abs(ref["v"]) + 1e-5)
+    assert rel_norm(F, ref["f"]) < tol
+    assert np.all(np.abs(U - ref["u"]) <= 1e-5 * np.abs(ref["u"]) + 1e-5)
+    Xc = restart_clamp(X, np.array([6.0, 6.0, 6.0]))
+    assert np.all(np.abs(Xc - ref["x"]) <= 1e-5 * np.abs(ref["x"]) + 1e-5)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_ten_steps_track_oracle(kernel):
+    cells = (10, 10, 10)
+    n = 3000
+    ids, t, x, v = random_state(cells, n, 2, seed=21)
+    box = O.Box(cells, BINARY_A, BINARY_G, seed=5)
+    # relax the random start so forces are moderate
+    s = O.System(box, ids, t, x, v)
+    s.run(30)
+    x, u, v, f = s.state()
+    say("ten steps", kernel)
+    e = D.Engine(cells, BINARY_A, BINARY_G, seed=5, pair_kernel=kernel)
+    e.upload(ids, t, x, v, f=f, unpbc=u, step0=30)
+    e.step(10, observe=True)
+    X, U, V, F = e.download()
+    s.run(10, step0=30)
+    xr, ur, vr, fr = s.state()
+    say(f"  u {rel_norm(U, ur):.2e} v {rel_norm(V, vr):.2e} f {rel_norm(F, fr):.2e}")
+    assert rel_norm(U, ur) < 1e-6
+    assert rel_norm(V, vr) < 1e-4
+    assert rel_norm(F, fr) < 1e-4
+    ob, obr = e.observables(), s.observables()
+    assert abs(ob["kT"] - obr["kT"]) / obr["kT"] < 1e-4
+    assert abs(ob["pressure"] - obr["pressure"]) / obr["pressure"] < 1e-4
+    assert ob["step"] == 40
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_momentum_conserved(kernel):
+    cells = (16, 16, 16)
+    ids, t, x, v = random_state(cells, 3 * 16 ** 3, 2, seed=4)
+    e = D.Engine(cells, BINARY_A, BINARY_G, seed=8, pair_kernel=kernel)
+    e.upload(ids, t, x, v)
+    e.step(200, observe=True)
+    ob = e.observables()
+    say("momentum", ob["momentum"], "kT", ob["kT"])
+    assert np.abs(ob["momentum"]).max() < 1e-2 * np.sqrt(len(ids))
+    assert ob["jumps"] == 0
+    X, U, V, F = e.download()
+    L = np.array(cells, dtype=float)
+    img = (U - X) / L
+    assert np.abs(img - np.round(img)).max() < 1e-6
+    assert (X >= 0).all() and (X < L).all()

@@ -1,0 +1,261 @@
+"""CPU tests of the FP64 oracle: pinned against the reference's own outputs (golden
+snapshots written by the compiled reference), rocRAND, closed forms and brute force."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import GOLDEN, SEEDS, load_beads, load_history, restart_clamp, six_digit_close
+
+WATER_A = [[25.0]]
+WATER_G = [[4.5]]
+BINARY_A = [[25.0, 80.0], [80.0, 25.0]]
+BINARY_G = [[4.5, 4.5], [4.5, 4.5]]
+SIMPLE = {"water6": ((6, 6, 6), WATER_A, WATER_G), "binary6": ((6, 6, 6), BINARY_A, BINARY_G)}
+
+
+def test_lcg_matches_reference_draws():
+    """CCNTCell::Randf (src/CNTCell.cpp:2454-2465) vs the reference's GenerateRNGUniform log."""
+    ref = np.loadtxt(os.path.join(GOLDEN, "rng.txt"))
+    got = O.lcg_sequence(SEEDS["water6"], len(ref))
+    assert six_digit_close(got, ref).all()
+    # plausible mistakes fail: another multiplier, or the low word instead of the high word
+    def seq(mult, shift):
+        s, out = SEEDS["water6"], []
+        for _ in range(len(ref)):
+            s = (mult * s + 1) % 2**64
+            out.append(((s >> shift) & 0xFFFFFFFF) / 2**32)
+        return np.array(out)
+    assert six_digit_close(seq(6364136223846793005, 32), ref).all()
+    assert not six_digit_close(seq(6364136223846793007, 32), ref).all()
+    assert not six_digit_close(seq(6364136223846793005, 0), ref).all()
+
+
+def test_philox_matches_rocrand():
+    rows = [l.split() for l in open(os.path.join(GOLDEN, "philox_kat.txt")) if not l.startswith("#")]
+    assert len(rows) >= 6
+    for r in rows:
+        seed = int(r[0], 16)
+        ctr = [int(t, 16) for t in r[1:5]]
+        exp = tuple(int(t, 16) for t in r[5:9])
+        assert O.philox4x32_10(ctr, (seed & 0xFFFFFFFF, seed >> 32)) == exp
+
+
+def test_pair_uniform_symmetric_and_step_keyed():
+    u1 = O.pair_uniform(7, 11, 3, 9)
+    assert u1 == O.pair_uniform(7, 11, 9, 3)
+    assert u1 != O.pair_uniform(7, 12, 3, 9)
+    assert 0.0 <= u1 < 1.0
+
+
+def test_pair_uniform_moments():
+    rng = np.random.default_rng(0)
+    ids = rng.integers(1, 10**6, size=(20000, 2))
+    u = np.array([O.pair_uniform(99, 5, int(a), int(b)) for a, b in ids])
+    assert abs(u.mean() - 0.5) < 0.01
+    assert abs(u.var() - 1 / 12) < 0.003
+    # consecutive steps are uncorrelated
+    u2 = np.array([O.pair_uniform(99, 6, int(a), int(b)) for a, b in ids])
+    assert abs(np.corrcoef(u, u2)[0, 1]) < 0.03
+
+
+def _run_golden(case, name, nsteps, lam=0.5, thermostat=None, seed=None):
+    cells, a, g = SIMPLE[case]
+    d = load_beads(case, "in")
+    thermo = name != "nf1" if thermostat is None else thermostat
+    gam = np.array(g) if thermo else np.zeros_like(np.array(g))
+    box = O.Box(cells, np.array(a), gam, seed=SEEDS[case] if seed is None else seed, lam=lam,
+                rng_mode=O.RNG_LCG)
+    s = O.System(box, d["id"], d["type"], d["x"], d["v"], d["f"], unpbc=d["u"])
+    s.run(nsteps)
+    return s, box
+
+
+@pytest.mark.parametrize("case", ["water6", "binary6"])
+@pytest.mark.parametrize("name,nsteps", [("nf1", 1), ("th1", 1), ("th10", 10)])
+def test_oracle_matches_reference_snapshot(case, name, nsteps):
+    s, box = _run_golden(case, name, nsteps)
+    x, u, v, f = s.state()
+    ref = load_beads(case, name)
+    assert (ref["id"] == load_beads(case, "in")["id"]).all()
+    assert six_digit_close(restart_clamp(x, box.lengths), ref["x"]).all()
+    assert six_digit_close(u, ref["u"]).all()
+    assert six_digit_close(v, ref["v"]).all()
+    assert six_digit_close(f, ref["f"]).all()
+
+
+@pytest.mark.parametrize("case", ["water6", "binary6"])
+def test_oracle_matches_reference_history(case):
+    """kT and P rows of dmpchs (src/Monitor.cpp:979-982) over 10 thermostatted steps."""
+    cells, a, g = SIMPLE[case]
+    d = load_beads(case, "in")
+    box = O.Box(cells, np.array(a), np.array(g), seed=SEEDS[case], rng_mode=O.RNG_LCG)
+    s = O.System(box, d["id"], d["type"], d["x"], d["v"], d["f"], unpbc=d["u"])
+    hist = load_history(case)
+    for row in hist:
+        s.run(1)
+        ob = s.observables()
+        assert six_digit_close(np.array([ob["kT"], ob["pressure"]]), row[1:3]).all(), row
+        assert ob["wrong_cell"] == 0
+
+
+def test_planted_mistakes_are_caught():
+    """The snapshot checks fail for plausible mistakes planted in the oracle's inputs."""
+    ref = load_beads("water6", "th1")
+    # wrong lambda (velocity prediction) changes the dissipative force
+    s, _ = _run_golden("water6", "th1", 1, lam=0.0)
+    assert not six_digit_close(s.state()[3], ref["f"]).all()
+    # wrong seed changes the random force
+    s, _ = _run_golden("water6", "th1", 1, seed=SEEDS["water6"] + 1)
+    assert not six_digit_close(s.state()[3], ref["f"]).all()
+    # thermostat dropped
+    s, _ = _run_golden("water6", "th1", 1, thermostat=False)
+    assert not six_digit_close(s.state()[3], ref["f"]).all()
+
+
+# ------------------------------------------------------------ closed forms --------
+
+def _two_beads(r, dv=(0.0, 0.0, 0.0), a=25.0, g=4.5, thermostat=2):
+    box = O.Box((4, 4, 4), np.array([[a]]), np.array([[g]]), seed=3)
+    x = np.array([[1.5, 1.5, 1.5], [1.5 + r, 1.5, 1.5]])
+    v = np.array([[0.0, 0.0, 0.0], list(dv)])
+    f, vir, pe = O.forces(box, [1, 2], [0, 0], x, v, step=1, thermostat=thermostat)
+    return f, vir, pe
+
+
+def test_conservative_closed_form():
+    for r in (0.1, 0.5, 0.9):
+        f, vir, pe = _two_beads(r, g=0.0)
+        # force on bead 0 points away from bead 1 (-x): a (1 - r)
+        assert np.allclose(f[0], [-25.0 * (1 - r), 0, 0], rtol=1e-14)
+        assert np.allclose(f[1], -f[0], rtol=0, atol=0)
+        assert np.isclose(vir[0], r * 25.0 * (1 - r), rtol=1e-14)
+        assert np.isclose(pe[0], 0.5 * 25.0 * (1 - r) ** 2, rtol=1e-14)
+    f, _, _ = _two_beads(1.0, g=0.0)
+    assert np.all(f == 0.0)
+
+
+def test_dissipative_closed_form():
+    r, vrel = 0.6, 2.0
+    f, _, _ = _two_beads(r, dv=(vrel, 0.0, 0.0), a=0.0)
+    # bead 1 recedes along +x: dissipative force pulls bead 0 towards it
+    expect = 4.5 * (1 - r) ** 2 * vrel
+    assert np.allclose(f[0], [expect, 0, 0], rtol=1e-14)
+
+
+def test_random_force_moments():
+    """sqrt(gamma) w sqrt(24 kT/dt) (0.5 - U): mean 0, variance 2 gamma kT w^2 / dt."""
+    r = 0.5
+    box = O.Box((4, 4, 4), np.array([[0.0]]), np.array([[4.5]]), seed=11)
+    x = np.array([[1.5, 1.5, 1.5], [1.5 + r, 1.5, 1.5]])
+    v = np.zeros((2, 3))
+    fr = np.array([O.forces(box, [1, 2], [0, 0], x, v, step=t)[0][0, 0] for t in range(1, 4001)])
+    var_expect = 2 * 4.5 * 1.0 * (1 - r) ** 2 / 0.02
+    assert abs(fr.mean()) < 4 * np.sqrt(var_expect / len(fr))
+    assert abs(fr.var() / var_expect - 1) < 0.08
+
+
+def _random_state(cells, n, ntypes, seed):
+    rng = np.random.default_rng(seed)
+    L = np.array(cells, dtype=float)
+    x = rng.uniform(0, 1, size=(n, 3)) * L
+    v = rng.normal(0, 1, size=(n, 3))
+    t = rng.integers(0, ntypes, size=n).astype(np.int32)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    return ids, t, x, v
+
+
+def test_cell_list_matches_brute_force():
+    cells = (5, 4, 6)
+    ids, t, x, v = _random_state(cells, 360, 2, 1)
+    box = O.Box(cells, np.array(BINARY_A), np.array(BINARY_G), seed=5)
+    f1, vir1, _ = O.forces(box, ids, t, x, v, step=3)
+    f2, vir2 = O.pair_forces_brute(box, ids, t, x, v, step=3)
+    assert np.allclose(f1, f2, rtol=1e-12, atol=1e-10)
+    assert np.allclose(vir1, vir2, rtol=1e-12, atol=1e-9)
+    # Newton's third law: total force vanishes
+    assert np.abs(f1.sum(axis=0)).max() < 1e-9
+
+
+def test_bond_closed_form():
+    box = O.Box((4, 4, 4), np.array([[0.0]]), np.array([[0.0]]))
+    x = np.array([[1.0, 1.0, 1.0], [1.8, 1.0, 1.0]])
+    topo = O.Topo(head=[0], tail=[1], k=[128.0], l0=[0.5])
+    f, vir, pe = O.forces(box, [1, 2], [0, 0], x, np.zeros((2, 3)), step=1, topo=topo)
+    # stretched by 0.3: head pulled towards tail (+x) with k (L - l0)
+    assert np.allclose(f[0], [128.0 * 0.3, 0, 0], rtol=1e-12)
+    assert np.allclose(f[1], -f[0])
+    assert np.isclose(pe[1], 0.5 * 128 * 0.3 ** 2)
+    # head bead carries the stress d_x F_x with d = head - tail
+    assert np.isclose(vir[0], (-0.8) * (128.0 * 0.3))
+
+
+def _angle(theta_deg, k=20.0, phi0=0.0):
+    """Beads a-b-c with bonds (a->b), (b->c) and bond pair first=(b->c), second=(a->b).
+    theta is the bond angle at b (180 = straight)."""
+    th = np.radians(theta_deg)
+    b = np.array([2.0, 2.0, 2.0])
+    a = b + 0.5 * np.array([-1.0, 0.0, 0.0])
+    c = b + 0.5 * np.array([-np.cos(th), np.sin(th), 0.0])
+    x = np.array([a, b, c])
+    topo = O.Topo(head=[0, 1], tail=[1, 2], k=[0.0, 0.0], l0=[0.5, 0.5], first=[1], second=[0],
+                  modulus=[k], phi0=[phi0])
+    box = O.Box((4, 4, 4), np.array([[0.0]]), np.array([[0.0]]))
+    f, _, pe = O.forces(box, [1, 2, 3], [0, 0, 0], x, np.zeros((3, 3)), step=1, topo=topo)
+    return x, f, pe[2]
+
+
+def test_angle_closed_forms():
+    # straight chain: no force, zero energy (phi measured from straight)
+    _, f, e = _angle(180.0)
+    assert np.abs(f).max() < 1e-12 and abs(e) < 1e-12
+    # right angle: U = k, force on the end bead is k / |b| perpendicular to its bond
+    x, f, e = _angle(90.0)
+    assert np.isclose(e, 20.0)
+    assert np.allclose(np.linalg.norm(f[2]), 20.0 / 0.5)
+    assert np.abs(f.sum(axis=0)).max() < 1e-12
+
+
+def test_angle_finite_difference_phi0_zero():
+    """F = -dU/dx for U = k (1 - cos phi), phi from straight (src/BondPair.cpp:189-193)."""
+    x, f, _ = _angle(130.0)
+    topo = O.Topo(head=[0, 1], tail=[1, 2], k=[0.0, 0.0], l0=[0.5, 0.5], first=[1], second=[0],
+                  modulus=[20.0], phi0=[0.0])
+    box = O.Box((4, 4, 4), np.array([[0.0]]), np.array([[0.0]]))
+    h = 1e-6
+    for bead in range(3):
+        for d in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[bead, d] += h
+            xm[bead, d] -= h
+            ep = O.forces(box, [1, 2, 3], [0] * 3, xp, np.zeros((3, 3)), 1, topo=topo)[2][2]
+            em = O.forces(box, [1, 2, 3], [0] * 3, xm, np.zeros((3, 3)), 1, topo=topo)[2][2]
+            assert np.isclose(f[bead, d], -(ep - em) / (2 * h), rtol=1e-6, atol=1e-6)
+
+
+# --------------------------------------------- bonded cases via the dmpci parser -----
+
+@pytest.mark.parametrize("case", ["chains7", "lipid8", "lipidphi8"])
+@pytest.mark.parametrize("name,nsteps", [("nf1", 1), ("th1", 1), ("th10", 10)])
+def test_oracle_bonded_matches_reference(case, name, nsteps):
+    """Bonds (src/Bond.cpp:111-162) and bond pairs incl. phi0 > 0 (src/BondPair.cpp:138-286),
+    topology from the product's dmpci parser, against the reference's snapshots."""
+    from dpd_amd import host as H
+    inp = H.Input(os.path.join(GOLDEN, case, "case.dmpci"))
+    (h, t, k, l0), (fi, se, mo, ph) = inp.topology()
+    ids, ty, pid, _ = inp.beads()
+    d = load_beads(case, "in")
+    assert (d["id"] == ids).all() and (d["type"] == ty).all() and (d["poly"] == pid).all()
+    g = inp.gamma if name != "nf1" else np.zeros_like(inp.gamma)
+    box = O.Box(inp.cells, inp.a, g, seed=SEEDS[case], rng_mode=O.RNG_LCG)
+    s = O.System(box, d["id"], d["type"], d["x"], d["v"], d["f"], unpbc=d["u"],
+                 topo=O.Topo(h, t, k, l0, fi, se, mo, ph))
+    s.run(nsteps)
+    x, u, v, f = s.state()
+    ref = load_beads(case, name)
+    assert six_digit_close(restart_clamp(x, box.lengths), ref["x"]).all()
+    assert six_digit_close(u, ref["u"]).all()
+    assert six_digit_close(v, ref["v"]).all()
+    assert six_digit_close(f, ref["f"]).all()

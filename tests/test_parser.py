@@ -1,0 +1,114 @@
+"""CPU tests of the dmpci reader and polymer expansion (the product's host layer) against
+the counts the compiled reference creates for the same control files."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from dpd_amd import host as H
+from helpers import GOLDEN
+
+PARSER = os.path.join(GOLDEN, "parser")
+_counts = json.load(open(os.path.join(PARSER, "counts.json"))) if os.path.exists(
+    os.path.join(PARSER, "counts.json")) else {}
+
+
+@pytest.mark.parametrize("case", ["water6", "binary6", "chains7", "lipid8", "lipidphi8"])
+def test_golden_case_counts(case):
+    inp = H.Input(os.path.join(GOLDEN, case, "case.dmpci"))
+    exp = json.load(open(os.path.join(GOLDEN, case, "counts.json")))
+    c = inp.counts
+    assert {"polymers": c["npolymers"], "beads": c["nbeads"], "bonds": c["nbonds"],
+            "bondpairs": c["npairs"]} == exp
+
+
+@pytest.mark.parametrize("name", sorted(k for k, v in _counts.items() if v is not None))
+def test_reference_example_counts(name):
+    """Every example control file the reference ships (incl. the Biology-2023 set)."""
+    path = os.path.join(PARSER, name + ".dmpci")
+    exp = _counts[name]
+    try:
+        inp = H.Input(path)
+    except H.InputError as e:
+        assert "Wall" in str(e) or "Gravity" in str(e), str(e)
+        pytest.skip("Wall/Gravity inputs are outside the step engine")
+    c = inp.counts
+    assert {"polymers": c["npolymers"], "beads": c["nbeads"], "bonds": c["nbonds"],
+            "bondpairs": c["npairs"]} == exp
+
+
+def test_lipid_topology_shape():
+    """(H H (* (T T T T)) H T T T T): 11 beads, the branch hangs off bead 2, bond heads are
+    the earlier beads, bond pairs store (later bond, earlier bond)."""
+    text = open(os.path.join(GOLDEN, "lipid8", "case.dmpci")).read()
+    inp = H.Input(text=text)
+    ids, t, pid, pt = inp.beads()
+    (h, tl, k, l0), (fi, se, mo, ph) = inp.topology()
+    lip = np.where(pt == 1)[0]
+    first = lip[0]
+    sel = (h >= first) & (h < first + 11)
+    pairs = sorted(zip(h[sel] - first, tl[sel] - first))
+    assert pairs == [(0, 1), (1, 2), (1, 6), (2, 3), (3, 4), (4, 5), (6, 7), (7, 8), (8, 9), (9, 10)]
+    bsel = np.where(sel)[0]
+    b0 = bsel[0]
+    psel = (fi >= b0) & (fi < b0 + 10)
+    trip = sorted((h[s] - first, tl[s] - first, tl[f] - first) for f, s in zip(fi[psel], se[psel]))
+    # H-T-T and T-T-T triples only (HHT, HHH have no bond-pair type)
+    assert trip == [(1, 2, 3), (2, 3, 4), (3, 4, 5), (6, 7, 8), (7, 8, 9), (8, 9, 10)]
+    assert np.allclose(mo[psel], 20.0) and np.allclose(ph[psel], 0.0)
+
+
+def test_polymer_counts_floor_rounding():
+    """counts = floor(fraction * rho V / sum(size * fraction)) (src/InitialState.cpp:471)."""
+    text = open(os.path.join(GOLDEN, "chains7", "case.dmpci")).read()
+    inp = H.Input(text=text)
+    rho_v = 3 * 7 ** 3
+    nalpha = 0.984615 * 1 + 0.015385 * 16
+    assert inp.counts["npolymers"] == int(0.984615 * rho_v / nalpha) + int(0.015385 * rho_v / nalpha)
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ("Title", "Title"), ("Density\t\t3", "Density"), ("Lambda\t\t0.5", "lambda"),
+])
+def test_malformed_inputs_fail_with_the_item_named(bad, msg):
+    text = open(os.path.join(GOLDEN, "water6", "case.dmpci")).read()
+    broken = text.replace(bad, "Bogus", 1)
+    with pytest.raises(H.InputError) as e:
+        H.Input(text=broken)
+    assert msg.lower() in str(e.value).lower()
+
+
+def test_commands_are_read():
+    text = open(os.path.join(GOLDEN, "water6", "case.dmpci")).read()
+    text += "\nCommand ToggleDPDBeadThermostat 5\nCommand SetTimeStepSize 7 0.01\n"
+    inp = H.Input(text=text)
+    assert inp.commands() == [("ToggleDPDBeadThermostat", 5, []), ("SetTimeStepSize", 7, ["0.01"])]
+
+
+def test_random_state_builder():
+    inp = H.Input(text=open(os.path.join(GOLDEN, "lipid8", "case.dmpci")).read().replace(
+        "restart RunId golden StateId 0", "random"))
+    x, u, v = inp.build_state(seed=5)
+    L = np.array(inp.cells, float)
+    assert (x >= 0).all() and (x < L).all()
+    img = (u - x) / L
+    assert np.abs(img - np.round(img)).max() < 1e-9
+    n = len(x)
+    assert abs((v ** 2).sum() / (3 * n) - 1.0) < 1e-12
+    assert np.abs(v.sum(axis=0)).max() < 1e-9
+    (h, t, k, l0), _ = inp.topology()
+    blen = np.linalg.norm(u[h] - u[t], axis=1)
+    assert np.allclose(blen, 0.5)
+
+
+def test_restart_roundtrip(tmp_path):
+    from helpers import load_beads
+    d = load_beads("water6", "in")
+    L = np.array([6.0, 6.0, 6.0])
+    p = tmp_path / "dmpcrs.t.con.0.dat"
+    H.write_restart(p, d["poly"], d["ptype"], d["id"], d["type"], L, d["x"], d["u"], d["v"], d["f"])
+    r = H.read_restart(p, len(d["id"]))
+    assert (r["id"] == d["id"]).all()
+    for key in ("u", "v", "f"):
+        assert np.allclose(r[key], d[key], rtol=5e-6, atol=0)
