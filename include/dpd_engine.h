@@ -84,6 +84,8 @@ typedef struct dpd_obs {
 /* flags of dpd_step / dpd_compute_forces */
 #define DPD_STEP_OBSERVE   1u   /* accumulate FP64 observables at the last step      */
 #define DPD_NO_THERMOSTAT  2u   /* dpd_compute_forces: drop dissipative + random     */
+#define DPD_STEP_TIMING    4u   /* time every pair-kernel launch with HIP events     */
+#define DPD_STEP_GRAPH     8u   /* replay captured hipGraphs of up to 64 steps       */
 
 /* toggles of dpd_toggle (command handlers of src/CNTCell.cpp:223-346) */
 #define DPD_TOGGLE_THERMOSTAT     1   /* ToggleDPDBeadThermostat: on=0 zeroes gamma  */
@@ -133,9 +135,30 @@ int dpd_set_interactions(dpd_engine* e, const double* a, const double* gamma);
 int dpd_set_timestep(dpd_engine* e, double dt);
 int dpd_toggle(dpd_engine* e, int32_t what, int32_t on);
 
-/* Timing of the last dpd_step call's kernels (ms, from HIP events on the
- * engine's stream): [0] whole call, [1] pair kernel total, [2] other kernels.
- * Returns the number of pair-kernel launches timed. */
+/* ---- batches of independent boxes (replicas of uneven size and parameters) ----
+ * No counterpart in the reference, which runs one box per process (src/dmpc.cpp:203-213
+ * runs several runIds one after another).  All boxes advance together: every kernel of
+ * a step spans all boxes, and per-box constants and interaction tables live in device
+ * memory.  Bead and topology indices of dpd_upload_box* are local to the box. */
+int dpd_create_batch(int32_t nbox, const dpd_params* p, dpd_engine** out);   /* p[nbox] */
+int dpd_upload_box(dpd_engine* e, int32_t box, int64_t n, const int32_t* id, const int32_t* type,
+                   const double* x, const double* unpbc, const double* v, const double* f);
+int dpd_upload_box_topology(dpd_engine* e, int32_t box, int64_t nbonds, const int32_t* head,
+                            const int32_t* tail, const double* k, const double* l0, int64_t npairs,
+                            const int32_t* first_bond, const int32_t* second_bond,
+                            const double* modulus, const double* phi0);
+/* move the staged boxes to the device and sort them; step0 = simulation time */
+int dpd_commit(dpd_engine* e, int64_t step0);
+int32_t dpd_box_count(const dpd_engine* e);
+int dpd_observables_box(dpd_engine* e, int32_t box, dpd_obs* out);
+/* box = -1: all beads of all boxes in upload order */
+int dpd_download_box(dpd_engine* e, int32_t box, double* x, double* unpbc, double* v, double* f);
+/* box = -1: every box */
+int dpd_set_box_interactions(dpd_engine* e, int32_t box, const double* a, const double* gamma);
+
+/* Timing of the last dpd_step call (ms, from HIP events on the engine's stream):
+ * [0] whole call, [1] pair kernels (with DPD_STEP_TIMING), [2] the rest.  Returns the
+ * number of pair-kernel launches timed. */
 int64_t dpd_last_timing(dpd_engine* e, double* ms3);
 int64_t dpd_get_step(const dpd_engine* e);
 

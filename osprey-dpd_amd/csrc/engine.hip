@@ -1,5 +1,6 @@
-// engine.hip -- the DPD step engine for MI355X (gfx950): kernels of one box and the
-// C ABI of include/dpd_engine.h.
+// engine.hip -- the DPD step engine for MI355X (gfx950): kernels and the C ABI of
+// include/dpd_engine.h.  One engine holds one box or a batch of independent boxes
+// (replicas of uneven size and parameters); every kernel spans all boxes at once.
 //
 // One step (the reference's CSimBox::Evolve, src/SimBox.cpp:977-1120) runs as:
 //   k_advance   UpdateMom of the previous step fused with UpdatePos of this one
@@ -10,12 +11,15 @@
 //   k_permute   scatter into the new cell-sorted SoA buffers
 //   pair kernel DPD pair forces (src/CNTCell.cpp:1144-1555)
 // and a last k_finish closes the final step (UpdateMom + bonded + FP64 observables).
+// The simulation time lives in device memory (bumped by k_advance), so a captured
+// sequence of steps replays as a hipGraph without re-capture.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -29,21 +33,14 @@ namespace {
 
 enum AdvanceMode { kRebin = 0, kBegin = 1, kEndBegin = 2 };
 
-// FP64 observable accumulator layout
-enum ObsSlot {
-    O_V2 = 0,      // 3
-    O_MOM = 3,     // 3
-    O_VIR = 6,     // 9
-    O_PEP = 15,
-    O_PEB = 16,
-    O_PEA = 17,
-    O_N = 18
-};
+// FP64 accumulator slots per box
+enum ObsSlot { O_V2 = 0, O_MOM = 3, O_VIR = 6, O_PEP = 15, O_PEB = 16, O_PEA = 17, O_N = 18 };
+static_assert(O_VIR == kObsVir && O_PEP == kObsPEP, "accumulator layout");
 
 struct Topo {
-    // CSR by bead index
+    // CSR by bead index (all boxes)
     const int* bond_ptr;      // [n+1]
-    const int4* bond_ent;     // (other, is_head, bond type row, 0)
+    const int4* bond_ent;     // (other, is_head, 0, 0)
     const double2* bond_par;  // (k, l0) per entry
     const int* ang_ptr;       // [n+1]
     const int2* ang_ent;      // (angle, role)
@@ -53,17 +50,23 @@ struct Topo {
 };
 
 struct Buffers {
-    float4* pos;   // offset xyz, type bits
+    float4* pos;   // offset xyz, type | box << 8
     float4* vel;   // velocity xyz, id bits
     float4* fold;  // previous force
     int4* aux;     // image xyz, bead index
-    int* key;      // cell key
+    int* key;      // global cell key
 };
+
+template <bool MULTI>
+__device__ inline const BoxDev& box_of(const BoxDev* boxes, const BoxDev& B0, int w)
+{
+    return MULTI ? boxes[bead_box(w)] : B0;
+}
 
 __device__ inline void unwrapped_pos(int slot, const Buffers& S, const BoxDev& B, double u[3])
 {
     int c[3];
-    key_cell(S.key[slot], B, c);
+    key_cell(S.key[slot] - B.koff, B, c);
     const float4 p = S.pos[slot];
     const int4 a = S.aux[slot];
     u[0] = (double)(c[0] + a.x * B.n[0]) * B.wd[0] + (double)p.x;
@@ -71,11 +74,11 @@ __device__ inline void unwrapped_pos(int slot, const Buffers& S, const BoxDev& B
     u[2] = (double)(c[2] + a.z * B.n[2]) * B.wd[2] + (double)p.z;
 }
 
-// CBondPair::AddForce (src/BondPair.cpp:138-286) for one angle, FP64.  first = bond
+// CBondPair::AddForce (src/BondPair.cpp:138-286) for one angle in FP64.  first = bond
 // (b -> c), second = bond (a -> b); returns the force on the bead of `role`
 // (0: c = tail of first, 1: b = shared bead, 2: a = head of second).
-__device__ inline void angle_force(const double ua[3], const double ub[3], const double uc[3],
-                                   const double4 par, int role, double f[3], double* pe)
+__device__ inline void angle_force(const double ua[3], const double ub[3], const double uc[3], const double4 par,
+                                   int role, double f[3], double* pe)
 {
     double first[3], second[3];
     for (int d = 0; d < 3; ++d) { first[d] = ub[d] - uc[d]; second[d] = ua[d] - ub[d]; }
@@ -114,25 +117,24 @@ __device__ inline void angle_force(const double ua[3], const double ub[3], const
     }
 }
 
-// Bonded force on the bead in `slot` (index q) in FP64, plus (OBS) its share of the
-// bond virial (head bead only, as CBond::AddForce adds stress to the head) and the
-// bonded potential energies.
-__device__ inline void bonded_force(int slot, int q, const Buffers& S, const int* slot_of,
-                                    const Topo& T, const BoxDev& B, double f[3], double vir[9],
-                                    double* pe_b, double* pe_a, bool obs)
+// Bonded force on the bead in `slot` (index q) in FP64, plus (obs) its share of the bond
+// virial (head bead only: CBond::AddForce adds the stress to the head) and the bonded
+// potential energies.  Bonds never cross boxes, so the partners share B.
+__device__ inline void bonded_force(int slot, int q, const Buffers& S, const int* slot_of, const Topo& T,
+                                    const BoxDev& B, double f[3], double vir[9], double* pe_b, double* pe_a,
+                                    bool obs)
 {
     f[0] = f[1] = f[2] = 0.0;
-    double us[3];
-    bool have = false;
     if (T.has_bonds) {
         const int e0 = T.bond_ptr[q], e1 = T.bond_ptr[q + 1];
-        if (e0 < e1) { unwrapped_pos(slot, S, B, us); have = true; }
+        double us[3];
+        if (e0 < e1) unwrapped_pos(slot, S, B, us);
         for (int e = e0; e < e1; ++e) {
             const int4 ent = T.bond_ent[e];
             const double2 par = T.bond_par[e];
             double uo[3];
             unwrapped_pos(slot_of[ent.x], S, B, uo);
-            double d[3] = {us[0] - uo[0], us[1] - uo[1], us[2] - uo[2]};
+            const double d[3] = {us[0] - uo[0], us[1] - uo[1], us[2] - uo[2]};
             const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
             const double s = par.x * (par.y - L) / L;
             const double fb[3] = {s * d[0], s * d[1], s * d[2]};
@@ -159,81 +161,57 @@ __device__ inline void bonded_force(int slot, int q, const Buffers& S, const int
             if (obs && ent.y == 1) *pe_a += e_ang;
         }
     }
-    (void)have;
-}
-
-// block reduction of FP64 observables into acc[] with one atomic per slot per block
-template <int N>
-__device__ inline void block_accumulate(double (&v)[N], double* acc)
-{
-    __shared__ double red[N][4];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const double s = wave_sum(v[i]);
-        if (lane == 0) red[i][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < N) {
-        const int nw = (blockDim.x + 63) >> 6;
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
-        atomicAdd(acc + threadIdx.x, s);
-    }
 }
 
 // ---------------------------------------------------------------- kernels ------
 
 // Close the previous step (kEndBegin) and start the next one: new cell key and rank.
-template <int MODE, bool OBS>
+template <int MODE, bool MULTI>
 __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc,
-                                                 const int* __restrict__ slot_of, Topo T, BoxDev B,
-                                                 int* __restrict__ cnt, int* __restrict__ bcnt,
+                                                 const int* __restrict__ slot_of, Topo T, const BoxDev* boxes,
+                                                 BoxDev B0, int* __restrict__ cnt, int* __restrict__ bcnt,
                                                  int* __restrict__ nkey, int* __restrict__ rank,
-                                                 int* __restrict__ flags, double* __restrict__ acc)
+                                                 int* __restrict__ flags, int64_t* __restrict__ step)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    double ob[O_N];
-    if (OBS)
-        for (int i = 0; i < O_N; ++i) ob[i] = 0.0;
-    if (s < n) {
-        const float4 p = S.pos[s];
-        float4 v = S.vel[s];
-        float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
-        const int4 ax = S.aux[s];
-        if (MODE == kEndBegin) {
-            double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
-            bonded_force(s, ax.w, S, slot_of, T, B, fb, vir, &peb, &pea, OBS);
-            f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
-            const float4 fo = S.fold[s];
-            v.x = v.x + B.hdt * (f.x + fo.x);
-            v.y = v.y + B.hdt * (f.y + fo.y);
-            v.z = v.z + B.hdt * (f.z + fo.z);
-            S.vel[s] = v;
-            frc[s] = f;
-            if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
-                atomicAdd(flags + 0, 1);
-        }
-        int c[3];
-        key_cell(S.key[s], B, c);
-        const float o[3] = {p.x, p.y, p.z};
-        const int img[3] = {ax.x, ax.y, ax.z};
-        float d[3] = {0.f, 0.f, 0.f};
-        if (MODE != kRebin) {
-            d[0] = B.dt * v.x + B.hdt2 * f.x;
-            d[1] = B.dt * v.y + B.hdt2 * f.y;
-            d[2] = B.dt * v.z + B.hdt2 * f.z;
-        }
-        Moved m;
-        advance(o, c, img, d, B, m);
-        const int k = cell_key(m.c[0], m.c[1], m.c[2], B);
-        nkey[s] = k;
-        rank[s] = atomicAdd(cnt + k, 1);
-        atomicAdd(bcnt + (k >> 6), 1);
-        if (m.jump) atomicAdd(flags + 1, 1);
-        if (m.bad) atomicAdd(flags + 0, 1);
+    if (MODE != kRebin && s == 0) *step += 1;   // simulation time of the coming force evaluation
+    if (s >= n) return;
+    const float4 p = S.pos[s];
+    const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
+    float4 v = S.vel[s];
+    float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
+    const int4 ax = S.aux[s];
+    if (MODE == kEndBegin) {
+        double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
+        bonded_force(s, ax.w, S, slot_of, T, B, fb, vir, &peb, &pea, false);
+        f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
+        const float4 fo = S.fold[s];
+        v.x = v.x + B.hdt * (f.x + fo.x);
+        v.y = v.y + B.hdt * (f.y + fo.y);
+        v.z = v.z + B.hdt * (f.z + fo.z);
+        S.vel[s] = v;
+        frc[s] = f;
+        if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
+            atomicAdd(flags + 0, 1);
     }
-    if (OBS) block_accumulate(ob, acc);
+    int c[3];
+    key_cell(S.key[s] - B.koff, B, c);
+    const float o[3] = {p.x, p.y, p.z};
+    const int img[3] = {ax.x, ax.y, ax.z};
+    float d[3] = {0.f, 0.f, 0.f};
+    if (MODE != kRebin) {
+        d[0] = B.dt * v.x + B.hdt2 * f.x;
+        d[1] = B.dt * v.y + B.hdt2 * f.y;
+        d[2] = B.dt * v.z + B.hdt2 * f.z;
+    }
+    Moved m;
+    advance(o, c, img, d, B, m);
+    const int k = B.koff + cell_key(m.c[0], m.c[1], m.c[2], B);
+    nkey[s] = k;
+    rank[s] = atomicAdd(cnt + k, 1);
+    atomicAdd(bcnt + (k >> 6), 1);
+    if (m.jump) atomicAdd(flags + 1, 1);
+    if (m.bad) atomicAdd(flags + 0, 1);
 }
 
 // exclusive scan of brick counts (one workgroup), clears the counts
@@ -242,13 +220,13 @@ __global__ __launch_bounds__(1024) void k_brick_scan(int nbricks, int* __restric
     __shared__ int part[1024];
     const int t = threadIdx.x;
     const int per = (nbricks + 1023) / 1024;
-    const int b0 = t * per, b1 = min(nbricks, b0 + per);
+    const int b0 = min(nbricks, t * per), b1 = min(nbricks, b0 + per);
     int s = 0;
     for (int b = b0; b < b1; ++b) s += bcnt[b];
     part[t] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        int add = t >= off ? part[t - off] : 0;
+        const int add = t >= off ? part[t - off] : 0;
         __syncthreads();
         part[t] += add;
         __syncthreads();
@@ -283,21 +261,21 @@ __global__ __launch_bounds__(256) void k_cell_start(int nbricks, int n, const in
 }
 
 // scatter into the new sorted buffers (recomputes the same advance as k_advance)
-template <int MODE>
-__global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, const float4* __restrict__ frc,
-                                                 float4* __restrict__ vp, BoxDev B,
+template <int MODE, bool MULTI>
+__global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, float4* __restrict__ frc,
+                                                 float4* __restrict__ vp, const BoxDev* boxes, BoxDev B0,
                                                  const int* __restrict__ nkey, const int* __restrict__ rank,
-                                                 const int* __restrict__ cstart, int* __restrict__ slot_of,
-                                                 float4* __restrict__ frc_out)
+                                                 const int* __restrict__ cstart, int* __restrict__ slot_of)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const float4 p = S.pos[s];
+    const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
     const float4 v = S.vel[s];
     const float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
     const int4 ax = S.aux[s];
     int c[3];
-    key_cell(S.key[s], B, c);
+    key_cell(S.key[s] - B.koff, B, c);
     const float o[3] = {p.x, p.y, p.z};
     const int img[3] = {ax.x, ax.y, ax.z};
     float d[3] = {0.f, 0.f, 0.f};
@@ -317,24 +295,34 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, co
     D.key[dst] = k;
     if (MODE == kRebin) {
         vp[dst] = v;
-        frc_out[dst] = f;
     } else {
         vp[dst] = make_float4(v.x + B.lamdt * f.x, v.y + B.lamdt * f.y, v.z + B.lamdt * f.z, v.w);
     }
     slot_of[ax.w] = dst;
 }
 
+// REBIN leaves the uploaded force in fold; the canonical state needs it in frc too
+__global__ void k_copy_force(int n, const float4* __restrict__ fold, float4* __restrict__ frc)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) frc[s] = fold[s];
+}
+
 // Close the last step: bonded forces, UpdateMom (UPDATE_V) and FP64 observables.
-template <bool UPDATE_V, bool OBS>
+template <bool UPDATE_V, bool OBS, bool MULTI>
 __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __restrict__ frc,
-                                                const int* __restrict__ slot_of, Topo T, BoxDev B,
-                                                int* __restrict__ flags, double* __restrict__ acc)
+                                                const int* __restrict__ slot_of, Topo T, const BoxDev* boxes,
+                                                BoxDev B0, int* __restrict__ flags, double* __restrict__ acc)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double ob[O_N];
     if (OBS)
         for (int i = 0; i < O_N; ++i) ob[i] = 0.0;
+    int box = -1;
     if (s < n) {
+        const int w = __float_as_int(S.pos[s].w);
+        box = bead_box(w);
+        const BoxDev& B = box_of<MULTI>(boxes, B0, w);
         float4 f = frc[s];
         const int4 ax = S.aux[s];
         double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
@@ -363,19 +351,25 @@ __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __rest
             ob[O_PEA] = pea;
         }
     }
-    if (OBS) block_accumulate(ob, acc);
+    if (OBS) {
+        const int box0 = MULTI ? bead_box(__float_as_int(S.pos[blockIdx.x * blockDim.x].w)) : 0;
+        accumulate_boxes<O_N>(ob, MULTI ? box : (s < n ? 0 : -1), box0, 0, acc);
+    }
 }
 
 // state back to upload order in FP64
-__global__ __launch_bounds__(256) void k_gather(int n, Buffers S, const float4* __restrict__ frc, BoxDev B,
-                                                double* __restrict__ x, double* __restrict__ u,
-                                                double* __restrict__ v, double* __restrict__ f)
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_gather(int n, Buffers S, const float4* __restrict__ frc,
+                                                const BoxDev* boxes, BoxDev B0, double* __restrict__ x,
+                                                double* __restrict__ u, double* __restrict__ v,
+                                                double* __restrict__ f)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    int c[3];
-    key_cell(S.key[s], B, c);
     const float4 p = S.pos[s];
+    const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
+    int c[3];
+    key_cell(S.key[s] - B.koff, B, c);
     const int4 a = S.aux[s];
     const float4 vv = S.vel[s];
     const float4 ff = frc[s];
@@ -391,37 +385,52 @@ __global__ __launch_bounds__(256) void k_gather(int n, Buffers S, const float4* 
     f[3 * q + 0] = ff.x; f[3 * q + 1] = ff.y; f[3 * q + 2] = ff.z;
 }
 
-__global__ void k_zero_doubles(double* p, int n)
+// clears the FP64 accumulators and error flags (a kernel rather than a memset, so that
+// it can sit inside a captured graph)
+__global__ void k_zero(double* acc, int nacc, int* flags, int nflags)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
+    if (i < nacc) acc[i] = 0.0;
+    if (i < nflags) flags[i] = 0;
 }
 
-__global__ void k_zero_ints(int* p, int n)
+__global__ void k_set_step(int64_t* step, int64_t value)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step = value;
 }
 
 }  // namespace
 
 // ------------------------------------------------------------------ host side ---
 
-struct dpd_engine {
+struct BoxHost {
     dpd_params p{};
-    std::vector<double> a_h, g_h, a_backup, g_backup;
+    std::vector<double> a, g;
     bool thermostat_on = true, conservative_on = true;
+    double dt = 0, lambda = 0, kT = 0;
+    BoxDev B{};
+    // staged beads and topology (box-local indices)
+    int64_t n = 0;
+    std::vector<int32_t> id, type;
+    std::vector<double> x, u, v, f;
+    bool has_f = false;
+    std::vector<int32_t> bh, bt, pf, ps;
+    std::vector<double> bk, bl0, pk, pphi;
+    int64_t bead_off = 0;
+};
+
+struct dpd_engine {
+    std::vector<BoxHost> boxes;
     int device = 0;
     hipStream_t stream = nullptr;
-    BoxDev B{};
-    double dt = 0, lambda = 0, kT = 0;
-    uint64_t seed = 0;
     int nbricks = 0, nkeys = 0;
     int64_t n = 0;
     int64_t step = 0;
     int cur = 0;
     int pair_kernel = DPD_PAIR_FULLSHELL;
+    bool committed = false;
     // device buffers
+    BoxDev* dboxes = nullptr;
     float4 *pos[2] = {}, *vel[2] = {}, *fold[2] = {};
     int4* aux[2] = {};
     int* key[2] = {};
@@ -429,6 +438,7 @@ struct dpd_engine {
     int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
     int *cnt = nullptr, *bcnt = nullptr, *bstart = nullptr, *cstart = nullptr;
     int* flags = nullptr;
+    int64_t* dstep = nullptr;
     double* acc = nullptr;
     float* table = nullptr;
     double* gather = nullptr;   // 12n doubles for downloads
@@ -440,13 +450,15 @@ struct dpd_engine {
     int4* ang_bead = nullptr;
     double4* ang_par = nullptr;
     int has_bonds = 0, has_angles = 0;
-    int64_t nbonds = 0, nangles = 0;
     // observables
-    dpd_obs obs{};
+    std::vector<dpd_obs> obs;
     bool obs_valid = false;
     int64_t jumps = 0;
+    // graphs (steps, observe) -> executable graph; valid while buffers stay put
+    std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
+    std::map<std::pair<int64_t, int>, hipGraph_t> graph_src;
     // timing
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[2] = {};
     std::vector<hipEvent_t> pev;
     double t_total = 0, t_pair = 0;
     int64_t n_pair = 0;
@@ -478,36 +490,73 @@ int dalloc(dpd_engine* e, T** p, size_t count)
     return DPD_OK;
 }
 
-void set_step_constants(dpd_engine* e)
+void drop_graphs(dpd_engine* e)
 {
-    // CCNTCell::SetTimeStepConstants / ChangeTimeStepConstants (src/CNTCell.cpp:180-203, :357-377)
-    e->B.dt = (float)e->dt;
-    e->B.hdt = (float)(0.5 * e->dt);
-    e->B.hdt2 = (float)(0.5 * e->dt * e->dt);
-    e->B.lamdt = (float)(e->lambda * e->dt);
-    e->B.invrootdt = (float)std::sqrt(24.0 * e->kT / e->dt);
+    for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : e->graph_src) (void)hipGraphDestroy(g.second);
+    e->graphs.clear();
+    e->graph_src.clear();
 }
 
-int upload_table(dpd_engine* e)
+void set_step_constants(BoxHost& b)
 {
-    const int nt = e->p.ntypes;
-    std::vector<float> t(3 * nt * nt);
-    for (int i = 0; i < nt * nt; ++i) {
-        const double a = e->conservative_on ? e->a_h[i] : 0.0;
-        const double g = e->thermostat_on ? e->g_h[i] : 0.0;
-        t[i] = (float)a;
-        t[nt * nt + i] = (float)g;
-        t[2 * nt * nt + i] = (float)std::sqrt(g);
+    // CCNTCell::SetTimeStepConstants / ChangeTimeStepConstants (src/CNTCell.cpp:180-203, :357-377)
+    b.B.dt = (float)b.dt;
+    b.B.hdt = (float)(0.5 * b.dt);
+    b.B.hdt2 = (float)(0.5 * b.dt * b.dt);
+    b.B.lamdt = (float)(b.lambda * b.dt);
+    b.B.invrootdt = (float)std::sqrt(24.0 * b.kT / b.dt);
+}
+
+int check_params(dpd_engine* e, const dpd_params* p)
+{
+    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(e, DPD_ERR_PARAM, "need at least 3 cells per axis");
+    if (p->wx < 1.0 || p->wy < 1.0 || p->wz < 1.0) return fail(e, DPD_ERR_PARAM, "cell widths must be >= 1");
+    if (p->ntypes < 1 || p->ntypes > DPD_MAX_TYPES) return fail(e, DPD_ERR_PARAM, "ntypes out of range");
+    if (!(p->dt > 0.0) || p->lambda < 0.0 || p->lambda > 1.0 || p->kT < 0.0)
+        return fail(e, DPD_ERR_PARAM, "invalid dt, lambda or kT");
+    if (!p->a || !p->gamma) return fail(e, DPD_ERR_PARAM, "null interaction matrices");
+    const int nt = p->ntypes;
+    for (int i = 0; i < nt; ++i)
+        for (int j = 0; j < nt; ++j)
+            if (p->a[i * nt + j] != p->a[j * nt + i] || p->gamma[i * nt + j] != p->gamma[j * nt + i] ||
+                p->gamma[i * nt + j] < 0.0)
+                return fail(e, DPD_ERR_PARAM, "interaction matrices must be symmetric, gamma >= 0");
+    return DPD_OK;
+}
+
+int upload_tables(dpd_engine* e)
+{
+    const int nb = (int)e->boxes.size();
+    std::vector<float> t((size_t)nb * kTableStride, 0.0f);
+    for (int b = 0; b < nb; ++b) {
+        const BoxHost& h = e->boxes[b];
+        const int nt = h.p.ntypes;
+        float* tb = t.data() + (size_t)b * kTableStride;
+        for (int i = 0; i < nt; ++i)
+            for (int j = 0; j < nt; ++j) {
+                const double a = h.conservative_on ? h.a[i * nt + j] : 0.0;
+                const double g = h.thermostat_on ? h.g[i * nt + j] : 0.0;
+                tb[i * kMaxTypes + j] = (float)a;
+                tb[kTypeStride + i * kMaxTypes + j] = (float)g;
+                tb[2 * kTypeStride + i * kMaxTypes + j] = (float)std::sqrt(g);
+            }
     }
     HIPCHK(hipMemcpyAsync(e->table, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return DPD_OK;
 }
 
-Buffers bufs(dpd_engine* e, int i)
+int upload_boxes(dpd_engine* e)
 {
-    return Buffers{e->pos[i], e->vel[i], e->fold[i], e->aux[i], e->key[i]};
+    std::vector<BoxDev> bd;
+    for (auto& b : e->boxes) bd.push_back(b.B);
+    HIPCHK(hipMemcpyAsync(e->dboxes, bd.data(), sizeof(BoxDev) * bd.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DPD_OK;
 }
+
+Buffers bufs(dpd_engine* e, int i) { return Buffers{e->pos[i], e->vel[i], e->fold[i], e->aux[i], e->key[i]}; }
 
 Topo topo(dpd_engine* e)
 {
@@ -515,34 +564,31 @@ Topo topo(dpd_engine* e)
                 e->has_bonds, e->has_angles};
 }
 
-void set_rng_step(dpd_engine* e, int64_t step)
-{
-    e->B.step0 = (uint32_t)(uint64_t)step;
-    e->B.step1 = (uint32_t)((uint64_t)step >> 32);
-}
-
 inline int grid(int64_t n, int b) { return (int)((n + b - 1) / b); }
+inline bool multi(dpd_engine* e) { return e->boxes.size() > 1; }
 
-// counting sort of buffer `src` into buffer `src^1`; MODE decides the move
-template <int MODE>
-int sort_step(dpd_engine* e, bool obs)
+// counting sort of buffer `cur` into buffer `cur^1`; MODE decides the move
+template <int MODE, bool MULTI>
+void sort_step_t(dpd_engine* e)
 {
     const int n = (int)e->n;
-    const int src = e->cur, dst = e->cur ^ 1;
-    Buffers S = bufs(e, src), D = bufs(e, dst);
-    Topo T = topo(e);
-    if (obs)
-        hipLaunchKernelGGL((k_advance<MODE, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
-                           e->slot_of, T, e->B, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->acc);
-    else
-        hipLaunchKernelGGL((k_advance<MODE, false>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
-                           e->slot_of, T, e->B, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->acc);
+    Buffers S = bufs(e, e->cur), D = bufs(e, e->cur ^ 1);
+    const BoxDev B0 = e->boxes[0].B;
+    hipLaunchKernelGGL((k_advance<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
+                       e->slot_of, topo(e), e->dboxes, B0, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->dstep);
     hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, e->stream, e->nbricks, e->bcnt, e->bstart);
-    hipLaunchKernelGGL(k_cell_start, dim3(grid(e->nbricks, 4)), dim3(256), 0, e->stream, e->nbricks, n,
-                       e->bstart, e->cnt, e->cstart);
-    hipLaunchKernelGGL((k_permute<MODE>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, D, e->frc, e->vp,
-                       e->B, e->nkey, e->rank, e->cstart, e->slot_of, e->frc);
-    e->cur = dst;
+    hipLaunchKernelGGL(k_cell_start, dim3(grid(e->nbricks, 4)), dim3(256), 0, e->stream, e->nbricks, n, e->bstart,
+                       e->cnt, e->cstart);
+    hipLaunchKernelGGL((k_permute<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, D, e->frc,
+                       e->vp, e->dboxes, B0, e->nkey, e->rank, e->cstart, e->slot_of);
+    e->cur ^= 1;
+}
+
+template <int MODE>
+int sort_step(dpd_engine* e)
+{
+    if (multi(e)) sort_step_t<MODE, true>(e);
+    else sort_step_t<MODE, false>(e);
     HIPCHK(hipGetLastError());
     return DPD_OK;
 }
@@ -551,17 +597,41 @@ int launch_pair(dpd_engine* e, bool obs)
 {
     PairArgs a;
     a.n = (int)e->n;
+    a.nbox = (int)e->boxes.size();
+    a.nbricks = e->nbricks;
+    a.boxes = e->dboxes;
     a.pos = e->pos[e->cur];
     a.vp = e->vp;
     a.key = e->key[e->cur];
     a.cstart = e->cstart;
     a.table = e->table;
+    a.step = e->dstep;
     a.frc = e->frc;
-    a.acc = e->acc + O_VIR;
-    a.nbricks = e->nbricks;
-    return launch_pair_kernel(e->pair_kernel, a, e->B, obs, e->stream) == hipSuccess
+    a.acc = e->acc;
+    return launch_pair_kernel(e->pair_kernel, a, e->boxes[0].B, obs, e->stream) == hipSuccess
                ? DPD_OK
                : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
+}
+
+template <bool UPDATE_V, bool OBS>
+void launch_finish(dpd_engine* e)
+{
+    const int n = (int)e->n;
+    const BoxDev B0 = e->boxes[0].B;
+    if (multi(e))
+        hipLaunchKernelGGL((k_finish<UPDATE_V, OBS, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n,
+                           bufs(e, e->cur), e->frc, e->slot_of, topo(e), e->dboxes, B0, e->flags, e->acc);
+    else
+        hipLaunchKernelGGL((k_finish<UPDATE_V, OBS, false>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n,
+                           bufs(e, e->cur), e->frc, e->slot_of, topo(e), e->dboxes, B0, e->flags, e->acc);
+}
+
+// clear the observable accumulators (and, when `flags`, the error flags)
+void launch_zero(dpd_engine* e, bool flags)
+{
+    const int nacc = (int)e->boxes.size() * kObsStride;
+    hipLaunchKernelGGL(k_zero, dim3(grid(std::max(nacc, 4), 256)), dim3(256), 0, e->stream, e->acc, nacc, e->flags,
+                       flags ? 4 : 0);
 }
 
 int read_flags(dpd_engine* e)
@@ -574,166 +644,81 @@ int read_flags(dpd_engine* e)
     return DPD_OK;
 }
 
-int zero_acc(dpd_engine* e)
+int finish_obs(dpd_engine* e)
 {
-    hipLaunchKernelGGL(k_zero_doubles, dim3(1), dim3(64), 0, e->stream, e->acc, 32);
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, e->stream, e->flags, 4);
-    HIPCHK(hipGetLastError());
-    return DPD_OK;
-}
-
-int finish_obs(dpd_engine* e, bool update_v)
-{
-    (void)update_v;
-    double h[32];
-    HIPCHK(hipMemcpyAsync(h, e->acc, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    const size_t nb = e->boxes.size();
+    std::vector<double> h(nb * kObsStride);
+    HIPCHK(hipMemcpyAsync(h.data(), e->acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    dpd_obs& o = e->obs;
-    std::memset(&o, 0, sizeof(o));
-    o.step = e->step;
-    o.nbeads = e->n;
-    const double V = (double)e->p.nx * e->p.wx * (double)e->p.ny * e->p.wy * (double)e->p.nz * e->p.wz;
-    for (int d = 0; d < 3; ++d) { o.sum_v2[d] = h[O_V2 + d]; o.momentum[d] = h[O_MOM + d]; }
-    for (int i = 0; i < 9; ++i) o.stress[i] = h[O_VIR + i] / V;
-    const double v2 = o.sum_v2[0] + o.sum_v2[1] + o.sum_v2[2];
-    o.kT = v2 / (3.0 * (double)e->n);
-    // src/Monitor.cpp:981-982 keeps the 10-digit constant
-    o.pressure = 0.3333333333 * (1.0 / V) * (v2 + h[O_VIR + 0] + h[O_VIR + 4] + h[O_VIR + 8]);
-    o.pe_pair = h[O_PEP];
-    o.pe_bond = h[O_PEB];
-    o.pe_angle = h[O_PEA];
-    o.jumps = e->jumps;
+    e->obs.assign(nb, dpd_obs{});
+    for (size_t b = 0; b < nb; ++b) {
+        const double* a = h.data() + b * kObsStride;
+        const dpd_params& p = e->boxes[b].p;
+        dpd_obs& o = e->obs[b];
+        o.step = e->step;
+        o.nbeads = e->boxes[b].n;
+        const double V = (double)p.nx * p.wx * (double)p.ny * p.wy * (double)p.nz * p.wz;
+        for (int d = 0; d < 3; ++d) { o.sum_v2[d] = a[O_V2 + d]; o.momentum[d] = a[O_MOM + d]; }
+        for (int i = 0; i < 9; ++i) o.stress[i] = a[O_VIR + i] / V;
+        const double v2 = o.sum_v2[0] + o.sum_v2[1] + o.sum_v2[2];
+        o.kT = o.nbeads ? v2 / (3.0 * (double)o.nbeads) : 0.0;
+        // src/Monitor.cpp:981-982 keeps the 10-digit constant
+        o.pressure = 0.3333333333 * (1.0 / V) * (v2 + a[O_VIR + 0] + a[O_VIR + 4] + a[O_VIR + 8]);
+        o.pe_pair = a[O_PEP];
+        o.pe_bond = a[O_PEB];
+        o.pe_angle = a[O_PEA];
+        o.jumps = e->jumps;
+    }
     e->obs_valid = true;
     return DPD_OK;
 }
 
-}  // namespace
-
-// ------------------------------------------------------------------ C ABI ------
-
-extern "C" {
-
-const char* dpd_last_error(const dpd_engine* e) { return e ? e->err.c_str() : "null engine"; }
-
-int dpd_create(const dpd_params* p, dpd_engine** out)
+// the steps of one dpd_step call, enqueued on the stream (capturable)
+int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, bool timing)
 {
-    if (!out) return DPD_ERR_PARAM;
-    *out = nullptr;
-    if (!p) return DPD_ERR_PARAM;
-    dpd_engine* e = new dpd_engine();
-    *out = e;
-    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(e, DPD_ERR_PARAM, "need at least 3 cells per axis");
-    if (p->wx < 1.0 || p->wy < 1.0 || p->wz < 1.0) return fail(e, DPD_ERR_PARAM, "cell widths must be >= 1");
-    if (p->ntypes < 1 || p->ntypes > DPD_MAX_TYPES) return fail(e, DPD_ERR_PARAM, "ntypes out of range");
-    if (!(p->dt > 0.0) || p->lambda < 0.0 || p->lambda > 1.0 || p->kT < 0.0)
-        return fail(e, DPD_ERR_PARAM, "invalid dt, lambda or kT");
-    if (!p->a || !p->gamma) return fail(e, DPD_ERR_PARAM, "null interaction matrices");
-    e->p = *p;
-    const int nt = p->ntypes;
-    e->a_h.assign(p->a, p->a + nt * nt);
-    e->g_h.assign(p->gamma, p->gamma + nt * nt);
-    for (int i = 0; i < nt; ++i)
-        for (int j = 0; j < nt; ++j)
-            if (e->a_h[i * nt + j] != e->a_h[j * nt + i] || e->g_h[i * nt + j] != e->g_h[j * nt + i] ||
-                e->g_h[i * nt + j] < 0.0)
-                return fail(e, DPD_ERR_PARAM, "interaction matrices must be symmetric, gamma >= 0");
-    e->p.a = e->a_h.data();
-    e->p.gamma = e->g_h.data();
-    e->device = p->device;
-    HIPCHK(hipSetDevice(e->device));
-    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    for (auto& ev : e->ev) HIPCHK(hipEventCreate(&ev));
-    e->dt = p->dt; e->lambda = p->lambda; e->kT = p->kT; e->seed = p->seed;
-    BoxDev& B = e->B;
-    B.n[0] = p->nx; B.n[1] = p->ny; B.n[2] = p->nz;
-    for (int d = 0; d < 3; ++d) B.nb[d] = (B.n[d] + kBrick - 1) / kBrick;
-    B.wd[0] = p->wx; B.wd[1] = p->wy; B.wd[2] = p->wz;
-    for (int d = 0; d < 3; ++d) B.w[d] = (float)B.wd[d];
-    B.ntypes = nt;
-    B.seed0 = (uint32_t)p->seed;
-    B.seed1 = (uint32_t)(p->seed >> 32);
-    set_step_constants(e);
-    e->nbricks = B.nb[0] * B.nb[1] * B.nb[2];
-    e->nkeys = e->nbricks * kBrickCells;
-    e->pair_kernel = (p->pair_kernel == DPD_PAIR_AUTO) ? default_pair_kernel() : p->pair_kernel;
-    if (e->pair_kernel != DPD_PAIR_BRICK && e->pair_kernel != DPD_PAIR_FULLSHELL)
-        return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
     int rc;
-    if ((rc = dalloc(e, &e->cnt, e->nkeys)) || (rc = dalloc(e, &e->cstart, e->nkeys + 1)) ||
-        (rc = dalloc(e, &e->bcnt, e->nbricks)) || (rc = dalloc(e, &e->bstart, e->nbricks)) ||
-        (rc = dalloc(e, &e->flags, 4)) || (rc = dalloc(e, &e->acc, 32)) || (rc = dalloc(e, &e->table, 3 * nt * nt)))
-        return rc;
-    HIPCHK(hipMemsetAsync(e->cnt, 0, sizeof(int) * e->nkeys, e->stream));
-    HIPCHK(hipMemsetAsync(e->bcnt, 0, sizeof(int) * e->nbricks, e->stream));
-    HIPCHK(hipMemsetAsync(e->flags, 0, sizeof(int) * 4, e->stream));
-    HIPCHK(hipMemsetAsync(e->acc, 0, sizeof(double) * 32, e->stream));
-    if ((rc = upload_table(e))) return rc;
-    e->err.clear();
+    launch_zero(e, false);
+    for (int64_t t = 0; t < nsteps; ++t) {
+        rc = (t == 0) ? sort_step<kBegin>(e) : sort_step<kEndBegin>(e);
+        if (rc) return rc;
+        if (timing) HIPCHK(hipEventRecord(e->pev[2 * t], e->stream));
+        if ((rc = launch_pair(e, obs && t == nsteps - 1))) return rc;
+        if (timing) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
+    }
+    if (obs) launch_finish<true, true>(e);
+    else launch_finish<true, false>(e);
+    HIPCHK(hipGetLastError());
     return DPD_OK;
 }
 
-void dpd_destroy(dpd_engine* e)
+// Assemble the staged boxes into device buffers and sort them.
+int commit_beads(dpd_engine* e, int64_t step0)
 {
-    if (!e) return;
-    if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->aux[0], e->aux[1],
-                    e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
-                    e->bstart, e->cstart, e->flags, e->acc, e->table, e->gather, e->bond_ptr, e->ang_ptr,
-                    e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par};
-    for (void* q : ptrs)
-        if (q) (void)hipFree(q);
-    for (auto& ev : e->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    for (auto& ev : e->pev) (void)hipEventDestroy(ev);
-    if (e->stream) (void)hipStreamDestroy(e->stream);
-    delete e;
-}
-
-int dpd_upload_beads(dpd_engine* e, int64_t n, const int32_t* id, const int32_t* type, const double* x,
-                     const double* unpbc, const double* v, const double* f, int64_t step0)
-{
-    if (!e) return DPD_ERR_PARAM;
-    if (n < 1 || n > (int64_t)1 << 30) return fail(e, DPD_ERR_PARAM, "bead count out of range");
-    if (!id || !type || !x || !v) return fail(e, DPD_ERR_PARAM, "null bead array");
-    HIPCHK(hipSetDevice(e->device));
-    const BoxDev& B = e->B;
-    const double L[3] = {B.n[0] * B.wd[0], B.n[1] * B.wd[1], B.n[2] * B.wd[2]};
-    std::vector<float4> hp(n), hv(n), hf(n);
-    std::vector<int4> ha(n);
-    std::vector<int> hk(n);
-    for (int64_t i = 0; i < n; ++i) {
-        if (type[i] < 0 || type[i] >= e->p.ntypes) return fail(e, DPD_ERR_PARAM, "bead type out of range");
-        int c[3], im[3];
-        float o[3];
-        for (int d = 0; d < 3; ++d) {
-            const double xd = x[3 * i + d];
-            if (!(xd >= 0.0 && xd < L[d]))
-                return fail(e, DPD_ERR_OUT_OF_BOX, "bead " + std::to_string(i) + " outside the box");
-            int ci = (int)std::floor(xd / B.wd[d]);
-            ci = std::min(std::max(ci, 0), B.n[d] - 1);
-            double od = xd - ci * B.wd[d];
-            float of = (float)od;
-            if (of >= B.w[d]) of = B.w[d] * 0.99999994f;
-            if (of < 0.f) of = 0.f;
-            c[d] = ci;
-            o[d] = of;
-            const double ud = unpbc ? unpbc[3 * i + d] : xd;
-            im[d] = (int)std::lround((ud - xd) / L[d]);
-        }
-        float tb;
-        int32_t ty = type[i];
-        std::memcpy(&tb, &ty, 4);
-        float ib;
-        int32_t idv = id[i];
-        std::memcpy(&ib, &idv, 4);
-        hp[i] = make_float4(o[0], o[1], o[2], tb);
-        hv[i] = make_float4((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2], ib);
-        hf[i] = f ? make_float4((float)f[3 * i], (float)f[3 * i + 1], (float)f[3 * i + 2], 0.f)
-                  : make_float4(0.f, 0.f, 0.f, 0.f);
-        ha[i] = make_int4(im[0], im[1], im[2], (int)i);
-        hk[i] = cell_key(c[0], c[1], c[2], B);
+    int64_t n = 0;
+    int nbr = 0;
+    for (size_t b = 0; b < e->boxes.size(); ++b) {
+        BoxHost& h = e->boxes[b];
+        if (h.n == 0) return fail(e, DPD_ERR_STATE, "box " + std::to_string(b) + " has no beads");
+        h.bead_off = n;
+        h.B.koff = nbr * kBrickCells;
+        h.B.boff = nbr;
+        h.B.box = (int)b;
+        nbr += h.B.nbricks;
+        n += h.n;
     }
+    if (n > ((int64_t)1 << 30) || (int64_t)nbr * kBrickCells > ((int64_t)1 << 30))
+        return fail(e, DPD_ERR_PARAM, "batch too large");
+    drop_graphs(e);
     int rc;
+    if (nbr != e->nbricks) {
+        e->nbricks = nbr;
+        e->nkeys = nbr * kBrickCells;
+        if ((rc = dalloc(e, &e->cnt, e->nkeys)) || (rc = dalloc(e, &e->cstart, e->nkeys + 1)) ||
+            (rc = dalloc(e, &e->bcnt, e->nbricks)) || (rc = dalloc(e, &e->bstart, e->nbricks)))
+            return rc;
+        HIPCHK(hipMemsetAsync(e->cnt, 0, sizeof(int) * e->nkeys, e->stream));
+        HIPCHK(hipMemsetAsync(e->bcnt, 0, sizeof(int) * e->nbricks, e->stream));
+    }
     if (n != e->n) {
         for (int b = 0; b < 2; ++b)
             if ((rc = dalloc(e, &e->pos[b], n)) || (rc = dalloc(e, &e->vel[b], n)) ||
@@ -744,8 +729,41 @@ int dpd_upload_beads(dpd_engine* e, int64_t n, const int32_t* id, const int32_t*
             (rc = dalloc(e, &e->nkey, n)) || (rc = dalloc(e, &e->rank, n)) || (rc = dalloc(e, &e->gather, 12 * n)))
             return rc;
         e->n = n;
-        // topology refers to bead indices: a new bead count invalidates it
-        e->has_bonds = e->has_angles = 0;
+    }
+    if ((rc = upload_boxes(e))) return rc;
+    std::vector<float4> hp(n), hv(n), hf(n);
+    std::vector<int4> ha(n);
+    std::vector<int> hk(n);
+    for (size_t b = 0; b < e->boxes.size(); ++b) {
+        const BoxHost& h = e->boxes[b];
+        const BoxDev& B = h.B;
+        const double L[3] = {B.n[0] * B.wd[0], B.n[1] * B.wd[1], B.n[2] * B.wd[2]};
+        for (int64_t i = 0; i < h.n; ++i) {
+            const int64_t g = h.bead_off + i;
+            int c[3], im[3];
+            float o[3];
+            for (int d = 0; d < 3; ++d) {
+                const double xd = h.x[3 * i + d];
+                int ci = (int)std::floor(xd / B.wd[d]);
+                ci = std::min(std::max(ci, 0), B.n[d] - 1);
+                float of = (float)(xd - ci * B.wd[d]);
+                if (of >= B.w[d]) of = B.w[d] * 0.99999994f;
+                if (of < 0.f) of = 0.f;
+                c[d] = ci;
+                o[d] = of;
+                im[d] = (int)std::lround((h.u[3 * i + d] - xd) / L[d]);
+            }
+            const int32_t w = h.type[i] | ((int32_t)b << 8);
+            float wb, ib;
+            std::memcpy(&wb, &w, 4);
+            std::memcpy(&ib, &h.id[i], 4);
+            hp[g] = make_float4(o[0], o[1], o[2], wb);
+            hv[g] = make_float4((float)h.v[3 * i], (float)h.v[3 * i + 1], (float)h.v[3 * i + 2], ib);
+            hf[g] = h.has_f ? make_float4((float)h.f[3 * i], (float)h.f[3 * i + 1], (float)h.f[3 * i + 2], 0.f)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+            ha[g] = make_int4(im[0], im[1], im[2], (int)g);
+            hk[g] = B.koff + cell_key(c[0], c[1], c[2], B);
+        }
     }
     const int src = e->cur;
     HIPCHK(hipMemcpyAsync(e->pos[src], hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice, e->stream));
@@ -753,66 +771,66 @@ int dpd_upload_beads(dpd_engine* e, int64_t n, const int32_t* id, const int32_t*
     HIPCHK(hipMemcpyAsync(e->fold[src], hf.data(), sizeof(float4) * n, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->aux[src], ha.data(), sizeof(int4) * n, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->key[src], hk.data(), sizeof(int) * n, hipMemcpyHostToDevice, e->stream));
-    if ((rc = zero_acc(e))) return rc;
-    if ((rc = sort_step<kRebin>(e, false))) return rc;
+    launch_zero(e, true);
+    if ((rc = sort_step<kRebin>(e))) return rc;
+    hipLaunchKernelGGL(k_copy_force, dim3(grid(n, 256)), dim3(256), 0, e->stream, (int)n, e->fold[e->cur], e->frc);
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, step0);
+    HIPCHK(hipGetLastError());
     e->step = step0;
     e->jumps = 0;
     e->obs_valid = false;
+    e->committed = true;
     HIPCHK(hipStreamSynchronize(e->stream));
-    return DPD_OK;
+    return read_flags(e);
 }
 
-int dpd_upload_topology(dpd_engine* e, int64_t nbonds, const int32_t* head, const int32_t* tail, const double* k,
-                        const double* l0, int64_t npairs, const int32_t* first_bond, const int32_t* second_bond,
-                        const double* modulus, const double* phi0)
+// Global CSR topology from the staged per-box bonds and bond pairs.
+int commit_topology(dpd_engine* e)
 {
-    if (!e) return DPD_ERR_PARAM;
-    if (e->n == 0) return fail(e, DPD_ERR_STATE, "upload beads before topology");
-    if (nbonds < 0 || npairs < 0) return fail(e, DPD_ERR_PARAM, "negative counts");
-    if (nbonds && (!head || !tail || !k || !l0)) return fail(e, DPD_ERR_PARAM, "null bond array");
-    if (npairs && (!first_bond || !second_bond || !modulus || !phi0)) return fail(e, DPD_ERR_PARAM, "null pair array");
-    HIPCHK(hipSetDevice(e->device));
     const int64_t n = e->n;
-    for (int64_t b = 0; b < nbonds; ++b)
-        if (head[b] < 0 || head[b] >= n || tail[b] < 0 || tail[b] >= n || head[b] == tail[b])
-            return fail(e, DPD_ERR_PARAM, "bond bead index out of range");
-    for (int64_t q = 0; q < npairs; ++q) {
-        if (first_bond[q] < 0 || first_bond[q] >= nbonds || second_bond[q] < 0 || second_bond[q] >= nbonds)
-            return fail(e, DPD_ERR_PARAM, "bond pair index out of range");
-        if (tail[second_bond[q]] != head[first_bond[q]])
-            return fail(e, DPD_ERR_PARAM, "bond pair bonds do not share a bead");
-    }
-    // CSR by bead
     std::vector<int> bp(n + 1, 0), ap(n + 1, 0);
-    for (int64_t b = 0; b < nbonds; ++b) { bp[head[b] + 1]++; bp[tail[b] + 1]++; }
+    std::vector<int> H, Tl, F, Sd;
+    std::vector<double> K, L0, M, P;
+    for (const BoxHost& h : e->boxes) {
+        const int off = (int)h.bead_off;
+        const int boff = (int)H.size();
+        for (size_t b = 0; b < h.bh.size(); ++b) {
+            H.push_back(h.bh[b] + off); Tl.push_back(h.bt[b] + off);
+            K.push_back(h.bk[b]); L0.push_back(h.bl0[b]);
+        }
+        for (size_t q = 0; q < h.pf.size(); ++q) {
+            F.push_back(h.pf[q] + boff); Sd.push_back(h.ps[q] + boff);
+            M.push_back(h.pk[q]); P.push_back(h.pphi[q]);
+        }
+    }
+    const int64_t nbonds = (int64_t)H.size(), npairs = (int64_t)F.size();
+    for (int64_t b = 0; b < nbonds; ++b) { bp[H[b] + 1]++; bp[Tl[b] + 1]++; }
     std::vector<int4> abead(std::max<int64_t>(npairs, 1));
     std::vector<double4> apar(std::max<int64_t>(npairs, 1));
     for (int64_t q = 0; q < npairs; ++q) {
-        const int fb = first_bond[q], sb = second_bond[q];
-        const int a = head[sb], bm = head[fb], c = tail[fb];
+        const int a = H[Sd[q]], bm = H[F[q]], c = Tl[F[q]];
         abead[q] = make_int4(a, bm, c, 0);
-        apar[q] = make_double4(modulus[q], phi0[q], std::cos(phi0[q]), std::sin(phi0[q]));
+        apar[q] = make_double4(M[q], P[q], std::cos(P[q]), std::sin(P[q]));
         ap[a + 1]++; ap[bm + 1]++; ap[c + 1]++;
     }
     for (int64_t i = 0; i < n; ++i) { bp[i + 1] += bp[i]; ap[i + 1] += ap[i]; }
     std::vector<int4> bent(std::max<int>(bp[n], 1));
     std::vector<double2> bpar(std::max<int>(bp[n], 1));
     std::vector<int2> aent(std::max<int>(ap[n], 1));
-    {
-        std::vector<int> fill(bp.begin(), bp.end() - 1);
-        for (int64_t b = 0; b < nbonds; ++b) {
-            const int h = head[b], t = tail[b];
-            bent[fill[h]] = make_int4(t, 1, 0, 0); bpar[fill[h]++] = make_double2(k[b], l0[b]);
-            bent[fill[t]] = make_int4(h, 0, 0, 0); bpar[fill[t]++] = make_double2(k[b], l0[b]);
-        }
-        std::vector<int> afill(ap.begin(), ap.end() - 1);
-        for (int64_t q = 0; q < npairs; ++q) {
-            const int4 abc = abead[q];
-            aent[afill[abc.z]++] = make_int2((int)q, 0);
-            aent[afill[abc.y]++] = make_int2((int)q, 1);
-            aent[afill[abc.x]++] = make_int2((int)q, 2);
-        }
+    std::vector<int> fill(bp.begin(), bp.end() - 1);
+    for (int64_t b = 0; b < nbonds; ++b) {
+        const int h = H[b], t = Tl[b];
+        bent[fill[h]] = make_int4(t, 1, 0, 0); bpar[fill[h]++] = make_double2(K[b], L0[b]);
+        bent[fill[t]] = make_int4(h, 0, 0, 0); bpar[fill[t]++] = make_double2(K[b], L0[b]);
     }
+    std::vector<int> afill(ap.begin(), ap.end() - 1);
+    for (int64_t q = 0; q < npairs; ++q) {
+        const int4 abc = abead[q];
+        aent[afill[abc.z]++] = make_int2((int)q, 0);
+        aent[afill[abc.y]++] = make_int2((int)q, 1);
+        aent[afill[abc.x]++] = make_int2((int)q, 2);
+    }
+    drop_graphs(e);
     int rc;
     if ((rc = dalloc(e, &e->bond_ptr, n + 1)) || (rc = dalloc(e, &e->ang_ptr, n + 1)) ||
         (rc = dalloc(e, &e->bond_ent, bent.size())) || (rc = dalloc(e, &e->bond_par, bpar.size())) ||
@@ -829,22 +847,204 @@ int dpd_upload_topology(dpd_engine* e, int64_t nbonds, const int32_t* head, cons
     HIPCHK(hipStreamSynchronize(e->stream));
     e->has_bonds = nbonds > 0;
     e->has_angles = npairs > 0;
-    e->nbonds = nbonds;
-    e->nangles = npairs;
     return DPD_OK;
 }
+
+int stage_beads(dpd_engine* e, int32_t box, int64_t n, const int32_t* id, const int32_t* type, const double* x,
+                const double* unpbc, const double* v, const double* f)
+{
+    if (box < 0 || box >= (int32_t)e->boxes.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    if (n < 1 || n > ((int64_t)1 << 30)) return fail(e, DPD_ERR_PARAM, "bead count out of range");
+    if (!id || !type || !x || !v) return fail(e, DPD_ERR_PARAM, "null bead array");
+    BoxHost& h = e->boxes[box];
+    const BoxDev& B = h.B;
+    const double L[3] = {B.n[0] * B.wd[0], B.n[1] * B.wd[1], B.n[2] * B.wd[2]};
+    for (int64_t i = 0; i < n; ++i) {
+        if (type[i] < 0 || type[i] >= h.p.ntypes) return fail(e, DPD_ERR_PARAM, "bead type out of range");
+        for (int d = 0; d < 3; ++d)
+            if (!(x[3 * i + d] >= 0.0 && x[3 * i + d] < L[d]))
+                return fail(e, DPD_ERR_OUT_OF_BOX, "bead " + std::to_string(i) + " outside the box");
+    }
+    if (h.n != n) { h.bh.clear(); h.bt.clear(); h.pf.clear(); h.ps.clear(); }   // topology refers to indices
+    h.n = n;
+    h.id.assign(id, id + n);
+    h.type.assign(type, type + n);
+    h.x.assign(x, x + 3 * n);
+    h.u.assign(unpbc ? unpbc : x, (unpbc ? unpbc : x) + 3 * n);
+    h.v.assign(v, v + 3 * n);
+    h.has_f = f != nullptr;
+    h.f.assign(f ? f : x, (f ? f : x) + (f ? 3 * n : 0));
+    return DPD_OK;
+}
+
+int stage_topology(dpd_engine* e, int32_t box, int64_t nbonds, const int32_t* head, const int32_t* tail,
+                   const double* k, const double* l0, int64_t npairs, const int32_t* first, const int32_t* second,
+                   const double* modulus, const double* phi0)
+{
+    if (box < 0 || box >= (int32_t)e->boxes.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    BoxHost& h = e->boxes[box];
+    if (h.n == 0) return fail(e, DPD_ERR_STATE, "upload beads before topology");
+    if (nbonds < 0 || npairs < 0) return fail(e, DPD_ERR_PARAM, "negative counts");
+    if (nbonds && (!head || !tail || !k || !l0)) return fail(e, DPD_ERR_PARAM, "null bond array");
+    if (npairs && (!first || !second || !modulus || !phi0)) return fail(e, DPD_ERR_PARAM, "null pair array");
+    for (int64_t b = 0; b < nbonds; ++b)
+        if (head[b] < 0 || head[b] >= h.n || tail[b] < 0 || tail[b] >= h.n || head[b] == tail[b])
+            return fail(e, DPD_ERR_PARAM, "bond bead index out of range");
+    for (int64_t q = 0; q < npairs; ++q) {
+        if (first[q] < 0 || first[q] >= nbonds || second[q] < 0 || second[q] >= nbonds)
+            return fail(e, DPD_ERR_PARAM, "bond pair index out of range");
+        if (tail[second[q]] != head[first[q]]) return fail(e, DPD_ERR_PARAM, "bond pair bonds do not share a bead");
+    }
+    h.bh.assign(head, head + nbonds); h.bt.assign(tail, tail + nbonds);
+    h.bk.assign(k, k + nbonds); h.bl0.assign(l0, l0 + nbonds);
+    h.pf.assign(first, first + npairs); h.ps.assign(second, second + npairs);
+    h.pk.assign(modulus, modulus + npairs); h.pphi.assign(phi0, phi0 + npairs);
+    return DPD_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI ------
+
+extern "C" {
+
+const char* dpd_last_error(const dpd_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int dpd_create_batch(int32_t nbox, const dpd_params* p, dpd_engine** out)
+{
+    if (!out) return DPD_ERR_PARAM;
+    *out = nullptr;
+    if (!p || nbox < 1 || nbox > (1 << 22)) return DPD_ERR_PARAM;
+    dpd_engine* e = new dpd_engine();
+    *out = e;
+    int rc;
+    e->boxes.resize(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        if ((rc = check_params(e, p + b))) return rc;
+        BoxHost& h = e->boxes[b];
+        h.p = p[b];
+        const int nt = p[b].ntypes;
+        h.a.assign(p[b].a, p[b].a + nt * nt);
+        h.g.assign(p[b].gamma, p[b].gamma + nt * nt);
+        h.p.a = h.a.data();
+        h.p.gamma = h.g.data();
+        h.dt = p[b].dt; h.lambda = p[b].lambda; h.kT = p[b].kT;
+        BoxDev& B = h.B;
+        B.n[0] = p[b].nx; B.n[1] = p[b].ny; B.n[2] = p[b].nz;
+        for (int d = 0; d < 3; ++d) B.nb[d] = (B.n[d] + kBrick - 1) / kBrick;
+        B.wd[0] = p[b].wx; B.wd[1] = p[b].wy; B.wd[2] = p[b].wz;
+        for (int d = 0; d < 3; ++d) B.w[d] = (float)B.wd[d];
+        B.ntypes = nt;
+        B.seed0 = (uint32_t)p[b].seed;
+        B.seed1 = (uint32_t)(p[b].seed >> 32);
+        B.nbricks = B.nb[0] * B.nb[1] * B.nb[2];
+        B.box = b;
+        set_step_constants(h);
+    }
+    e->device = p[0].device;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (auto& ev : e->ev) HIPCHK(hipEventCreate(&ev));
+    e->pair_kernel = (p[0].pair_kernel == DPD_PAIR_AUTO) ? default_pair_kernel() : p[0].pair_kernel;
+    if (e->pair_kernel != DPD_PAIR_BRICK && e->pair_kernel != DPD_PAIR_FULLSHELL)
+        return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
+    if ((rc = dalloc(e, &e->flags, 4)) || (rc = dalloc(e, &e->dstep, 1)) ||
+        (rc = dalloc(e, &e->acc, (size_t)nbox * kObsStride)) || (rc = dalloc(e, &e->table, (size_t)nbox * kTableStride)) ||
+        (rc = dalloc(e, &e->dboxes, nbox)))
+        return rc;
+    HIPCHK(hipMemsetAsync(e->flags, 0, sizeof(int) * 4, e->stream));
+    HIPCHK(hipMemsetAsync(e->dstep, 0, sizeof(int64_t), e->stream));
+    if ((rc = upload_tables(e))) return rc;
+    e->err.clear();
+    return DPD_OK;
+}
+
+int dpd_create(const dpd_params* p, dpd_engine** out) { return dpd_create_batch(1, p, out); }
+
+void dpd_destroy(dpd_engine* e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graphs(e);
+    void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->aux[0], e->aux[1],
+                    e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
+                    e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
+                    e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->dboxes};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (auto& ev : e->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : e->pev) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int dpd_upload_beads(dpd_engine* e, int64_t n, const int32_t* id, const int32_t* type, const double* x,
+                     const double* unpbc, const double* v, const double* f, int64_t step0)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (e->boxes.size() != 1) return fail(e, DPD_ERR_STATE, "batch engine: use dpd_upload_box and dpd_commit");
+    HIPCHK(hipSetDevice(e->device));
+    const bool keep_topo = !e->boxes[0].bh.empty() && e->boxes[0].n == n;
+    int rc;
+    if ((rc = stage_beads(e, 0, n, id, type, x, unpbc, v, f))) return rc;
+    if ((rc = commit_beads(e, step0))) return rc;
+    if (keep_topo) return commit_topology(e);
+    e->has_bonds = e->has_angles = 0;
+    return DPD_OK;
+}
+
+int dpd_upload_topology(dpd_engine* e, int64_t nbonds, const int32_t* head, const int32_t* tail, const double* k,
+                        const double* l0, int64_t npairs, const int32_t* first_bond, const int32_t* second_bond,
+                        const double* modulus, const double* phi0)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (e->boxes.size() != 1) return fail(e, DPD_ERR_STATE, "batch engine: use dpd_upload_box_topology");
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "upload beads before topology");
+    HIPCHK(hipSetDevice(e->device));
+    int rc;
+    if ((rc = stage_topology(e, 0, nbonds, head, tail, k, l0, npairs, first_bond, second_bond, modulus, phi0)))
+        return rc;
+    return commit_topology(e);
+}
+
+int dpd_upload_box(dpd_engine* e, int32_t box, int64_t n, const int32_t* id, const int32_t* type, const double* x,
+                   const double* unpbc, const double* v, const double* f)
+{
+    if (!e) return DPD_ERR_PARAM;
+    return stage_beads(e, box, n, id, type, x, unpbc, v, f);
+}
+
+int dpd_upload_box_topology(dpd_engine* e, int32_t box, int64_t nbonds, const int32_t* head, const int32_t* tail,
+                            const double* k, const double* l0, int64_t npairs, const int32_t* first_bond,
+                            const int32_t* second_bond, const double* modulus, const double* phi0)
+{
+    if (!e) return DPD_ERR_PARAM;
+    return stage_topology(e, box, nbonds, head, tail, k, l0, npairs, first_bond, second_bond, modulus, phi0);
+}
+
+int dpd_commit(dpd_engine* e, int64_t step0)
+{
+    if (!e) return DPD_ERR_PARAM;
+    HIPCHK(hipSetDevice(e->device));
+    int rc;
+    if ((rc = commit_beads(e, step0))) return rc;
+    return commit_topology(e);
+}
+
+int32_t dpd_box_count(const dpd_engine* e) { return e ? (int32_t)e->boxes.size() : -1; }
 
 int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
 {
     if (!e) return DPD_ERR_PARAM;
-    if (e->n == 0) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
     if (nsteps < 0) return fail(e, DPD_ERR_PARAM, "negative step count");
     if (nsteps == 0) return DPD_OK;
     HIPCHK(hipSetDevice(e->device));
     const bool obs = flags & DPD_STEP_OBSERVE;
-    const bool timing = flags & 4u;
+    const bool timing = flags & DPD_STEP_TIMING;
+    const bool use_graph = (flags & DPD_STEP_GRAPH) && !timing;
     int rc;
-    if ((rc = zero_acc(e))) return rc;
     e->jumps = 0;
     if (timing) {
         while ((int64_t)e->pev.size() < 2 * nsteps) {
@@ -852,31 +1052,54 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
             HIPCHK(hipEventCreate(&ev));
             e->pev.push_back(ev);
         }
-        HIPCHK(hipEventRecord(e->ev[0], e->stream));
     }
-    for (int64_t t = 0; t < nsteps; ++t) {
-        e->step += 1;
-        set_rng_step(e, e->step);
-        rc = (t == 0) ? sort_step<kBegin>(e, false) : sort_step<kEndBegin>(e, false);
-        if (rc) return rc;
-        if (timing) HIPCHK(hipEventRecord(e->pev[2 * t], e->stream));
-        if ((rc = launch_pair(e, obs && t == nsteps - 1))) return rc;
-        if (timing) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
+    launch_zero(e, true);
+    HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    if (use_graph) {
+        // whole calls of an even step count replay one captured graph: the two SoA buffers
+        // swap once per step, so an even count returns to the captured pointers
+        int64_t chunk = std::min<int64_t>(nsteps, 64);
+        if (chunk & 1) chunk -= 1;
+        int64_t done = 0;
+        while (chunk >= 2 && nsteps - done >= chunk) {
+            const bool last_obs = obs && (nsteps - done == chunk);
+            const auto keyg = std::make_pair(chunk, (int)last_obs);
+            auto it = e->graphs.find(keyg);
+            if (it == e->graphs.end()) {
+                const int cur0 = e->cur;
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                rc = enqueue_steps(e, chunk, last_obs, false);
+                hipError_t st = hipStreamEndCapture(e->stream, &g);
+                if (rc) return rc;
+                if (st != hipSuccess) return fail(e, DPD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(st));
+                e->cur = cur0;
+                hipGraphExec_t ge;
+                HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                e->graphs[keyg] = ge;
+                e->graph_src[keyg] = g;
+                it = e->graphs.find(keyg);
+            }
+            HIPCHK(hipGraphLaunch(it->second, e->stream));
+            done += chunk;
+            e->step += chunk;
+        }
+        if (done < nsteps) {
+            if ((rc = enqueue_steps(e, nsteps - done, obs, false))) return rc;
+            e->step += nsteps - done;
+        }
+    } else {
+        if ((rc = enqueue_steps(e, nsteps, obs, timing))) return rc;
+        e->step += nsteps;
     }
-    const int n = (int)e->n;
-    if (obs)
-        hipLaunchKernelGGL((k_finish<true, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur),
-                           e->frc, e->slot_of, topo(e), e->B, e->flags, e->acc);
-    else
-        hipLaunchKernelGGL((k_finish<true, false>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur),
-                           e->frc, e->slot_of, topo(e), e->B, e->flags, e->acc);
-    HIPCHK(hipGetLastError());
-    if (timing) HIPCHK(hipEventRecord(e->ev[1], e->stream));
+    HIPCHK(hipEventRecord(e->ev[1], e->stream));
     if ((rc = read_flags(e))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+    e->t_total = ms;
+    e->t_pair = 0.0;
+    e->n_pair = 0;
     if (timing) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
-        e->t_total = ms;
         double tp = 0.0;
         for (int64_t t = 0; t < nsteps; ++t) {
             HIPCHK(hipEventElapsedTime(&ms, e->pev[2 * t], e->pev[2 * t + 1]));
@@ -885,7 +1108,7 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
         e->t_pair = tp;
         e->n_pair = nsteps;
     }
-    if (obs) return finish_obs(e, true);
+    if (obs) return finish_obs(e);
     e->obs_valid = false;
     return DPD_OK;
 }
@@ -893,94 +1116,134 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
 int dpd_compute_forces(dpd_engine* e, int64_t step, uint32_t flags, double* f_out, double* virial, double* pe)
 {
     if (!e) return DPD_ERR_PARAM;
-    if (e->n == 0) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
     HIPCHK(hipSetDevice(e->device));
     int rc;
-    const bool thermo_saved = e->thermostat_on;
+    std::vector<bool> saved;
     if (flags & DPD_NO_THERMOSTAT) {
-        e->thermostat_on = false;
-        if ((rc = upload_table(e))) return rc;
+        for (auto& b : e->boxes) { saved.push_back(b.thermostat_on); b.thermostat_on = false; }
+        if ((rc = upload_tables(e))) return rc;
     }
-    if ((rc = zero_acc(e))) return rc;
-    set_rng_step(e, step);
+    launch_zero(e, true);
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, step);
     // the uploaded state is sorted with vp = v: evaluate at it
     if ((rc = launch_pair(e, true))) return rc;
-    const int n = (int)e->n;
-    hipLaunchKernelGGL((k_finish<false, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur),
-                       e->frc, e->slot_of, topo(e), e->B, e->flags, e->acc);
+    launch_finish<false, true>(e);
     HIPCHK(hipGetLastError());
     if ((rc = read_flags(e))) return rc;
-    double h[32];
-    HIPCHK(hipMemcpyAsync(h, e->acc, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    std::vector<double> h(e->boxes.size() * kObsStride);
+    HIPCHK(hipMemcpyAsync(h.data(), e->acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, e->stream));
+    const int n = (int)e->n;
     if (f_out) {
         double* g = e->gather;
-        hipLaunchKernelGGL(k_gather, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->frc, e->B,
-                           g, g + 3 * n, g + 6 * n, g + 9 * n);
+        if (multi(e))
+            hipLaunchKernelGGL(k_gather<true>, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->frc,
+                               e->dboxes, e->boxes[0].B, g, g + 3 * n, g + 6 * n, g + 9 * n);
+        else
+            hipLaunchKernelGGL(k_gather<false>, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur),
+                               e->frc, e->dboxes, e->boxes[0].B, g, g + 3 * n, g + 6 * n, g + 9 * n);
         HIPCHK(hipMemcpyAsync(f_out, g + 9 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, e->stream));
     }
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, e->step);
     HIPCHK(hipStreamSynchronize(e->stream));
+    // totals over all boxes
     if (virial)
-        for (int i = 0; i < 9; ++i) virial[i] = h[O_VIR + i];
-    if (pe) { pe[0] = h[O_PEP]; pe[1] = h[O_PEB]; pe[2] = h[O_PEA]; }
+        for (int i = 0; i < 9; ++i) {
+            virial[i] = 0.0;
+            for (size_t b = 0; b < e->boxes.size(); ++b) virial[i] += h[b * kObsStride + O_VIR + i];
+        }
+    if (pe)
+        for (int i = 0; i < 3; ++i) {
+            pe[i] = 0.0;
+            for (size_t b = 0; b < e->boxes.size(); ++b) pe[i] += h[b * kObsStride + O_PEP + i];
+        }
     if (flags & DPD_NO_THERMOSTAT) {
-        e->thermostat_on = thermo_saved;
-        if ((rc = upload_table(e))) return rc;
+        for (size_t b = 0; b < e->boxes.size(); ++b) e->boxes[b].thermostat_on = saved[b];
+        if ((rc = upload_tables(e))) return rc;
     }
     return DPD_OK;
 }
 
-int dpd_observables(dpd_engine* e, dpd_obs* out)
+int dpd_observables_box(dpd_engine* e, int32_t box, dpd_obs* out)
 {
     if (!e || !out) return DPD_ERR_PARAM;
     if (!e->obs_valid) return fail(e, DPD_ERR_STATE, "no observables: call dpd_step with DPD_STEP_OBSERVE");
-    *out = e->obs;
+    if (box < 0 || box >= (int32_t)e->obs.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    *out = e->obs[box];
+    return DPD_OK;
+}
+
+int dpd_observables(dpd_engine* e, dpd_obs* out) { return dpd_observables_box(e, 0, out); }
+
+int dpd_download_box(dpd_engine* e, int32_t box, double* x, double* unpbc, double* v, double* f)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (box < -1 || box >= (int32_t)e->boxes.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    HIPCHK(hipSetDevice(e->device));
+    const int n = (int)e->n;
+    double* g = e->gather;
+    if (multi(e))
+        hipLaunchKernelGGL(k_gather<true>, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->frc,
+                           e->dboxes, e->boxes[0].B, g, g + 3 * n, g + 6 * n, g + 9 * n);
+    else
+        hipLaunchKernelGGL(k_gather<false>, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->frc,
+                           e->dboxes, e->boxes[0].B, g, g + 3 * n, g + 6 * n, g + 9 * n);
+    HIPCHK(hipGetLastError());
+    const int64_t off = box < 0 ? 0 : e->boxes[box].bead_off;
+    const int64_t cnt = box < 0 ? e->n : e->boxes[box].n;
+    const size_t b3 = sizeof(double) * 3 * cnt;
+    if (x) HIPCHK(hipMemcpyAsync(x, g + 3 * off, b3, hipMemcpyDeviceToHost, e->stream));
+    if (unpbc) HIPCHK(hipMemcpyAsync(unpbc, g + 3 * n + 3 * off, b3, hipMemcpyDeviceToHost, e->stream));
+    if (v) HIPCHK(hipMemcpyAsync(v, g + 6 * n + 3 * off, b3, hipMemcpyDeviceToHost, e->stream));
+    if (f) HIPCHK(hipMemcpyAsync(f, g + 9 * n + 3 * off, b3, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     return DPD_OK;
 }
 
 int dpd_download(dpd_engine* e, double* x, double* unpbc, double* v, double* f)
 {
+    return dpd_download_box(e, -1, x, unpbc, v, f);
+}
+
+int dpd_set_box_interactions(dpd_engine* e, int32_t box, const double* a, const double* gamma)
+{
     if (!e) return DPD_ERR_PARAM;
-    if (e->n == 0) return fail(e, DPD_ERR_STATE, "no beads uploaded");
-    HIPCHK(hipSetDevice(e->device));
-    const int n = (int)e->n;
-    double* g = e->gather;
-    hipLaunchKernelGGL(k_gather, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->frc, e->B, g,
-                       g + 3 * n, g + 6 * n, g + 9 * n);
-    HIPCHK(hipGetLastError());
-    const size_t b3 = sizeof(double) * 3 * n;
-    if (x) HIPCHK(hipMemcpyAsync(x, g, b3, hipMemcpyDeviceToHost, e->stream));
-    if (unpbc) HIPCHK(hipMemcpyAsync(unpbc, g + 3 * n, b3, hipMemcpyDeviceToHost, e->stream));
-    if (v) HIPCHK(hipMemcpyAsync(v, g + 6 * n, b3, hipMemcpyDeviceToHost, e->stream));
-    if (f) HIPCHK(hipMemcpyAsync(f, g + 9 * n, b3, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    return DPD_OK;
+    const int32_t b0 = box < 0 ? 0 : box, b1 = box < 0 ? (int32_t)e->boxes.size() : box + 1;
+    if (b0 >= (int32_t)e->boxes.size() || b1 > (int32_t)e->boxes.size())
+        return fail(e, DPD_ERR_PARAM, "box index out of range");
+    for (int32_t b = b0; b < b1; ++b) {
+        BoxHost& h = e->boxes[b];
+        const int nt = h.p.ntypes;
+        if (a) h.a.assign(a, a + nt * nt);
+        if (gamma) h.g.assign(gamma, gamma + nt * nt);
+    }
+    return upload_tables(e);
 }
 
 int dpd_set_interactions(dpd_engine* e, const double* a, const double* gamma)
 {
-    if (!e) return DPD_ERR_PARAM;
-    const int nt = e->p.ntypes;
-    if (a) e->a_h.assign(a, a + nt * nt);
-    if (gamma) e->g_h.assign(gamma, gamma + nt * nt);
-    return upload_table(e);
+    return dpd_set_box_interactions(e, -1, a, gamma);
 }
 
 int dpd_set_timestep(dpd_engine* e, double dt)
 {
     if (!e) return DPD_ERR_PARAM;
     if (!(dt > 0.0)) return fail(e, DPD_ERR_PARAM, "dt must be positive");
-    e->dt = dt;
-    set_step_constants(e);
-    return DPD_OK;
+    for (auto& b : e->boxes) { b.dt = dt; set_step_constants(b); }
+    drop_graphs(e);
+    return e->committed ? upload_boxes(e) : DPD_OK;
 }
 
 int dpd_toggle(dpd_engine* e, int32_t what, int32_t on)
 {
     if (!e) return DPD_ERR_PARAM;
-    if (what == DPD_TOGGLE_THERMOSTAT) e->thermostat_on = on != 0;
-    else if (what == DPD_TOGGLE_CONSERVATIVE) e->conservative_on = on != 0;
-    else return fail(e, DPD_ERR_PARAM, "unknown toggle");
-    return upload_table(e);
+    for (auto& b : e->boxes) {
+        if (what == DPD_TOGGLE_THERMOSTAT) b.thermostat_on = on != 0;
+        else if (what == DPD_TOGGLE_CONSERVATIVE) b.conservative_on = on != 0;
+        else return fail(e, DPD_ERR_PARAM, "unknown toggle");
+    }
+    return upload_tables(e);
 }
 
 int64_t dpd_last_timing(dpd_engine* e, double* ms3)

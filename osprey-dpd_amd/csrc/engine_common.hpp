@@ -1,17 +1,18 @@
 // engine_common.hpp -- device-side building blocks shared by the DPD step kernels.
 //
-// Layout in HBM (one box): beads are sorted by a cell key every step.  The key orders
-// cells brick-major: the box is tiled into 4x4x4-cell bricks (partial bricks at the
-// far faces), key = brick * 64 + local cell (x fastest).  A brick's beads are one
-// contiguous slot range, which is what the LDS pair kernel stages.
+// Layout in HBM: an engine holds one or more boxes (replicas).  Beads of all boxes are
+// sorted together every step by a global cell key.  Within a box the key orders cells
+// brick-major: the box is tiled into 4x4x4-cell bricks (partial bricks at the far faces),
+// local key = brick * 64 + local cell (x fastest); a box's keys start at its key offset.
+// A brick's beads are therefore one contiguous slot range, which is what the LDS pair
+// kernel stages, and a box's beads are one contiguous slot range too.
 //
-// Positions are stored as FP32 offsets inside the bead's cell, o in [0, w), plus the
-// integer cell (from the key) and integer periodic image counts.  Differences of
-// positions are formed as (o_i - o_j) + (c_i - c_j) * w with exact integer cell
-// deltas, so FP32 keeps ~1e-7 absolute precision at any box size (absolute FP32
-// coordinates would lose 7.6e-6 at L = 64).  The absolute FP64 position c*w + o and
-// the unwrapped position (c + img*n)*w + o are formed only on download and in the
-// FP64 bonded terms.
+// Positions are FP32 offsets inside the bead's cell, o in [0, w), plus the integer cell
+// (from the key) and integer periodic image counts.  Differences of positions are
+// (o_i - o_j) + (c_i - c_j) * w with exact integer cell deltas, so FP32 keeps ~1e-7
+// absolute precision at any box size (absolute FP32 coordinates would lose 7.6e-6 at
+// L = 64).  The absolute FP64 position c*w + o and the unwrapped position
+// (c + img*n)*w + o are formed only on download and in the FP64 bonded terms.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -21,9 +22,12 @@ namespace dpd {
 constexpr int kBrick = 4;        // cells per brick edge
 constexpr int kBrickCells = 64;  // cells per brick
 constexpr int kMaxTypes = 16;
+constexpr int kTypeStride = kMaxTypes * kMaxTypes;   // one matrix of a box's table
+constexpr int kTableStride = 3 * kTypeStride;        // a | gamma | sqrt(gamma) per box
+constexpr int kObsStride = 32;                       // FP64 accumulators per box
 
-// Constants of one box, passed by value (kernarg).  Mirrors the CCNTCell statics set by
-// SetSimBoxLengths / SetTimeStepConstants (src/CNTCell.cpp:143-203).
+// Constants of one box.  Mirrors the CCNTCell statics set by SetSimBoxLengths /
+// SetTimeStepConstants (src/CNTCell.cpp:143-203).
 struct BoxDev {
     int n[3];          // cells per axis
     int nb[3];         // bricks per axis
@@ -33,15 +37,20 @@ struct BoxDev {
     float invrootdt;              // sqrt(24 kT / dt)
     int ntypes;
     uint32_t seed0, seed1;        // key of the pair stream
-    uint32_t step0, step1;        // simulation time of the force evaluation
+    int koff;                     // first global cell key of this box
+    int boff;                     // first global brick of this box
+    int nbricks;
+    int box;                      // index of this box
 };
 
+// local key of cell (cx, cy, cz) of a box
 __host__ __device__ inline int cell_key(int cx, int cy, int cz, const BoxDev& B)
 {
     const int brick = (cx >> 2) + B.nb[0] * ((cy >> 2) + B.nb[1] * (cz >> 2));
     return brick * kBrickCells + (((cz & 3) * 4 + (cy & 3)) * 4 + (cx & 3));
 }
 
+// cell of a local key
 __host__ __device__ inline void key_cell(int key, const BoxDev& B, int c[3])
 {
     const int brick = key >> 6, l = key & 63;
@@ -58,6 +67,10 @@ __host__ __device__ inline int floordiv(int a, int n)
 {
     return (a >= 0) ? a / n : -((-a + n - 1) / n);
 }
+
+// bead word packing: pos.w = type | box << 8
+__host__ __device__ inline int bead_type(int w) { return w & 0xFF; }
+__host__ __device__ inline int bead_box(int w) { return w >> 8; }
 
 // Philox4x32-10 (Salmon et al., SC'11).  Pair uniform: counter (min id, max id, step),
 // key = seed; U = (word0 >> 8) * 2^-24, exactly representable in FP32.
@@ -78,39 +91,43 @@ __device__ inline uint32_t philox_word0(uint32_t c0, uint32_t c1, uint32_t c2, u
     return c0;
 }
 
-__device__ inline float pair_uniform(uint32_t ida, uint32_t idb, const BoxDev& B)
+// key of the pair stream of one box at one step
+struct RngKey {
+    uint32_t seed0, seed1, step0, step1;
+};
+
+__device__ inline RngKey rng_key(const BoxDev& B, int64_t step)
+{
+    return RngKey{B.seed0, B.seed1, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32)};
+}
+
+__device__ inline float pair_uniform(uint32_t ida, uint32_t idb, const RngKey& K)
 {
     const uint32_t lo = ida < idb ? ida : idb;
     const uint32_t hi = ida < idb ? idb : ida;
-    const uint32_t r = philox_word0(lo, hi, B.step0, B.step1, B.seed0, B.seed1);
+    const uint32_t r = philox_word0(lo, hi, K.step0, K.step1, K.seed0, K.seed1);
     return (float)(r >> 8) * 0x1p-24f;
 }
-
-// Type-pair table in device memory: [0, nt^2) a_ij, [nt^2, 2nt^2) gamma_ij,
-// [2nt^2, 3nt^2) sqrt(gamma_ij).
-struct PairTable {
-    const float* t;
-};
 
 // DPD pair force on bead i from bead j (src/CNTCell.cpp:1191-1225):
 //   F = [a w + (-gamma w^2 (r^.v)) + sqrt(gamma) w sqrt(24kT/dt) (0.5 - U)] r^,  w = 1 - r
 // returns the scalar F/r so that F_vec = s * dx.
-__device__ inline float pair_scalar(float dr2, float dxdv, float a, float sg, float g,
-                                    uint32_t idi, uint32_t idj, const BoxDev& B, float* pe)
+__device__ inline float pair_scalar(float dr2, float dxdv, float a, float sg, float g, uint32_t idi,
+                                    uint32_t idj, float invrootdt, const RngKey& K, float* pe)
 {
     const float dr = sqrtf(dr2);
     const float inv = 1.0f / dr;
     const float wr = 1.0f - dr;
     const float con = a * wr;
     const float diss = -(g * wr * wr) * (dxdv * inv);
-    const float u = pair_uniform(idi, idj, B);
-    const float rnd = sg * wr * B.invrootdt * (0.5f - u);
+    const float u = pair_uniform(idi, idj, K);
+    const float rnd = sg * wr * invrootdt * (0.5f - u);
     if (pe) *pe = 0.5f * a * wr * wr;
     return (con + diss + rnd) * inv;
 }
 
-// Move a bead by displacement d from offset o in cell c (image img): returns the new
-// offset in [0, w), cell and image; sets *jump when it crossed more than one cell.
+// Move a bead by displacement d from offset o in cell c (image img): new offset in
+// [0, w), cell and image; `jump` when it crossed more than one cell.
 struct Moved {
     float o[3];
     int c[3];
@@ -119,17 +136,16 @@ struct Moved {
     int bad;
 };
 
-__device__ inline void advance(const float o[3], const int c[3], const int img[3],
-                               const float d[3], const BoxDev& B, Moved& m)
+__device__ inline void advance(const float o[3], const int c[3], const int img[3], const float d[3],
+                               const BoxDev& B, Moved& m)
 {
     m.jump = 0;
     m.bad = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float x = o[a] + d[a];
-        int s = 0;
         if (!(fabsf(x) < 1.0e6f)) { m.bad = 1; x = 0.0f; }
-        s = (int)floorf(x / B.w[a]);
+        int s = (int)floorf(x / B.w[a]);
         x = x - (float)s * B.w[a];
         if (x < 0.0f) { x += B.w[a]; s -= 1; }
         if (x >= B.w[a]) x = B.w[a] * 0.99999994f;
@@ -150,6 +166,32 @@ __device__ inline double wave_sum(double v)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
+}
+
+// Add per-thread FP64 values v[0..N) into acc[box * kObsStride + base + i]: one
+// block-wide reduction for the threads of the block's first box, direct atomics for
+// threads of another box (only blocks straddling a box boundary have any).
+template <int N>
+__device__ inline void accumulate_boxes(const double (&v)[N], int box, int box0, int base, double* acc)
+{
+    __shared__ double red[N][4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const bool mine = box == box0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double s = wave_sum(mine ? v[i] : 0.0);
+        if (lane == 0) red[i][wid] = s;
+    }
+    if (!mine && box >= 0)
+        for (int i = 0; i < N; ++i)
+            if (v[i] != 0.0) atomicAdd(acc + box * kObsStride + base + i, v[i]);
+    __syncthreads();
+    if (threadIdx.x < N) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
+        atomicAdd(acc + box0 * kObsStride + base + threadIdx.x, s);
+    }
 }
 
 }  // namespace dpd

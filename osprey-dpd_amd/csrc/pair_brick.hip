@@ -46,7 +46,7 @@ struct Smem {
     int start[kStaged + 1];     // LDS index of each staged cell's first bead
     int gstart[kStaged];        // global slot of each staged cell's first bead
     uint32_t queue[kWaves][kQueue];
-    float table[3 * kMaxTypes * kMaxTypes];
+    float table[kTableStride];
     double red[10][kWaves];
     int nstaged;
 };
@@ -56,7 +56,8 @@ __device__ inline int sidx(int lx, int ly, int lz) { return (lx + 1) + 6 * ((ly 
 
 // evaluate one queued pair; returns virial/PE contributions through acc (OBS)
 template <bool OBS>
-__device__ inline void eval_pair(Smem& S, uint32_t packed, int nhome, int nt, const BoxDev& B, float* vir)
+__device__ inline void eval_pair(Smem& S, uint32_t packed, int nhome, float invrootdt, const RngKey& K,
+                                 float* vir)
 {
     const int li = (int)(packed & 0xFFFFu), lj = (int)(packed >> 16);
     const float4 pi = S.pos[li], pj = S.pos[lj];
@@ -64,12 +65,12 @@ __device__ inline void eval_pair(Smem& S, uint32_t packed, int nhome, int nt, co
     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
     const float dr2 = dx * dx + dy * dy + dz * dz;
     const int ti = __float_as_int(pi.w) & 0xFF, tj = __float_as_int(pj.w) & 0xFF;
-    const int tt = ti * nt + tj;
-    const float a = S.table[tt], g = S.table[nt * nt + tt], sg = S.table[2 * nt * nt + tt];
+    const int tt = ti * kMaxTypes + tj;
+    const float a = S.table[tt], g = S.table[kTypeStride + tt], sg = S.table[2 * kTypeStride + tt];
     const float dxdv = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
     float pe;
     const float s = pair_scalar(dr2, dxdv, a, sg, g, (uint32_t)__float_as_int(vi.w),
-                                (uint32_t)__float_as_int(vj.w), B, OBS ? &pe : nullptr);
+                                (uint32_t)__float_as_int(vj.w), invrootdt, K, OBS ? &pe : nullptr);
     const float Fx = s * dx, Fy = s * dy, Fz = s * dz;
     atomicAdd(&S.fx[li], Fx);
     atomicAdd(&S.fy[li], Fy);
@@ -93,15 +94,15 @@ __device__ inline void eval_pair(Smem& S, uint32_t packed, int nhome, int nt, co
 // thread-per-bead 27-cell loop over global memory (overflow fallback); same arithmetic
 // as k_pair_fullshell
 template <bool OBS>
-__device__ inline void fullshell_bead(const PairArgs& A, const BoxDev& B, int i, const float* table, float* vir)
+__device__ inline void fullshell_bead(const PairArgs& A, const BoxDev& B, const RngKey& K, int i, const float* table,
+                                      float* vir)
 {
-    const int nt = B.ntypes;
     const float4 pi = A.pos[i];
     const float4 vi = A.vp[i];
-    const int ti = __float_as_int(pi.w);
+    const int ti = bead_type(__float_as_int(pi.w));
     const uint32_t idi = (uint32_t)__float_as_int(vi.w);
     int c[3];
-    key_cell(A.key[i], B, c);
+    key_cell(A.key[i] - B.koff, B, c);
     float fx = 0.f, fy = 0.f, fz = 0.f;
     for (int dz = -1; dz <= 1; ++dz) {
         int cz = c[2] + dz;
@@ -112,7 +113,7 @@ __device__ inline void fullshell_bead(const PairArgs& A, const BoxDev& B, int i,
             for (int dx = -1; dx <= 1; ++dx) {
                 int cx = c[0] + dx;
                 cx += (cx < 0) ? B.n[0] : ((cx >= B.n[0]) ? -B.n[0] : 0);
-                const int k = cell_key(cx, cy, cz, B);
+                const int k = B.koff + cell_key(cx, cy, cz, B);
                 const int j0 = A.cstart[k], j1 = A.cstart[k + 1];
                 const float sx = -(float)dx * B.w[0], sy = -(float)dy * B.w[1], sz = -(float)dz * B.w[2];
                 for (int j = j0; j < j1; ++j) {
@@ -122,11 +123,12 @@ __device__ inline void fullshell_bead(const PairArgs& A, const BoxDev& B, int i,
                     const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     if (dr2 < 1.0f && dr2 > 1e-18f) {
                         const float4 vj = A.vp[j];
-                        const int tt = ti * nt + __float_as_int(pj.w);
+                        const int tt = ti * kMaxTypes + bead_type(__float_as_int(pj.w));
                         const float dxdv = ddx * (vi.x - vj.x) + ddy * (vi.y - vj.y) + ddz * (vi.z - vj.z);
                         float pe;
-                        const float s = pair_scalar(dr2, dxdv, table[tt], table[2 * nt * nt + tt], table[nt * nt + tt],
-                                                    idi, (uint32_t)__float_as_int(vj.w), B, OBS ? &pe : nullptr);
+                        const float s = pair_scalar(dr2, dxdv, table[tt], table[2 * kTypeStride + tt],
+                                                    table[kTypeStride + tt], idi, (uint32_t)__float_as_int(vj.w),
+                                                    B.invrootdt, K, OBS ? &pe : nullptr);
                         const float Fx = s * ddx, Fy = s * ddy, Fz = s * ddz;
                         fx += Fx; fy += Fy; fz += Fz;
                         if (OBS) {
@@ -144,20 +146,31 @@ __device__ inline void fullshell_bead(const PairArgs& A, const BoxDev& B, int i,
 }
 
 template <bool OBS>
-__global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, int per_xcd)
+__global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, int per_xcd)
 {
     __shared__ Smem S;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int brick = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (brick >= A.nbricks) return;   // whole workgroup exits together
-    const int nt = B.ntypes;
+    const int gbrick = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (gbrick >= A.nbricks) return;   // whole workgroup exits together
+    // the box owning this brick (bricks of a box are contiguous)
+    int lo = 0, hi = A.nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.boxes[mid].boff <= gbrick) lo = mid; else hi = mid - 1;
+    }
+    const BoxDev B = A.boxes[lo];
+    const int box = lo;
+    const int brick = gbrick - B.boff;
+    const RngKey K = rng_key(B, *A.step);
     const int bx = brick % B.nb[0], byz = brick / B.nb[0];
     const int by = byz % B.nb[1], bz = byz / B.nb[1];
     const int X0 = bx * kBrick, Y0 = by * kBrick, Z0 = bz * kBrick;
-    const int base = A.cstart[brick * kBrickCells];
-    const int nhome = A.cstart[brick * kBrickCells + kBrickCells] - base;
-    for (int q = tid; q < 3 * nt * nt; q += kThreads) S.table[q] = A.table[q];
+    const int kb = B.koff + brick * kBrickCells;
+    const int base = A.cstart[kb];
+    const int nhome = A.cstart[kb + kBrickCells] - base;
+    const float* gtab = A.table + box * kTableStride;
+    for (int q = tid; q < kTableStride; q += kThreads) S.table[q] = gtab[q];
 
     // ---- staged cell table: home cells keep their global order, halo cells follow
     int cnt = 0, gs = 0;
@@ -169,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, i
         const int wx = gx < 0 ? gx + B.n[0] : (gx >= B.n[0] ? gx - B.n[0] : gx);
         const int wy = gy < 0 ? gy + B.n[1] : (gy >= B.n[1] ? gy - B.n[1] : gy);
         const int wz = gz < 0 ? gz + B.n[2] : (gz >= B.n[2] ? gz - B.n[2] : gz);
-        const int k = cell_key(wx, wy, wz, B);
+        const int k = B.koff + cell_key(wx, wy, wz, B);
         gs = A.cstart[k];
         cnt = A.cstart[k + 1] - gs;
         S.gstart[tid] = gs;
@@ -201,14 +214,16 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, i
     const bool fits = nhome <= kMaxHome && nstaged <= kMaxStaged;
     if (!fits) {
         float vir[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int i = tid; i < nhome; i += kThreads) fullshell_bead<OBS>(A, B, base + i, S.table, vir);
+        for (int i = tid; i < nhome; i += kThreads) fullshell_bead<OBS>(A, B, K, base + i, S.table, vir);
         if (OBS) {
             for (int q = 0; q < 10; ++q) {
                 const double s = wave_sum((double)vir[q]);
                 if (lane == 0) S.red[q][wave] = s;
             }
             __syncthreads();
-            if (tid < 10) atomicAdd(A.acc + tid, S.red[tid][0] + S.red[tid][1] + S.red[tid][2] + S.red[tid][3]);
+            if (tid < 10)
+                atomicAdd(A.acc + box * kObsStride + kObsVir + tid,
+                          S.red[tid][0] + S.red[tid][1] + S.red[tid][2] + S.red[tid][3]);
         }
         return;
     }
@@ -217,9 +232,9 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, i
     for (int i = tid; i < nhome; i += kThreads) {
         const int slot = base + i;
         const float4 p = A.pos[slot];
-        const int l = A.key[slot] & 63;
+        const int l = (A.key[slot] - B.koff) & 63;
         const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
-        const int packed = __float_as_int(p.w) | (sidx(lx, ly, lz) << 8);
+        const int packed = bead_type(__float_as_int(p.w)) | (sidx(lx, ly, lz) << 8);
         S.pos[i] = make_float4((float)(lx - 2) * B.w[0] + p.x, (float)(ly - 2) * B.w[1] + p.y,
                                (float)(lz - 2) * B.w[2] + p.z, __int_as_float(packed));
         S.vp[i] = A.vp[slot];
@@ -238,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, i
             const int wx = gx < 0 ? gx + B.n[0] : (gx >= B.n[0] ? gx - B.n[0] : gx);
             const int wy = gy < 0 ? gy + B.n[1] : (gy >= B.n[1] ? gy - B.n[1] : gy);
             const int wz = gz < 0 ? gz + B.n[2] : (gz >= B.n[2] ? gz - B.n[2] : gz);
-            k = cell_key(wx, wy, wz, B);
+            k = B.koff + cell_key(wx, wy, wz, B);
         }
         const int cntc = A.cstart[k + 1] - g0;
         for (int q = 0; q < cntc; ++q) {
@@ -256,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, i
         const int wx = gx < 0 ? gx + B.n[0] : (gx >= B.n[0] ? gx - B.n[0] : gx);
         const int wy = gy < 0 ? gy + B.n[1] : (gy >= B.n[1] ? gy - B.n[1] : gy);
         const int wz = gz < 0 ? gz + B.n[2] : (gz >= B.n[2] ? gz - B.n[2] : gz);
-        const int k = cell_key(wx, wy, wz, B);
+        const int k = B.koff + cell_key(wx, wy, wz, B);
         scount[tid] = A.cstart[k + 1] - A.cstart[k];
     }
     __syncthreads();
@@ -298,7 +313,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, i
             if (qn >= 64) {
                 __builtin_amdgcn_wave_barrier();
                 const uint32_t pk = q[lane];
-                eval_pair<OBS>(S, pk, nhome, nt, B, vir);
+                eval_pair<OBS>(S, pk, nhome, B.invrootdt, K, vir);
                 const uint32_t rest = (lane + 64 < qn) ? q[lane + 64] : 0u;
                 __builtin_amdgcn_wave_barrier();
                 if (lane + 64 < qn) q[lane] = rest;
@@ -308,7 +323,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, i
         }
     }
     __builtin_amdgcn_wave_barrier();
-    if (lane < qn) eval_pair<OBS>(S, q[lane], nhome, nt, B, vir);
+    if (lane < qn) eval_pair<OBS>(S, q[lane], nhome, B.invrootdt, K, vir);
     __syncthreads();
 
     for (int i = tid; i < nhome; i += kThreads) A.frc[base + i] = make_float4(S.fx[i], S.fy[i], S.fz[i], 0.f);
@@ -318,19 +333,20 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, BoxDev B, i
             if (lane == 0) S.red[c][wave] = s;
         }
         __syncthreads();
-        if (tid < 10) atomicAdd(A.acc + tid, S.red[tid][0] + S.red[tid][1] + S.red[tid][2] + S.red[tid][3]);
+        if (tid < 10)
+            atomicAdd(A.acc + box * kObsStride + kObsVir + tid,
+                      S.red[tid][0] + S.red[tid][1] + S.red[tid][2] + S.red[tid][3]);
     }
 }
 
 }  // namespace
 
-hipError_t launch_pair_brick(const PairArgs& a, const BoxDev& B, bool obs, hipStream_t s)
+hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s)
 {
-    if (B.ntypes > kMaxTypes) return hipErrorInvalidValue;
     const int per_xcd = (a.nbricks + 7) / 8;
     const dim3 grid(8 * per_xcd);
-    if (obs) hipLaunchKernelGGL((k_pair_brick<true>), grid, dim3(kThreads), 0, s, a, B, per_xcd);
-    else hipLaunchKernelGGL((k_pair_brick<false>), grid, dim3(kThreads), 0, s, a, B, per_xcd);
+    if (obs) hipLaunchKernelGGL((k_pair_brick<true>), grid, dim3(kThreads), 0, s, a, per_xcd);
+    else hipLaunchKernelGGL((k_pair_brick<false>), grid, dim3(kThreads), 0, s, a, per_xcd);
     return hipGetLastError();
 }
 

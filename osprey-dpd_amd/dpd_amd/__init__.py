@@ -15,7 +15,7 @@ OK = 0
 ERR_NAMES = {1: "PARAM", 2: "OOM", 3: "HIP", 4: "NAN", 5: "OUT_OF_BOX", 6: "STATE", 7: "IO",
              8: "UNSUPPORTED"}
 PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL = 0, 1, 2
-STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING = 1, 2, 4
+STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH = 1, 2, 4, 8
 TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE = 1, 2
 
 
@@ -62,6 +62,15 @@ SIGNATURES = {
     "dpd_toggle": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "dpd_last_timing": (C.c_int64, [_P, _P]),
     "dpd_get_step": (C.c_int64, [_P]),
+    "dpd_create_batch": (C.c_int, [C.c_int32, C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "dpd_upload_box": (C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P]),
+    "dpd_upload_box_topology": (C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, C.c_int64, _P, _P,
+                                          _P, _P]),
+    "dpd_commit": (C.c_int, [_P, C.c_int64]),
+    "dpd_box_count": (C.c_int32, [_P]),
+    "dpd_observables_box": (C.c_int, [_P, C.c_int32, C.POINTER(Obs)]),
+    "dpd_download_box": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
+    "dpd_set_box_interactions": (C.c_int, [_P, C.c_int32, _P, _P]),
 }
 
 _lib = None
@@ -157,8 +166,9 @@ class Engine:
                                               _ptr(l0), len(first), _ptr(first), _ptr(second),
                                               _ptr(modulus), _ptr(phi0)))
 
-    def step(self, nsteps=1, observe=False, timing=False):
-        flags = (STEP_OBSERVE if observe else 0) | (STEP_TIMING if timing else 0)
+    def step(self, nsteps=1, observe=False, timing=False, graph=False):
+        flags = (STEP_OBSERVE if observe else 0) | (STEP_TIMING if timing else 0) | \
+            (STEP_GRAPH if graph else 0)
         self._check(lib().dpd_step(self.h, int(nsteps), flags))
 
     def compute_forces(self, step, thermostat=True):
@@ -169,12 +179,13 @@ class Engine:
         self._check(lib().dpd_compute_forces(self.h, int(step), flags, _ptr(f), _ptr(vir), _ptr(pe)))
         return f, vir, pe
 
-    def observables(self):
+    def observables(self, box=0):
         o = Obs()
-        self._check(lib().dpd_observables(self.h, C.byref(o)))
+        self._check(lib().dpd_observables_box(self.h, int(box), C.byref(o)))
         return {"step": o.step, "kT": o.kT, "pressure": o.pressure, "stress": np.array(o.stress[:]),
                 "sum_v2": np.array(o.sum_v2[:]), "momentum": np.array(o.momentum[:]),
-                "pe": np.array([o.pe_pair, o.pe_bond, o.pe_angle]), "jumps": o.jumps}
+                "pe": np.array([o.pe_pair, o.pe_bond, o.pe_angle]), "jumps": o.jumps,
+                "nbeads": o.nbeads}
 
     def download(self):
         n = self.n
@@ -200,3 +211,78 @@ class Engine:
     @property
     def step_count(self):
         return lib().dpd_get_step(self.h)
+
+
+def _params(cells, a, gamma, dt, lam, kT, seed, widths, device, pair_kernel, keep):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    keep += [a, g]
+    p = Params()
+    p.nx, p.ny, p.nz = (int(c) for c in cells)
+    p.wx, p.wy, p.wz = (float(w) for w in widths)
+    p.dt, p.lam, p.kT = dt, lam, kT
+    p.seed = int(seed)
+    p.ntypes = a.shape[0]
+    p.a = a.ctypes.data_as(C.POINTER(C.c_double))
+    p.gamma = g.ctypes.data_as(C.POINTER(C.c_double))
+    p.device = device
+    p.pair_kernel = pair_kernel
+    return p
+
+
+class Batch(Engine):
+    """Many independent boxes advanced together (each kernel spans all boxes).
+
+    boxes: list of dicts with keys cells, a, gamma and optional dt, lam, kT, seed, widths.
+    """
+
+    def __init__(self, boxes, device=0, pair_kernel=PAIR_AUTO):
+        L = lib()
+        keep = []
+        arr = (Params * len(boxes))()
+        for i, b in enumerate(boxes):
+            arr[i] = _params(b["cells"], b["a"], b["gamma"], b.get("dt", 0.02), b.get("lam", 0.5),
+                             b.get("kT", 1.0), b.get("seed", 1), b.get("widths", (1.0, 1.0, 1.0)),
+                             device, pair_kernel, keep)
+        h = C.c_void_p()
+        rc = L.dpd_create_batch(len(boxes), arr, C.byref(h))
+        self.h = h
+        if rc != OK:
+            msg = L.dpd_last_error(h).decode() if h else ""
+            self.close()
+            raise DpdError(rc, msg)
+        self.nbox = len(boxes)
+        self.box_n = [0] * self.nbox
+        self.n = 0
+
+    def upload_box(self, box, ids, types, x, v, f=None, unpbc=None):
+        n = len(ids)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        x, v, f, u = _f64(x, (n, 3)), _f64(v, (n, 3)), _f64(f, (n, 3)), _f64(unpbc, (n, 3))
+        self._check(lib().dpd_upload_box(self.h, int(box), n, _ptr(ids), _ptr(types), _ptr(x), _ptr(u),
+                                         _ptr(v), _ptr(f)))
+        self.box_n[box] = n
+
+    def upload_box_topology(self, box, head=(), tail=(), k=(), l0=(), first=(), second=(), modulus=(),
+                            phi0=()):
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        head, tail, first, second = i32(head), i32(tail), i32(first), i32(second)
+        k, l0, modulus, phi0 = _f64(k), _f64(l0), _f64(modulus), _f64(phi0)
+        self._check(lib().dpd_upload_box_topology(self.h, int(box), len(head), _ptr(head), _ptr(tail),
+                                                  _ptr(k), _ptr(l0), len(first), _ptr(first),
+                                                  _ptr(second), _ptr(modulus), _ptr(phi0)))
+
+    def commit(self, step0=0):
+        self._check(lib().dpd_commit(self.h, int(step0)))
+        self.n = sum(self.box_n)
+
+    def download_box(self, box):
+        n = self.box_n[box]
+        x, u, v, f = (np.empty((n, 3)) for _ in range(4))
+        self._check(lib().dpd_download_box(self.h, int(box), _ptr(x), _ptr(u), _ptr(v), _ptr(f)))
+        return x, u, v, f
+
+    def set_box_interactions(self, box, a=None, gamma=None):
+        a, g = _f64(a), _f64(gamma)
+        self._check(lib().dpd_set_box_interactions(self.h, int(box), _ptr(a), _ptr(g)))
