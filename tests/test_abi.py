@@ -38,8 +38,11 @@ def test_exports_every_declared_symbol(header):
     assert set(names) <= exported
 
 
-def test_binding_covers_engine_header():
+def test_bindings_cover_headers():
+    from dpd_amd import host, mlp
     assert set(declared("dpd_engine.h")) == set(dpd_amd.SIGNATURES)
+    assert set(declared("dpd_host.h")) == set(host.HOST_SIGNATURES)
+    assert set(declared("dpd_mlp.h")) == set(mlp.MLP_SIGNATURES)
 
 
 def test_missing_library_fails_loudly(monkeypatch):
