@@ -92,6 +92,40 @@ def build_engine(wl, kernel):
     return e
 
 
+def bench_batched(nbox=256, steps=100, warmup=20, nsingle=16, seed=20261202):
+    """Batched replicas: `nbox` water/lipid boxes with side lengths L ~ U{8..24}, a_ij ~
+    U[20,90], k ~ U[64,256], advanced together (one graph replay per 50 steps), against
+    the same kind of box advanced alone, one box per launch sequence."""
+    from dpd_amd import surrogate as S
+    rng = np.random.default_rng(seed)
+    params = S.sample_params(nbox, rng)
+    sizes = rng.integers(8, 25, size=nbox)
+    seeds = rng.integers(1, 2 ** 31, size=nbox)
+    e, meta = S.build_batch(params, sizes, seeds)
+    e.step(warmup, graph=True)
+    t0 = time.perf_counter()
+    e.step(steps, observe=True, graph=True)
+    wall = time.perf_counter() - t0
+    batched = e.n * steps / wall
+    nb_total = e.n
+    del e
+    # the same kind of boxes one at a time (a sample of nsingle boxes)
+    tot_bs, tot_t = 0.0, 0.0
+    for i in range(nsingle):
+        es, _ = S.build_batch(params[i:i + 1], sizes[i:i + 1], seeds[i:i + 1])
+        es.step(warmup, graph=True)
+        t0 = time.perf_counter()
+        es.step(steps, observe=True, graph=True)
+        tot_t += time.perf_counter() - t0
+        tot_bs += es.n * steps
+        del es
+    single = tot_bs / tot_t
+    return {"boxes": nbox, "beads": int(nb_total), "steps": steps, "value": batched, "unit": UNIT,
+            "ms_per_step": wall / steps * 1e3, "single_box_value": single, "single_box_sample": nsingle,
+            "speedup_vs_single_box": batched / single,
+            "workload": "256 water/lipid replicas, L~U{8..24}, rho=3, a_ij~U[20,90], k~U[64,256]"}
+
+
 def cpu_baseline_reference(t1=2, t2=6):
     """Time the reference binary on the condensate configuration: run lengths t1 and t2
     concurrently (one core each); per-step time = (wall2 - wall1)/(t2 - t1)."""
@@ -151,6 +185,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--pair-kernel", type=int, default=D.PAIR_AUTO)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (multi-GPU "
@@ -177,6 +212,8 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
+    batched = None if args.no_batched else bench_batched()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_reference() or cpu_baseline_port(wl)
@@ -191,6 +228,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "pair", "pair_ms_per_launch": t_pair * 1e3,
                      "step_gbs": alg_bytes / (wall / args.steps) / 1e9},
+        "batched": batched,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "observables": {"kT": obs["kT"], "pressure": obs["pressure"], "jumps": obs["jumps"]},

@@ -396,6 +396,7 @@ bool build_system(const InputData& d, System& s, std::string& err)
             }
         }
     }
+    s.types = types;
     // CInitialState::CreatePolymers (src/InitialState.cpp:451-521)
     const double total = d.density * (d.nx * d.wx) * (d.ny * d.wy) * (d.nz * d.wz);
     double nalphasum = 0.0;

@@ -79,6 +79,7 @@ struct System {
     std::vector<int32_t> pair_first, pair_second;
     std::vector<double> pair_k, pair_phi0;   // radians (src/BondPairType.cpp:85)
     std::vector<int64_t> polymers_per_type;
+    std::vector<PolymerType> types;          // expanded polymer templates
     int64_t npolymers = 0;
 };
 

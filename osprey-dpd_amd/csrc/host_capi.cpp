@@ -148,7 +148,11 @@ bool build_lamella(const dpd_input* in, std::mt19937_64& rng, double* u, std::st
     const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
     std::uniform_real_distribution<double> U(0.0, 1.0);
     // depth of each bead of the polymer template (bonds from bead 0)
-    const auto& tmpl = d.polymers[ptype];
+    const auto& tmpl = s.types[ptype];
+    if (tmpl.bead_type.empty()) {
+        err = "lamella polymer has no beads";
+        return false;
+    }
     std::vector<int> depth(tmpl.bead_type.size(), 0);
     for (size_t b = 0; b < tmpl.bond_head.size(); ++b) depth[tmpl.bond_tail[b]] = depth[tmpl.bond_head[b]] + 1;
     const int maxd = std::max(1, *std::max_element(depth.begin(), depth.end()));

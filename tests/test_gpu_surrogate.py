@@ -1,0 +1,80 @@
+"""The learned part end to end: batched replica runs on the GPU produce (pressure, area
+per lipid) for random parameter sets; the BF16 surrogate trained on 80% of them must
+predict the held-out pressure within 2%; screening a large sweep with it costs a tiny
+fraction of simulating the sweep, and the simulation judges the accepted candidates."""
+import time
+
+import numpy as np
+import pytest
+
+from dpd_amd import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dataset():
+    rng = np.random.default_rng(2026)
+    n = 256
+    params = S.sample_params(n, rng)
+    sizes = rng.integers(8, 12, size=n)
+    seeds = rng.integers(1, 2 ** 31, size=n)
+    res = S.run_replicas(params, sizes, seeds, steps=3000, discard=1500)
+    print(f"replicas: {n} boxes, {res['nbeads']} beads, {res['bead_steps_per_s']:.3e} bead-steps/s, "
+          f"wall {res['wall']:.1f} s", flush=True)
+    return params, sizes, res
+
+
+def test_replica_pressures_are_converged(dataset):
+    params, sizes, res = dataset
+    rel_se = res["P_se"] / res["P"]
+    print(f"pressure: {res['P'].min():.2f}..{res['P'].max():.2f}, relative s.e. median "
+          f"{np.median(rel_se):.2e} max {rel_se.max():.2e}", flush=True)
+    assert np.isfinite(res["P"]).all()
+    assert np.median(rel_se) < 0.005
+    assert np.all(np.abs(res["kT"] - 1.0) < 0.05)
+
+
+def test_surrogate_predicts_held_out_pressure(dataset):
+    params, sizes, res = dataset
+    Y = np.stack([res["P"], res["apl"]], axis=1)
+    ok = np.isfinite(Y).all(axis=1)
+    X, Y = params[ok], Y[ok]
+    ntr = int(0.8 * len(X))
+    net, losses = S.train(X[:ntr], Y[:ntr], iters=3000, lr=2e-3)
+    pred = net.predict(X[ntr:])
+    errP = np.abs(pred[:, 0] - Y[ntr:, 0]) / Y[ntr:, 0]
+    errA = np.abs(pred[:, 1] - Y[ntr:, 1]) / Y[ntr:, 1]
+    # the method without the network: predict the training mean
+    base = np.abs(Y[:ntr, 0].mean() - Y[ntr:, 0]) / Y[ntr:, 0]
+    print(f"held-out pressure error: mean {errP.mean():.4f} median {np.median(errP):.4f} max {errP.max():.4f} "
+          f"(training-mean predictor {base.mean():.4f}); area per lipid mean error {errA.mean():.4f}", flush=True)
+    assert losses[-1] < 0.05 * losses[0]
+    assert errP.mean() < 0.02
+    assert errP.mean() < 0.2 * base.mean()
+
+
+def test_screening_gain_and_simulation_judges(dataset):
+    params, sizes, res = dataset
+    Y = np.stack([res["P"], res["apl"]], axis=1)
+    ok = np.isfinite(Y).all(axis=1)
+    net, _ = S.train(params[ok], Y[ok], iters=3000, lr=2e-3)
+    rng = np.random.default_rng(99)
+    cand = S.sample_params(20000, rng)
+    target = float(np.median(res["P"]))
+    t0 = time.perf_counter()
+    keep, pred = S.screen(net, cand, target, tol=0.02)
+    t_pred = time.perf_counter() - t0
+    assert len(keep) > 0
+    judge = keep[:32]
+    L = rng.integers(8, 12, size=len(judge))
+    sim = S.run_replicas(cand[judge], L, rng.integers(1, 2 ** 31, size=len(judge)), steps=3000, discard=1500)
+    err = np.abs(pred[judge, 0] - sim["P"]) / sim["P"]
+    t_sim_per = sim["wall"] / len(judge)
+    t_pred_per = t_pred / len(cand)
+    print(f"screened {len(cand)} candidates in {t_pred * 1e3:.1f} ms, accepted {len(keep)}; judged {len(judge)}: "
+          f"mean error {err.mean():.4f}, within 2%: {np.mean(err < 0.02):.2f}; cost per candidate "
+          f"{t_pred_per * 1e6:.2f} us vs {t_sim_per * 1e3:.1f} ms simulated (x{t_sim_per / t_pred_per:.0f})",
+          flush=True)
+    assert err.mean() < 0.03
+    assert t_sim_per / t_pred_per > 1000

@@ -112,3 +112,24 @@ def test_restart_roundtrip(tmp_path):
     assert (r["id"] == d["id"]).all()
     for key in ("u", "v", "f"):
         assert np.allclose(r[key], d[key], rtol=5e-6, atol=0)
+
+
+def test_lamella_state_builder():
+    """State lamella (src/Builder.cpp:842): lipids in two leaflets about the mid-plane,
+    heads outwards, water outside the slab, bonds near their rest length."""
+    text = open(os.path.join(GOLDEN, "lipid8", "case.dmpci")).read().replace(
+        "restart RunId golden StateId 0",
+        "lamella Polymer Lipid Normal 0 0 1 Centre 0.5 Thickness 5.0 Linearise 1 UpperFraction 0.5 Polymerise 0")
+    inp = H.Input(text=text)
+    assert inp.state == "lamella"
+    x, u, v = inp.build_state(seed=9)
+    ids, t, pid, pt = inp.beads()
+    zc = 4.0
+    head = (pt == 1) & (t == 0)
+    tail = (pt == 1) & (t == 1)
+    water = pt == 0
+    assert np.abs(u[tail, 2] - zc).mean() < np.abs(u[head, 2] - zc).mean()
+    assert (np.abs(u[head, 2] - zc) <= 2.5 + 1e-9).all()
+    assert (np.abs(np.mod(u[water, 2] - zc + 4.0, 8.0) - 4.0) >= 2.5 - 1e-9).all()
+    (h, tl, k, l0), _ = inp.topology()
+    assert np.linalg.norm(u[h] - u[tl], axis=1).max() < 1.0
