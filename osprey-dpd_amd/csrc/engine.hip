@@ -946,7 +946,8 @@ int dpd_create_batch(int32_t nbox, const dpd_params* p, dpd_engine** out)
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (auto& ev : e->ev) HIPCHK(hipEventCreate(&ev));
     e->pair_kernel = (p[0].pair_kernel == DPD_PAIR_AUTO) ? default_pair_kernel() : p[0].pair_kernel;
-    if (e->pair_kernel != DPD_PAIR_BRICK && e->pair_kernel != DPD_PAIR_FULLSHELL)
+    if (e->pair_kernel != DPD_PAIR_BRICK && e->pair_kernel != DPD_PAIR_FULLSHELL &&
+        e->pair_kernel != DPD_PAIR_BRICK512)
         return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
     if ((rc = dalloc(e, &e->flags, 4)) || (rc = dalloc(e, &e->dstep, 1)) ||
         (rc = dalloc(e, &e->acc, (size_t)nbox * kObsStride)) || (rc = dalloc(e, &e->table, (size_t)nbox * kTableStride)) ||
@@ -1254,5 +1255,88 @@ int64_t dpd_last_timing(dpd_engine* e, double* ms3)
 }
 
 int64_t dpd_get_step(const dpd_engine* e) { return e ? e->step : -1; }
+
+int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (kernel == DPD_PAIR_AUTO) kernel = default_pair_kernel();
+    if (kernel != DPD_PAIR_BRICK && kernel != DPD_PAIR_FULLSHELL && kernel != DPD_PAIR_BRICK512)
+        return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
+    if (kernel != e->pair_kernel) drop_graphs(e);
+    e->pair_kernel = kernel;
+    return DPD_OK;
+}
+
+int32_t dpd_get_pair_kernel(const dpd_engine* e) { return e ? e->pair_kernel : -1; }
+
+int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (nsteps < 1) return fail(e, DPD_ERR_PARAM, "nsteps must be positive");
+    HIPCHK(hipSetDevice(e->device));
+    const int64_t n = e->n;
+    // snapshot of the state the trials start from
+    float4 *sp = nullptr, *sv = nullptr, *sfo = nullptr, *sf = nullptr, *svp = nullptr;
+    int4* sa = nullptr;
+    int *sk = nullptr, *sso = nullptr, *scs = nullptr;
+    int rc;
+    if ((rc = dalloc(e, &sp, n)) || (rc = dalloc(e, &sv, n)) || (rc = dalloc(e, &sfo, n)) || (rc = dalloc(e, &sf, n)) ||
+        (rc = dalloc(e, &svp, n)) || (rc = dalloc(e, &sa, n)) || (rc = dalloc(e, &sk, n)) ||
+        (rc = dalloc(e, &sso, n)) || (rc = dalloc(e, &scs, e->nkeys + 1)))
+        return rc;
+    const int c0 = e->cur;
+    const int64_t step0 = e->step;
+    auto copy = [&](bool save) -> int {
+        const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        const size_t f4 = sizeof(float4) * n;
+        if (save) {
+            HIPCHK(hipMemcpyAsync(sp, e->pos[c0], f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(sv, e->vel[c0], f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(sfo, e->fold[c0], f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(sf, e->frc, f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(svp, e->vp, f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(sa, e->aux[c0], sizeof(int4) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(sk, e->key[c0], sizeof(int) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(sso, e->slot_of, sizeof(int) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(scs, e->cstart, sizeof(int) * (e->nkeys + 1), k, e->stream));
+        } else {
+            HIPCHK(hipMemcpyAsync(e->pos[c0], sp, f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->vel[c0], sv, f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->fold[c0], sfo, f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->frc, sf, f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->vp, svp, f4, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->aux[c0], sa, sizeof(int4) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->key[c0], sk, sizeof(int) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->slot_of, sso, sizeof(int) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->cstart, scs, sizeof(int) * (e->nkeys + 1), k, e->stream));
+            hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, step0);
+            e->cur = c0;
+            e->step = step0;
+        }
+        HIPCHK(hipStreamSynchronize(e->stream));
+        return DPD_OK;
+    };
+    if ((rc = copy(true))) return rc;
+    const int32_t kinds[3] = {DPD_PAIR_BRICK, DPD_PAIR_FULLSHELL, DPD_PAIR_BRICK512};
+    double best = 1e300;
+    int32_t bestk = e->pair_kernel;
+    for (int v = 0; v < 3; ++v) {
+        if ((rc = dpd_set_pair_kernel(e, kinds[v]))) break;
+        if ((rc = dpd_step(e, 1, 0))) break;          // warm-up (code objects, caches)
+        if ((rc = copy(false))) break;
+        if ((rc = dpd_step(e, nsteps, 0))) break;
+        const double ms = e->t_total / nsteps;
+        if (ms_per_step) ms_per_step[v] = ms;
+        if (ms < best) { best = ms; bestk = kinds[v]; }
+        if ((rc = copy(false))) break;
+    }
+    void* tmp[] = {sp, sv, sfo, sf, svp, sa, sk, sso, scs};
+    for (void* q : tmp) (void)hipFree(q);
+    if (rc) return rc;
+    e->obs_valid = false;
+    if (chosen) *chosen = bestk;
+    return dpd_set_pair_kernel(e, bestk);
+}
 
 }  // extern "C"

@@ -25,8 +25,7 @@ namespace dpd {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
+constexpr int kMaxWaves = 8;       // up to 512 threads per workgroup
 constexpr int kMaxStaged = 1024;   // beads in brick + halo
 constexpr int kMaxHome = 384;      // beads in the brick
 constexpr int kStaged = 216;       // 6x6x6 staged cells
@@ -45,9 +44,9 @@ struct Smem {
     float fx[kMaxHome], fy[kMaxHome], fz[kMaxHome];
     int start[kStaged + 1];     // LDS index of each staged cell's first bead
     int gstart[kStaged];        // global slot of each staged cell's first bead
-    uint32_t queue[kWaves][kQueue];
+    uint32_t queue[kMaxWaves][kQueue];
     float table[kTableStride];
-    double red[10][kWaves];
+    double red[10][kMaxWaves];
     int nstaged;
 };
 
@@ -145,9 +144,11 @@ __device__ inline void fullshell_bead(const PairArgs& A, const BoxDev& B, const 
     A.frc[i] = make_float4(fx, fy, fz, 0.f);
 }
 
-template <bool OBS>
+template <int kThreads, bool OBS>
 __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, int per_xcd)
 {
+    constexpr int kWaves = kThreads / 64;
+    static_assert(kThreads >= kStaged && kWaves <= kMaxWaves, "one thread per staged cell");
     __shared__ Smem S;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -221,9 +222,11 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, int per_xcd
                 if (lane == 0) S.red[q][wave] = s;
             }
             __syncthreads();
-            if (tid < 10)
-                atomicAdd(A.acc + box * kObsStride + kObsVir + tid,
-                          S.red[tid][0] + S.red[tid][1] + S.red[tid][2] + S.red[tid][3]);
+            if (tid < 10) {
+                double t = 0.0;
+                for (int w = 0; w < kWaves; ++w) t += S.red[tid][w];
+                atomicAdd(A.acc + box * kObsStride + kObsVir + tid, t);
+            }
         }
         return;
     }
@@ -333,20 +336,27 @@ __global__ __launch_bounds__(kThreads) void k_pair_brick(PairArgs A, int per_xcd
             if (lane == 0) S.red[c][wave] = s;
         }
         __syncthreads();
-        if (tid < 10)
-            atomicAdd(A.acc + box * kObsStride + kObsVir + tid,
-                      S.red[tid][0] + S.red[tid][1] + S.red[tid][2] + S.red[tid][3]);
+        if (tid < 10) {
+            double t = 0.0;
+            for (int w = 0; w < kWaves; ++w) t += S.red[tid][w];
+            atomicAdd(A.acc + box * kObsStride + kObsVir + tid, t);
+        }
     }
 }
 
 }  // namespace
 
-hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s)
+hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s, int threads)
 {
     const int per_xcd = (a.nbricks + 7) / 8;
     const dim3 grid(8 * per_xcd);
-    if (obs) hipLaunchKernelGGL((k_pair_brick<true>), grid, dim3(kThreads), 0, s, a, per_xcd);
-    else hipLaunchKernelGGL((k_pair_brick<false>), grid, dim3(kThreads), 0, s, a, per_xcd);
+    if (threads == 512) {
+        if (obs) hipLaunchKernelGGL((k_pair_brick<512, true>), grid, dim3(512), 0, s, a, per_xcd);
+        else hipLaunchKernelGGL((k_pair_brick<512, false>), grid, dim3(512), 0, s, a, per_xcd);
+    } else {
+        if (obs) hipLaunchKernelGGL((k_pair_brick<256, true>), grid, dim3(256), 0, s, a, per_xcd);
+        else hipLaunchKernelGGL((k_pair_brick<256, false>), grid, dim3(256), 0, s, a, per_xcd);
+    }
     return hipGetLastError();
 }
 

@@ -111,11 +111,12 @@ __global__ __launch_bounds__(256) void k_pair_fullshell(PairArgs A, BoxDev B0)
 
 inline int default_pair_kernel() { return DPD_PAIR_FULLSHELL; }
 
-hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s);
+hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s, int threads);
 
 inline hipError_t launch_pair_kernel(int which, const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s)
 {
-    if (which == DPD_PAIR_BRICK) return launch_pair_brick(a, obs, s);
+    if (which == DPD_PAIR_BRICK) return launch_pair_brick(a, obs, s, 256);
+    if (which == DPD_PAIR_BRICK512) return launch_pair_brick(a, obs, s, 512);
     const int g = (a.n + 255) / 256;
     const bool multi = a.nbox > 1;
     if (obs) {

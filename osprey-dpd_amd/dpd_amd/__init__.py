@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libospreydpd_amd.so")
 OK = 0
 ERR_NAMES = {1: "PARAM", 2: "OOM", 3: "HIP", 4: "NAN", 5: "OUT_OF_BOX", 6: "STATE", 7: "IO",
              8: "UNSUPPORTED"}
-PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL = 0, 1, 2
+PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL, PAIR_BRICK512 = 0, 1, 2, 3
 STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH = 1, 2, 4, 8
 TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE = 1, 2
 
@@ -62,6 +62,9 @@ SIGNATURES = {
     "dpd_toggle": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "dpd_last_timing": (C.c_int64, [_P, _P]),
     "dpd_get_step": (C.c_int64, [_P]),
+    "dpd_set_pair_kernel": (C.c_int, [_P, C.c_int32]),
+    "dpd_get_pair_kernel": (C.c_int32, [_P]),
+    "dpd_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P]),
     "dpd_create_batch": (C.c_int, [C.c_int32, C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "dpd_upload_box": (C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P]),
     "dpd_upload_box_topology": (C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, C.c_int64, _P, _P,
@@ -211,6 +214,21 @@ class Engine:
     @property
     def step_count(self):
         return lib().dpd_get_step(self.h)
+
+    @property
+    def pair_kernel(self):
+        return int(lib().dpd_get_pair_kernel(self.h))
+
+    def set_pair_kernel(self, kernel):
+        self._check(lib().dpd_set_pair_kernel(self.h, int(kernel)))
+
+    def autotune(self, nsteps=10):
+        """Time every pair kernel on the current state; keep the fastest.  Returns the
+        choice and {kernel: ms per step}."""
+        chosen = C.c_int32()
+        ms = np.zeros(3)
+        self._check(lib().dpd_autotune(self.h, int(nsteps), C.byref(chosen), _ptr(ms)))
+        return chosen.value, {PAIR_BRICK: ms[0], PAIR_FULLSHELL: ms[1], PAIR_BRICK512: ms[2]}
 
 
 def _params(cells, a, gamma, dt, lam, kT, seed, widths, device, pair_kernel, keep):
