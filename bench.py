@@ -163,6 +163,49 @@ def cpu_baseline_reference(t1=2, t2=6):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def cpu_all_cores_reference(t1=2, t2=6, copies=None):
+    """Whole-host throughput of the reference binary: `copies` independent condensate runs
+    at once (one per core), first all of length t1, then all of length t2; per-step time
+    under the same contention = (mean wall t2 - mean wall t1)/(t2 - t1)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "dpd")
+    if not os.path.exists(exe):
+        return None
+    if copies is None:
+        copies = min(16, len(os.sched_getaffinity(0)))
+    work = tempfile.mkdtemp(prefix="dpd_cpuall_")
+    try:
+        mean_wall = {}
+        for T in (t1, t2):
+            procs = []
+            for c in range(copies):
+                tag = f"c{c}t{T}"
+                with open(os.path.join(work, "dmpci." + tag), "w") as fh:
+                    fh.write(CONDENSATE_DMPCI.format(T=T))
+                procs.append((time.perf_counter(), subprocess.Popen([exe, tag], cwd=work, stdout=subprocess.DEVNULL,
+                                                                    stderr=subprocess.DEVNULL)))
+            walls = []
+            for t0, p in procs:
+                if p.wait(timeout=900) != 0:
+                    return None
+                walls.append(time.perf_counter() - t0)
+            mean_wall[T] = float(np.mean(walls))
+            for name in os.listdir(work):
+                if not name.startswith(("dmpci", "dmpcis")):
+                    os.remove(os.path.join(work, name))
+        with open(os.path.join(work, f"dmpcis.c0t{t1}")) as fh:
+            nbeads = int(fh.read().split("Number of beads created")[1].split()[0])
+        per_step = (mean_wall[t2] - mean_wall[t1]) / (t2 - t1)
+        if per_step <= 0:
+            return None
+        return {"value": copies * nbeads / per_step, "unit": UNIT, "cores": copies,
+                "sample": f"{copies} concurrent reference runs of the condensate box, {t1} then {t2} steps, "
+                          f"differenced (mean wall {mean_wall[t1]:.1f} s, {mean_wall[t2]:.1f} s)"}
+    except Exception:
+        return None
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def cpu_baseline_port(wl, nsteps=3):
     """Fallback: the FP64 oracle's serial step on the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -217,6 +260,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_reference() or cpu_baseline_port(wl)
+        if cpu and cpu["kind"] == "reference":
+            cpu["all_cores"] = cpu_all_cores_reference()
 
     out = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
@@ -231,6 +276,7 @@ def main():
         "batched": batched,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "speedup_vs_cpu_all_cores": (value / cpu["all_cores"]["value"]) if cpu and cpu.get("all_cores") else None,
         "observables": {"kT": obs["kT"], "pressure": obs["pressure"], "jumps": obs["jumps"]},
     }
     print(json.dumps(out))

@@ -63,6 +63,10 @@ typedef struct dpd_params {
 #define DPD_PAIR_BRICK     1    /* LDS cell-brick kernel, 256 threads per brick      */
 #define DPD_PAIR_FULLSHELL 2    /* thread-per-bead 27-cell kernel (cross-check)      */
 #define DPD_PAIR_BRICK512  3    /* LDS cell-brick kernel, 512 threads per brick      */
+#define DPD_PAIR_STREAM    4    /* LDS cell-stream kernel, 512 threads per brick     */
+#define DPD_PAIR_STREAM1024 5   /* LDS cell-stream kernel, 1024 threads per brick    */
+#define DPD_PAIR_COMPACT   6    /* thread-per-bead 27-cell kernel, compacted pairs   */
+#define DPD_PAIR_KINDS     6    /* kernels are numbered 1..DPD_PAIR_KINDS            */
 
 /* FP64 observables of the last completed step (CMonitor::CalculateSingleBeadData,
  * src/Monitor.cpp:790-1041).  stress = (sum over pairs and bonds of r (x) F) / V,
@@ -165,8 +169,8 @@ int64_t dpd_get_step(const dpd_engine* e);
 
 /* Pair-kernel choice.  dpd_autotune times `nsteps` steps of every pair kernel on the
  * current state (restoring it after each trial, so the simulation does not advance),
- * keeps the fastest and returns it; ms_per_step[3] (may be NULL) receives the times of
- * DPD_PAIR_BRICK, DPD_PAIR_FULLSHELL and DPD_PAIR_BRICK512. */
+ * keeps the fastest and returns it; ms_per_step[DPD_PAIR_KINDS] (may be NULL) receives
+ * the time per step of kernel k at index k - 1. */
 int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel);
 int32_t dpd_get_pair_kernel(const dpd_engine* e);
 int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step);

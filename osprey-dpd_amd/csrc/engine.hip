@@ -175,43 +175,56 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (MODE != kRebin && s == 0) *step += 1;   // simulation time of the coming force evaluation
-    if (s >= n) return;
-    const float4 p = S.pos[s];
-    const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
-    float4 v = S.vel[s];
-    float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
-    const int4 ax = S.aux[s];
-    if (MODE == kEndBegin) {
-        double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
-        bonded_force(s, ax.w, S, slot_of, T, B, fb, vir, &peb, &pea, false);
-        f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
-        const float4 fo = S.fold[s];
-        v.x = v.x + B.hdt * (f.x + fo.x);
-        v.y = v.y + B.hdt * (f.y + fo.y);
-        v.z = v.z + B.hdt * (f.z + fo.z);
-        S.vel[s] = v;
-        frc[s] = f;
-        if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
-            atomicAdd(flags + 0, 1);
+    const bool active = s < n;
+    int k = 0;
+    if (active) {
+        const float4 p = S.pos[s];
+        const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
+        float4 v = S.vel[s];
+        float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
+        const int4 ax = S.aux[s];
+        if (MODE == kEndBegin) {
+            double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
+            bonded_force(s, ax.w, S, slot_of, T, B, fb, vir, &peb, &pea, false);
+            f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
+            const float4 fo = S.fold[s];
+            v.x = v.x + B.hdt * (f.x + fo.x);
+            v.y = v.y + B.hdt * (f.y + fo.y);
+            v.z = v.z + B.hdt * (f.z + fo.z);
+            S.vel[s] = v;
+            frc[s] = f;
+            if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
+                atomicAdd(flags + 0, 1);
+        }
+        int c[3];
+        key_cell(S.key[s] - B.koff, B, c);
+        const float o[3] = {p.x, p.y, p.z};
+        const int img[3] = {ax.x, ax.y, ax.z};
+        float d[3] = {0.f, 0.f, 0.f};
+        if (MODE != kRebin) {
+            d[0] = B.dt * v.x + B.hdt2 * f.x;
+            d[1] = B.dt * v.y + B.hdt2 * f.y;
+            d[2] = B.dt * v.z + B.hdt2 * f.z;
+        }
+        Moved m;
+        advance(o, c, img, d, B, m);
+        k = B.koff + cell_key(m.c[0], m.c[1], m.c[2], B);
+        nkey[s] = k;
+        if (m.jump) atomicAdd(flags + 1, 1);
+        if (m.bad) atomicAdd(flags + 0, 1);
     }
-    int c[3];
-    key_cell(S.key[s] - B.koff, B, c);
-    const float o[3] = {p.x, p.y, p.z};
-    const int img[3] = {ax.x, ax.y, ax.z};
-    float d[3] = {0.f, 0.f, 0.f};
-    if (MODE != kRebin) {
-        d[0] = B.dt * v.x + B.hdt2 * f.x;
-        d[1] = B.dt * v.y + B.hdt2 * f.y;
-        d[2] = B.dt * v.z + B.hdt2 * f.z;
-    }
-    Moved m;
-    advance(o, c, img, d, B, m);
-    const int k = B.koff + cell_key(m.c[0], m.c[1], m.c[2], B);
-    nkey[s] = k;
-    rank[s] = atomicAdd(cnt + k, 1);
-    atomicAdd(bcnt + (k >> 6), 1);
-    if (m.jump) atomicAdd(flags + 1, 1);
-    if (m.bad) atomicAdd(flags + 0, 1);
+    // Counting-sort ranks.  Beads arrive cell-sorted, so a wavefront's keys repeat: one
+    // atomic per distinct cell (and per distinct brick) instead of one per bead, which
+    // would serialise ~192 same-address atomics per brick counter.
+    const int lane = threadIdx.x & 63;
+    int r, size, leader;
+    wave_groups(k, active, r, size, leader);
+    int base = 0;
+    if (active && lane == leader) base = atomicAdd(cnt + k, size);
+    base = __shfl(base, leader, 64);
+    if (active) rank[s] = base + r;
+    wave_groups(k >> 6, active, r, size, leader);
+    if (active && lane == leader) atomicAdd(bcnt + (k >> 6), size);
 }
 
 // exclusive scan of brick counts (one workgroup), clears the counts
@@ -511,6 +524,8 @@ void set_step_constants(BoxHost& b)
 int check_params(dpd_engine* e, const dpd_params* p)
 {
     if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(e, DPD_ERR_PARAM, "need at least 3 cells per axis");
+    if ((int64_t)p->nx * p->ny * p->nz > (int64_t)1 << 28)
+        return fail(e, DPD_ERR_PARAM, "more than 2^28 cells in one box");
     if (p->wx < 1.0 || p->wy < 1.0 || p->wz < 1.0) return fail(e, DPD_ERR_PARAM, "cell widths must be >= 1");
     if (p->ntypes < 1 || p->ntypes > DPD_MAX_TYPES) return fail(e, DPD_ERR_PARAM, "ntypes out of range");
     if (!(p->dt > 0.0) || p->lambda < 0.0 || p->lambda > 1.0 || p->kT < 0.0)
@@ -932,6 +947,7 @@ int dpd_create_batch(int32_t nbox, const dpd_params* p, dpd_engine** out)
         BoxDev& B = h.B;
         B.n[0] = p[b].nx; B.n[1] = p[b].ny; B.n[2] = p[b].nz;
         for (int d = 0; d < 3; ++d) B.nb[d] = (B.n[d] + kBrick - 1) / kBrick;
+        for (int d = 0; d < 2; ++d) B.inb[d] = 1.0f / (float)B.nb[d];
         B.wd[0] = p[b].wx; B.wd[1] = p[b].wy; B.wd[2] = p[b].wz;
         for (int d = 0; d < 3; ++d) B.w[d] = (float)B.wd[d];
         B.ntypes = nt;
@@ -946,9 +962,7 @@ int dpd_create_batch(int32_t nbox, const dpd_params* p, dpd_engine** out)
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (auto& ev : e->ev) HIPCHK(hipEventCreate(&ev));
     e->pair_kernel = (p[0].pair_kernel == DPD_PAIR_AUTO) ? default_pair_kernel() : p[0].pair_kernel;
-    if (e->pair_kernel != DPD_PAIR_BRICK && e->pair_kernel != DPD_PAIR_FULLSHELL &&
-        e->pair_kernel != DPD_PAIR_BRICK512)
-        return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
+    if (e->pair_kernel < 1 || e->pair_kernel > DPD_PAIR_KINDS) return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
     if ((rc = dalloc(e, &e->flags, 4)) || (rc = dalloc(e, &e->dstep, 1)) ||
         (rc = dalloc(e, &e->acc, (size_t)nbox * kObsStride)) || (rc = dalloc(e, &e->table, (size_t)nbox * kTableStride)) ||
         (rc = dalloc(e, &e->dboxes, nbox)))
@@ -1260,8 +1274,7 @@ int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel)
 {
     if (!e) return DPD_ERR_PARAM;
     if (kernel == DPD_PAIR_AUTO) kernel = default_pair_kernel();
-    if (kernel != DPD_PAIR_BRICK && kernel != DPD_PAIR_FULLSHELL && kernel != DPD_PAIR_BRICK512)
-        return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
+    if (kernel < 1 || kernel > DPD_PAIR_KINDS) return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
     if (kernel != e->pair_kernel) drop_graphs(e);
     e->pair_kernel = kernel;
     return DPD_OK;
@@ -1318,17 +1331,16 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
         return DPD_OK;
     };
     if ((rc = copy(true))) return rc;
-    const int32_t kinds[3] = {DPD_PAIR_BRICK, DPD_PAIR_FULLSHELL, DPD_PAIR_BRICK512};
     double best = 1e300;
     int32_t bestk = e->pair_kernel;
-    for (int v = 0; v < 3; ++v) {
-        if ((rc = dpd_set_pair_kernel(e, kinds[v]))) break;
+    for (int v = 0; v < DPD_PAIR_KINDS; ++v) {
+        if ((rc = dpd_set_pair_kernel(e, v + 1))) break;
         if ((rc = dpd_step(e, 1, 0))) break;          // warm-up (code objects, caches)
         if ((rc = copy(false))) break;
         if ((rc = dpd_step(e, nsteps, 0))) break;
         const double ms = e->t_total / nsteps;
         if (ms_per_step) ms_per_step[v] = ms;
-        if (ms < best) { best = ms; bestk = kinds[v]; }
+        if (ms < best) { best = ms; bestk = v + 1; }
         if ((rc = copy(false))) break;
     }
     void* tmp[] = {sp, sv, sfo, sf, svp, sa, sk, sso, scs};

@@ -31,6 +31,7 @@ constexpr int kObsStride = 32;                       // FP64 accumulators per bo
 struct BoxDev {
     int n[3];          // cells per axis
     int nb[3];         // bricks per axis
+    float inb[2];      // 1 / nb[0], 1 / nb[1] (division by reciprocal in key_cell)
     float w[3];        // cell widths
     double wd[3];
     float dt, hdt, hdt2, lamdt;   // dt, dt/2, dt^2/2, lambda*dt
@@ -50,14 +51,24 @@ __host__ __device__ inline int cell_key(int cx, int cy, int cz, const BoxDev& B)
     return brick * kBrickCells + (((cz & 3) * 4 + (cy & 3)) * 4 + (cx & 3));
 }
 
+// x / d for 0 <= x < 2^24, d >= 1, given inv = 1/d rounded to FP32: the FP32 estimate is
+// within one of the quotient, and one correction step makes it exact.
+__host__ __device__ inline int div_small(int x, int d, float inv)
+{
+    int q = (int)((float)x * inv);
+    const int r = x - q * d;
+    q += (r >= d) ? 1 : ((r < 0) ? -1 : 0);
+    return q;
+}
+
 // cell of a local key
 __host__ __device__ inline void key_cell(int key, const BoxDev& B, int c[3])
 {
     const int brick = key >> 6, l = key & 63;
-    const int bx = brick % B.nb[0];
-    const int t = brick / B.nb[0];
-    const int by = t % B.nb[1];
-    const int bz = t / B.nb[1];
+    const int t = div_small(brick, B.nb[0], B.inb[0]);
+    const int bx = brick - t * B.nb[0];
+    const int bz = div_small(t, B.nb[1], B.inb[1]);
+    const int by = t - bz * B.nb[1];
     c[0] = bx * 4 + (l & 3);
     c[1] = by * 4 + ((l >> 2) & 3);
     c[2] = bz * 4 + (l >> 4);
@@ -111,12 +122,13 @@ __device__ inline float pair_uniform(uint32_t ida, uint32_t idb, const RngKey& K
 
 // DPD pair force on bead i from bead j (src/CNTCell.cpp:1191-1225):
 //   F = [a w + (-gamma w^2 (r^.v)) + sqrt(gamma) w sqrt(24kT/dt) (0.5 - U)] r^,  w = 1 - r
-// returns the scalar F/r so that F_vec = s * dx.
+// returns the scalar F/r so that F_vec = s * dx.  1/r is the hardware reciprocal square
+// root (1 ulp) and r = r^2 / r; callers guarantee r^2 > 1e-18 (no denormals).
 __device__ inline float pair_scalar(float dr2, float dxdv, float a, float sg, float g, uint32_t idi,
                                     uint32_t idj, float invrootdt, const RngKey& K, float* pe)
 {
-    const float dr = sqrtf(dr2);
-    const float inv = 1.0f / dr;
+    const float inv = __builtin_amdgcn_rsqf(dr2);
+    const float dr = dr2 * inv;
     const float wr = 1.0f - dr;
     const float con = a * wr;
     const float diss = -(g * wr * wr) * (dxdv * inv);
@@ -152,7 +164,7 @@ __device__ inline void advance(const float o[3], const int c[3], const int img[3
         if (x < 0.0f) x = 0.0f;
         m.jump |= (s > 1 || s < -1);
         int cc = c[a] + s;
-        const int q = floordiv(cc, B.n[a]);
+        const int q = (cc >= 0 && cc < B.n[a]) ? 0 : floordiv(cc, B.n[a]);
         cc -= q * B.n[a];
         m.o[a] = x;
         m.c[a] = cc;
@@ -166,6 +178,36 @@ __device__ inline double wave_sum(double v)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
+}
+
+// number of lanes below this one whose bit is set in mask
+__device__ inline int lane_prefix(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+// Active lanes of a wavefront holding equal keys form a group: this lane's rank in its
+// group, the group's size and its lowest lane.  Must be reached by the whole wavefront.
+// One pass per distinct key, with wave-uniform bookkeeping; a wavefront of cell-sorted
+// beads holds ~20 distinct cells.
+__device__ inline void wave_groups(int key, bool active, int& rank, int& size, int& leader)
+{
+    uint64_t todo = __ballot(active);
+    rank = 0;
+    size = 0;
+    leader = 0;
+    while (todo) {
+        const int first = __builtin_ctzll(todo);
+        const int k0 = __builtin_amdgcn_readlane(key, first);
+        const bool mine = active && key == k0;
+        const uint64_t m = __ballot(mine);
+        if (mine) {
+            rank = lane_prefix(m);
+            size = __popcll(m);
+            leader = first;
+        }
+        todo &= ~m;
+    }
 }
 
 // Add per-thread FP64 values v[0..N) into acc[box * kObsStride + base + i]: one

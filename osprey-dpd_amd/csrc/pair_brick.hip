@@ -33,11 +33,6 @@ constexpr int kQueue = 128;        // per-wavefront pair queue
 
 __device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
 
-__device__ inline int lane_prefix(uint64_t mask)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-}
-
 struct Smem {
     float4 pos[kMaxStaged];     // brick-centred position xyz, w = type | local cell << 8
     float4 vp[kMaxStaged];      // predicted velocity xyz, w = id

@@ -8,6 +8,10 @@
 //                       halo are staged in LDS, candidate pairs are filtered by r < 1 and
 //                       compacted per wavefront, and the surviving pairs are evaluated
 //                       64 at a time with both forces accumulated in LDS (pair_brick.hip).
+//   DPD_PAIR_STREAM     the brick layout with a flat (bead, candidate) stream per home
+//                       cell, so the candidate test runs on full wavefronts (pair_stream.hip).
+//   DPD_PAIR_COMPACT    the full-shell walk with pairs in range compacted per wavefront
+//                       and evaluated 64 at a time (pair_compact.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -38,9 +42,12 @@ constexpr int kObsPEP = 15;
 // same operands (dx negated), so the two forces are exact negatives of each other;
 // virial and PE are halved.
 template <bool OBS, bool MULTI>
-__global__ __launch_bounds__(256) void k_pair_fullshell(PairArgs A, BoxDev B0)
+__global__ __launch_bounds__(256) void k_pair_fullshell(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-aware: workgroup b runs on XCD b % 8 and takes block (b % 8) * per_xcd + b / 8
+    const int blk = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (blk >= nblocks) return;
+    const int i = blk * 256 + threadIdx.x;
     double vir[10];
     if (OBS)
         for (int q = 0; q < 10; ++q) vir[q] = 0.0;
@@ -104,7 +111,7 @@ __global__ __launch_bounds__(256) void k_pair_fullshell(PairArgs A, BoxDev B0)
             for (int q = 0; q < 10; ++q) vir[q] = 0.5 * (double)sv[q];
     }
     if (OBS) {
-        const int box0 = MULTI ? bead_box(__float_as_int(A.pos[blockIdx.x * blockDim.x].w)) : 0;
+        const int box0 = MULTI ? bead_box(__float_as_int(A.pos[blk * 256].w)) : 0;
         accumulate_boxes<10>(vir, box, box0, kObsVir, A.acc);
     }
 }
@@ -112,19 +119,26 @@ __global__ __launch_bounds__(256) void k_pair_fullshell(PairArgs A, BoxDev B0)
 inline int default_pair_kernel() { return DPD_PAIR_FULLSHELL; }
 
 hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s, int threads);
+hipError_t launch_pair_stream(const PairArgs& a, bool obs, hipStream_t s, int threads);
+hipError_t launch_pair_compact(const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s);
 
 inline hipError_t launch_pair_kernel(int which, const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s)
 {
     if (which == DPD_PAIR_BRICK) return launch_pair_brick(a, obs, s, 256);
     if (which == DPD_PAIR_BRICK512) return launch_pair_brick(a, obs, s, 512);
+    if (which == DPD_PAIR_STREAM) return launch_pair_stream(a, obs, s, 512);
+    if (which == DPD_PAIR_STREAM1024) return launch_pair_stream(a, obs, s, 1024);
+    if (which == DPD_PAIR_COMPACT) return launch_pair_compact(a, B0, obs, s);
     const int g = (a.n + 255) / 256;
+    const int per_xcd = (g + 7) / 8;
+    const dim3 grid(8 * per_xcd);
     const bool multi = a.nbox > 1;
     if (obs) {
-        if (multi) hipLaunchKernelGGL((k_pair_fullshell<true, true>), dim3(g), dim3(256), 0, s, a, B0);
-        else hipLaunchKernelGGL((k_pair_fullshell<true, false>), dim3(g), dim3(256), 0, s, a, B0);
+        if (multi) hipLaunchKernelGGL((k_pair_fullshell<true, true>), grid, dim3(256), 0, s, a, B0, g, per_xcd);
+        else hipLaunchKernelGGL((k_pair_fullshell<true, false>), grid, dim3(256), 0, s, a, B0, g, per_xcd);
     } else {
-        if (multi) hipLaunchKernelGGL((k_pair_fullshell<false, true>), dim3(g), dim3(256), 0, s, a, B0);
-        else hipLaunchKernelGGL((k_pair_fullshell<false, false>), dim3(g), dim3(256), 0, s, a, B0);
+        if (multi) hipLaunchKernelGGL((k_pair_fullshell<false, true>), grid, dim3(256), 0, s, a, B0, g, per_xcd);
+        else hipLaunchKernelGGL((k_pair_fullshell<false, false>), grid, dim3(256), 0, s, a, B0, g, per_xcd);
     }
     return hipGetLastError();
 }

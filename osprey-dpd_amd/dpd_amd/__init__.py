@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(PKG, "lib", "libospreydpd_amd.so")
 OK = 0
 ERR_NAMES = {1: "PARAM", 2: "OOM", 3: "HIP", 4: "NAN", 5: "OUT_OF_BOX", 6: "STATE", 7: "IO",
              8: "UNSUPPORTED"}
-PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL, PAIR_BRICK512 = 0, 1, 2, 3
+PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL, PAIR_BRICK512, PAIR_STREAM, PAIR_STREAM1024, PAIR_COMPACT = 0, 1, 2, 3, 4, 5, 6
+PAIR_KINDS = 6
 STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH = 1, 2, 4, 8
 TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE = 1, 2
 
@@ -226,9 +227,9 @@ class Engine:
         """Time every pair kernel on the current state; keep the fastest.  Returns the
         choice and {kernel: ms per step}."""
         chosen = C.c_int32()
-        ms = np.zeros(3)
+        ms = np.zeros(PAIR_KINDS)
         self._check(lib().dpd_autotune(self.h, int(nsteps), C.byref(chosen), _ptr(ms)))
-        return chosen.value, {PAIR_BRICK: ms[0], PAIR_FULLSHELL: ms[1], PAIR_BRICK512: ms[2]}
+        return chosen.value, {k + 1: float(ms[k]) for k in range(PAIR_KINDS)}
 
 
 def _params(cells, a, gamma, dt, lam, kT, seed, widths, device, pair_kernel, keep):

@@ -4,7 +4,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import Engine, PAIR_AUTO, _ptr, lib
+from . import Batch, Engine, PAIR_AUTO, _ptr, lib
 
 STATE_NAMES = {0: "random", 1: "restart", 2: "lamella", 3: "other"}
 
@@ -142,6 +142,30 @@ class Input:
         c = self.counts
         return Engine(self.cells, self.a, self.gamma, dt=c["dt"], lam=c["lam"], kT=c["kT"],
                       seed=abs(c["seed"]), widths=self.widths, device=device, pair_kernel=pair_kernel)
+
+
+def replica_batch(inputs, seeds, device=0, pair_kernel=PAIR_AUTO):
+    """One batched engine with a box per (input, seed): box i runs inputs[i]'s control
+    file from the starting state built with seeds[i], and draws its pair random stream
+    from the same seed.  The same Input may appear several times.  Returns the engine
+    (committed at step 0) and per-box (bead types, polymer types, box lengths)."""
+    boxes = []
+    for inp, s in zip(inputs, seeds):
+        c = inp.counts
+        boxes.append({"cells": inp.cells, "a": inp.a, "gamma": inp.gamma, "dt": c["dt"], "lam": c["lam"],
+                      "kT": c["kT"], "seed": int(s), "widths": inp.widths})
+    e = Batch(boxes, device=device, pair_kernel=pair_kernel)
+    meta = []
+    for b, (inp, s) in enumerate(zip(inputs, seeds)):
+        ids, types, pid, pt = inp.beads()
+        x, u, v = inp.build_state(seed=int(s))
+        e.upload_box(b, ids, types, x, v, unpbc=u)
+        (h, t, k, l0), (fi, se, mo, ph) = inp.topology()
+        if len(h) or len(fi):
+            e.upload_box_topology(b, h, t, k, l0, fi, se, mo, ph)
+        meta.append((types, pt, np.array(inp.cells, float) * np.array(inp.widths)))
+    e.commit(0)
+    return e, meta
 
 
 def read_restart(path, n):

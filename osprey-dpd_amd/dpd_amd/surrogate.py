@@ -12,9 +12,9 @@ import time
 
 import numpy as np
 
-from . import Batch, PAIR_AUTO
+from . import PAIR_AUTO
 from . import analysis as A
-from .host import Input
+from .host import Input, replica_batch
 from .mlp import Surrogate
 
 FEATURES = ("a_HW", "a_TW", "a_HT", "a_TT", "lipid_fraction", "k_bond")
@@ -91,19 +91,8 @@ def replica_text(p, L, seed, steps=5000, dt=0.02):
 def build_batch(params, sizes, seeds, device=0, pair_kernel=PAIR_AUTO, steps=5000):
     """One batched engine holding a replica per parameter row."""
     inputs = [Input(text=replica_text(p, L, s, steps)) for p, L, s in zip(params, sizes, seeds)]
-    boxes = [{"cells": inp.cells, "a": inp.a, "gamma": inp.gamma, "dt": inp.counts["dt"],
-              "lam": inp.counts["lam"], "kT": inp.counts["kT"], "seed": abs(inp.counts["seed"])}
-             for inp in inputs]
-    e = Batch(boxes, device=device, pair_kernel=pair_kernel)
-    meta = []
-    for i, inp in enumerate(inputs):
-        ids, types, pid, pt = inp.beads()
-        x, u, v = inp.build_state(seed=int(seeds[i]))
-        e.upload_box(i, ids, types, x, v, unpbc=u)
-        (h, t, k, l0), (fi, se, mo, ph) = inp.topology()
-        e.upload_box_topology(i, h, t, k, l0, fi, se, mo, ph)
-        meta.append({"types": types, "L": np.array(inp.cells, float), "nlipids": int((pt == 1).sum() // 11)})
-    e.commit(0)
+    e, boxes = replica_batch(inputs, [int(s) for s in seeds], device=device, pair_kernel=pair_kernel)
+    meta = [{"types": t, "L": L, "nlipids": int((pt == 1).sum() // 11)} for t, pt, L in boxes]
     return e, meta
 
 

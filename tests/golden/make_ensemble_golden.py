@@ -14,9 +14,19 @@ Cases:
             cells, a_HT 35, a_TW 75, a_HW 35, others 25, started as a lamella
             (State lamella), 4000 steps: kT, P, density profiles along the normal and
             the mesh area per lipid from snapshots every 250 steps.
-Statistics use the second half (water: after step 1000, bilayer: after step 2000),
-block-averaged; results go to tests/golden/ensemble/<case>.json together with the
-control file used.
+Each case runs NRUNS times from independent starting states (RNGSeed differs per run).
+The bilayer runs start from this build's lamella states (dpd_amd.host.Input.build_state
+with the run's seed), handed to the reference as restart snapshots (State restart), so
+that the engine and the reference are compared from identical structures; NRUNS more
+bilayer runs use the reference's own lamella builder, and their kT and P are kept as
+"reference_builder_runs" to record how much the starting structure alone moves the
+4000-step averages.
+Per run, kT and P are block-averaged over the second part of the run (water: after step
+1000, bilayer: after step 2000); the fixtures keep every run's mean so that comparisons
+use the run-to-run spread, which for a bilayer relaxing over a few thousand steps is
+larger than a single run's block error.  g(r), density profiles and the area per lipid
+pool the snapshots of all runs.  Results go to tests/golden/ensemble/<case>.json together
+with the control file of the first run.
 """
 import json
 import os
@@ -24,6 +34,7 @@ import shutil
 import subprocess
 import sys
 import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -31,9 +42,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "osprey-dpd_amd"))
 from dpd_amd import analysis as A  # noqa: E402
+from dpd_amd import host as H  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "dpd")
 OUT = os.path.join(HERE, "ensemble")
+NRUNS = 4
+WATER_SEEDS = [20261202 + 7919 * r for r in range(NRUNS)]
+BILAYER_SEEDS = [777 + 7919 * r for r in range(NRUNS)]
 
 WATER = """dpd
 
@@ -53,7 +68,7 @@ Polymer	Water    1.0   " (W) "
 Box         20 20 20       1  1  1
 Density		3
 Temp        1
-RNGSeed		-20261202
+RNGSeed		-{seed}
 Lambda		0.5
 Step		0.02
 Time		5000
@@ -108,7 +123,7 @@ Polymer	Lipid    0.04667   " (H H (* (T T T T)) H T T T T) "
 Box         16 16 16       1  1  1
 Density		3
 Temp        1
-RNGSeed		-777
+RNGSeed		-{seed}
 Lambda		0.5
 Step		0.02
 Time		4000
@@ -151,42 +166,91 @@ def stats(series, nblocks=10):
     return [m, se]
 
 
-def water(work):
-    run(work, "w", WATER)
-    h = history(os.path.join(work, "dmpchs.w"))
+def water_run(work, r):
+    rid = f"w{r}"
+    run(work, rid, WATER.format(seed=WATER_SEEDS[r]))
+    h = history(os.path.join(work, "dmpchs." + rid))
     h = h[h[:, 0] > 1000]
     L = np.array([20.0, 20.0, 20.0])
     gs = []
     for t in range(1250, 5001, 250):
-        _, x = snapshot(os.path.join(work, f"dmpcrs.w.con.{t}.dat"))
-        r, g = A.rdf(x, L)
+        _, x = snapshot(os.path.join(work, f"dmpcrs.{rid}.con.{t}.dat"))
+        r_, g = A.rdf(x, L)
         gs.append(g)
-    gs = np.array(gs)
-    res = {"kT": stats(h[:, 1]), "P": stats(h[:, 2]), "rdf_r": r.tolist(), "rdf_g": gs.mean(0).tolist(),
-           "rdf_se": (gs.std(0, ddof=1) / np.sqrt(len(gs))).tolist(), "nsnap": len(gs),
-           "steps": 5000, "discard": 1000}
-    return res, WATER
+    return {"kT": stats(h[:, 1]), "P": stats(h[:, 2]), "r": r_, "g": gs}
 
 
-def bilayer(work):
-    run(work, "b", BILAYER)
-    h = history(os.path.join(work, "dmpchs.b"))
+def restart_text(text, run_id):
+    """The control file with its State block replaced by a restart from run_id, step 0."""
+    out, skip = [], False
+    for line in text.split("\n"):
+        if line.startswith("State"):
+            out.append(f"State\trestart RunId {run_id} StateId 0")
+            skip = True
+            continue
+        if skip and line[:1] in (" ", "\t") and line.strip():
+            continue
+        skip = False
+        out.append(line)
+    return "\n".join(out)
+
+
+def own_start(work, run_id, text, seed):
+    """Write this build's starting state for `text` (seed) as the restart snapshot of
+    run_id at step 0, next to the control file the reference checks a restart against."""
+    inp = H.Input(text=text)
+    ids, types, pid, pt = inp.beads()
+    x, u, v = inp.build_state(seed=seed)
+    L = np.array(inp.cells, float) * np.array(inp.widths)
+    H.write_restart(os.path.join(work, f"dmpcrs.{run_id}.con.0.dat"), pid, pt, ids, types, L, x, u, v,
+                    np.zeros_like(x))
+    with open(os.path.join(work, "dmpci." + run_id), "w") as fh:
+        fh.write(text)
+
+
+def bilayer_run(work, r, own=True):
+    rid = f"b{r}" if own else f"rb{r}"
+    text = BILAYER.format(seed=BILAYER_SEEDS[r])
+    if own:
+        own_start(work, f"own{r}", text, BILAYER_SEEDS[r])
+        text = restart_text(text, f"own{r}")
+    run(work, rid, text)
+    h = history(os.path.join(work, "dmpchs." + rid))
     h = h[h[:, 0] > 2000]
     L = np.array([16.0, 16.0, 16.0])
     profs, apls = [], []
-    nlip = None
     for t in range(2250, 4001, 250):
-        types, x = snapshot(os.path.join(work, f"dmpcrs.b.con.{t}.dat"))
+        types, x = snapshot(os.path.join(work, f"dmpcrs.{rid}.con.{t}.dat"))
         z, p = A.density_profile(x, types, L, centre_type=1, dz=0.5, ntypes=3)
         profs.append(p)
         nlip = int((types == 0).sum() // 3)
         apls.append(A.area_per_lipid(x, types, L, 0, 1, nlip))
-    profs = np.array(profs)
-    res = {"kT": stats(h[:, 1]), "P": stats(h[:, 2]), "z": z.tolist(), "profile": profs.mean(0).tolist(),
-           "profile_se": (profs.std(0, ddof=1) / np.sqrt(len(profs))).tolist(),
-           "apl": stats(apls, nblocks=len(apls)), "nlipids": nlip, "nsnap": len(profs),
-           "types": ["H", "T", "W"], "steps": 4000, "discard": 2000}
-    return res, BILAYER
+    return {"kT": stats(h[:, 1]), "P": stats(h[:, 2]), "z": z, "prof": profs, "apl": apls, "nlip": nlip}
+
+
+def pooled(samples):
+    a = np.array(samples)
+    return a.mean(0).tolist(), (a.std(0, ddof=1) / np.sqrt(len(a))).tolist()
+
+
+def water(runs):
+    g, gse = pooled([g for r in runs for g in r["g"]])
+    return {"runs": [{"kT": r["kT"], "P": r["P"]} for r in runs], "seeds": WATER_SEEDS,
+            "rdf_r": runs[0]["r"].tolist(), "rdf_g": g, "rdf_se": gse, "nsnap": sum(len(r["g"]) for r in runs),
+            "steps": 5000, "discard": 1000}, WATER.format(seed=WATER_SEEDS[0])
+
+
+def bilayer(runs, refb):
+    p, pse = pooled([p for r in runs for p in r["prof"]])
+    apl = [a for r in runs for a in r["apl"]]
+    return {"runs": [{"kT": r["kT"], "P": r["P"], "apl": float(np.mean(r["apl"]))} for r in runs],
+            "seeds": BILAYER_SEEDS, "start": "dpd_amd lamella builder, seed per run", "z": runs[0]["z"].tolist(),
+            "profile": p, "profile_se": pse,
+            "apl": [float(np.mean(apl)), float(np.std(apl, ddof=1) / np.sqrt(len(apl)))],
+            "nlipids": runs[0]["nlip"], "nsnap": len(apl), "types": ["H", "T", "W"], "steps": 4000,
+            "discard": 2000,
+            "reference_builder_runs": [{"kT": r["kT"], "P": r["P"], "apl": float(np.mean(r["apl"]))} for r in refb]}, \
+        BILAYER.format(seed=BILAYER_SEEDS[0])
 
 
 def main():
@@ -195,13 +259,18 @@ def main():
     os.makedirs(OUT, exist_ok=True)
     work = tempfile.mkdtemp(prefix="dpd_ens_")
     try:
-        for name, fn in (("water20", water), ("bilayer16", bilayer)):
-            res, text = fn(work)
+        with ThreadPoolExecutor(max_workers=8) as pool:
+            wf = [pool.submit(water_run, work, r) for r in range(NRUNS)]
+            bf = [pool.submit(bilayer_run, work, r) for r in range(NRUNS)]
+            rf = [pool.submit(bilayer_run, work, r, False) for r in range(NRUNS)]
+            cases = (("water20", water([f.result() for f in wf])),
+                     ("bilayer16", bilayer([f.result() for f in bf], [f.result() for f in rf])))
+        for name, (res, text) in cases:
             with open(os.path.join(OUT, name + ".json"), "w") as fh:
                 json.dump(res, fh, indent=1)
             with open(os.path.join(OUT, name + ".dmpci"), "w") as fh:
                 fh.write(text)
-            print(name, {k: res[k] for k in ("kT", "P")}, flush=True)
+            print(name, [(r["kT"][0], r["P"][0]) for r in res["runs"]], flush=True)
     finally:
         shutil.rmtree(work, ignore_errors=True)
 

@@ -1,7 +1,12 @@
 """Statistical parity of thermostatted runs: the GPU engine (counter-based pair random
-stream) against long runs of the reference (serial LCG stream) on the same control
-files.  Criteria: kT within 0.5%, pressure within 1%, g(r) and bilayer density profiles
-within their combined block-averaged error bars."""
+stream) against several independent runs of the reference (serial LCG stream) on the
+same control files.  The GPU side runs 8 replicas from independent starting states in
+one batched engine.  Bilayer runs on both sides start from this build's lamella states
+(the reference reads them as restart snapshots), since the starting structure alone
+moves 4000-step bilayer averages by ~0.2% in pressure.  Criteria: kT within 0.5% and
+pressure within 1% of the reference ensemble mean, and both consistent with it within 4
+standard errors of the run-to-run spread; g(r) and bilayer density profiles within their
+combined error bars."""
 import json
 import os
 
@@ -14,70 +19,76 @@ from helpers import GOLDEN
 
 pytestmark = pytest.mark.gpu
 ENS = os.path.join(GOLDEN, "ensemble")
+REPLICAS = 8
 
 
-def _run(name, snap_from, snap_every, seed=12345):
+def _run(name, discard, snap_from, snap_every, seeds):
     inp = H.Input(os.path.join(ENS, name + ".dmpci"))
-    c = inp.counts
-    ids, types, pid, pt = inp.beads()
-    x, u, v = inp.build_state(seed=seed)
-    e = inp.engine(pair_kernel=0)
-    e.upload(ids, types, x, v, unpbc=u)
-    (h, t, k, l0), (fi, se, mo, ph) = inp.topology()
-    if len(h):
-        e.upload_topology(h, t, k, l0, fi, se, mo, ph)
-    L = np.array(inp.cells, float)
-    hist, snaps = [], []
-    steps = c["total_time"]
+    e, meta = H.replica_batch([inp] * REPLICAS, seeds, pair_kernel=0)
+    steps = inp.counts["total_time"]
+    hist = [[] for _ in range(REPLICAS)]
+    snaps = [[] for _ in range(REPLICAS)]
     for s in range(10, steps + 1, 10):
-        e.step(10, observe=True)
-        o = e.observables()
-        hist.append((s, o["kT"], o["pressure"]))
+        e.step(10, observe=True, graph=True)
+        for b in range(REPLICAS):
+            o = e.observables(b)
+            if s > discard:
+                hist[b].append((o["kT"], o["pressure"]))
         if s >= snap_from and s % snap_every == 0:
-            snaps.append(e.download()[0])
-    assert o["jumps"] == 0
-    return np.array(hist), snaps, types, L
+            for b in range(REPLICAS):
+                snaps[b].append(e.download_box(b)[0])
+    assert e.observables(0)["jumps"] == 0
+    runs = [np.array(h) for h in hist]
+    return runs, snaps, meta[0][0], meta[0][2]
 
 
-def _close(name, gpu, ref, rel, nsig=4.0):
-    m, se = gpu
-    mr, ser = ref
-    print(f"  {name}: gpu {m:.5f} +- {se:.5f}  reference {mr:.5f} +- {ser:.5f}", flush=True)
+def _ensemble(values):
+    v = np.asarray(values, float)
+    return v.mean(), v.std(ddof=1) / np.sqrt(len(v))
+
+
+def _close(name, gpu_runs, ref_runs, rel, nsig=4.0):
+    m, se = _ensemble(gpu_runs)
+    mr, ser = _ensemble(ref_runs)
+    print(f"  {name}: gpu {m:.5f} +- {se:.5f} ({len(gpu_runs)} runs)  reference {mr:.5f} +- {ser:.5f} "
+          f"({len(ref_runs)} runs)  rel diff {abs(m - mr) / abs(mr):.2e}", flush=True)
     assert abs(m - mr) / abs(mr) < rel
-    assert abs(m - mr) < nsig * np.hypot(se, ser) + 1e-12 or abs(m - mr) / abs(mr) < 0.25 * rel
+    assert abs(m - mr) < nsig * np.hypot(se, ser)
 
 
 def test_water_ensemble():
     ref = json.load(open(os.path.join(ENS, "water20.json")))
-    hist, snaps, types, L = _run("water20", 1250, 250)
-    h = hist[hist[:, 0] > ref["discard"]]
-    _close("kT", A.block_average(h[:, 1]), ref["kT"], 0.005)
-    _close("P", A.block_average(h[:, 2]), ref["P"], 0.01)
-    gs = np.array([A.rdf(x, L)[1] for x in snaps])
+    runs, snaps, types, L = _run("water20", ref["discard"], 1250, 250, [4242 + 104729 * r for r in range(REPLICAS)])
+    _close("kT", [A.block_average(r[:, 0])[0] for r in runs], [r["kT"][0] for r in ref["runs"]], 0.005)
+    _close("P", [A.block_average(r[:, 1])[0] for r in runs], [r["P"][0] for r in ref["runs"]], 0.01)
+    gs = np.array([A.rdf(x, L)[1] for s in snaps for x in s])
     g, gse = gs.mean(0), gs.std(0, ddof=1) / np.sqrt(len(gs))
     r = np.array(ref["rdf_r"])
     sel = r > 0.3
-    z = np.abs(g - np.array(ref["rdf_g"]))[sel] / np.hypot(gse, np.array(ref["rdf_se"]))[sel]
-    print(f"  g(r): max |dg|/sigma {z.max():.2f}, max |dg| {np.abs(g - ref['rdf_g'])[sel].max():.4f}", flush=True)
-    assert np.abs(g - np.array(ref["rdf_g"]))[sel].max() < 0.02
+    dg = np.abs(g - np.array(ref["rdf_g"]))[sel]
+    z = dg / np.hypot(gse, np.array(ref["rdf_se"]))[sel]
+    print(f"  g(r): max |dg| {dg.max():.4f}, mean |dg|/sigma {z.mean():.2f}, max {z.max():.2f}", flush=True)
+    assert dg.max() < 0.02
     assert np.mean(z) < 2.0
 
 
 def test_bilayer_ensemble():
     ref = json.load(open(os.path.join(ENS, "bilayer16.json")))
-    hist, snaps, types, L = _run("bilayer16", 2250, 250)
-    h = hist[hist[:, 0] > ref["discard"]]
-    _close("kT", A.block_average(h[:, 1]), ref["kT"], 0.005)
-    _close("P", A.block_average(h[:, 2]), ref["P"], 0.01)
-    profs = np.array([A.density_profile(x, types, L, centre_type=1, dz=0.5, ntypes=3)[1] for x in snaps])
+    # the reference runs started from this build's lamella states with these seeds: the
+    # first replicas start from the same states, the rest from further seeds
+    seeds = list(ref["seeds"]) + [9090 + 104729 * r for r in range(REPLICAS - len(ref["seeds"]))]
+    runs, snaps, types, L = _run("bilayer16", ref["discard"], 2250, 250, seeds)
+    _close("kT", [A.block_average(r[:, 0])[0] for r in runs], [r["kT"][0] for r in ref["runs"]], 0.005)
+    _close("P", [A.block_average(r[:, 1])[0] for r in runs], [r["P"][0] for r in ref["runs"]], 0.01)
+    profs = np.array([A.density_profile(x, types, L, centre_type=1, dz=0.5, ntypes=3)[1] for s in snaps for x in s])
     p, pse = profs.mean(0), profs.std(0, ddof=1) / np.sqrt(len(profs))
     pr, prse = np.array(ref["profile"]), np.array(ref["profile_se"])
     dev = np.abs(p - pr) / (np.hypot(pse, prse) + 0.02)
-    print(f"  density profiles: max dev/sigma {dev.max():.2f}", flush=True)
+    print(f"  density profiles: mean dev/sigma {dev.mean():.2f}, max {dev.max():.2f}", flush=True)
     for t, name in enumerate(ref["types"]):
-        # integrated number in each half-box, and the peak positions, agree
+        # beads of each type in the profile window agree
         assert abs(p[t].sum() - pr[t].sum()) / pr[t].sum() < 0.02, name
     assert np.mean(dev) < 2.0
     nlip = ref["nlipids"]
-    apl = [A.area_per_lipid(x, types, L, 0, 1, nlip) for x in snaps]
-    _close("area per lipid", A.block_average(apl, len(apl)), ref["apl"], 0.03)
+    apl = [np.mean([A.area_per_lipid(x, types, L, 0, 1, nlip) for x in s]) for s in snaps]
+    _close("area per lipid", apl, [r["apl"] for r in ref["runs"]], 0.03)

@@ -14,12 +14,13 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def dataset():
+    # 1024 replicas of 6000 steps: ~2.6M beads advanced together, ~15 s on one MI355X
     rng = np.random.default_rng(2026)
-    n = 256
+    n = 1024
     params = S.sample_params(n, rng)
     sizes = rng.integers(8, 12, size=n)
     seeds = rng.integers(1, 2 ** 31, size=n)
-    res = S.run_replicas(params, sizes, seeds, steps=3000, discard=1500)
+    res = S.run_replicas(params, sizes, seeds, steps=6000, discard=2000)
     print(f"replicas: {n} boxes, {res['nbeads']} beads, {res['bead_steps_per_s']:.3e} bead-steps/s, "
           f"wall {res['wall']:.1f} s", flush=True)
     return params, sizes, res
