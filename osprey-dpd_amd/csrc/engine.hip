@@ -38,14 +38,17 @@ enum ObsSlot { O_V2 = 0, O_MOM = 3, O_VIR = 6, O_PEP = 15, O_PEB = 16, O_PEA = 1
 static_assert(O_VIR == kObsVir && O_PEP == kObsPEP, "accumulator layout");
 
 struct Topo {
-    // CSR by bead index (all boxes)
-    const int* bond_ptr;      // [n+1]
-    const int4* bond_ent;     // (other, is_head, 0, 0)
-    const double2* bond_par;  // (k, l0) per entry
-    const int* ang_ptr;       // [n+1]
+    const int* bond_ptr;      // [n+1] bond entries of each bead index (all boxes)
+    const int2* bond_ent;     // (bond, 1 if the bead is the bond's head)
+    const int2* bond_bead;    // (head, tail) bead index per bond
+    const double2* bond_par;  // (k, l0) per bond
+    const int* ang_ptr;       // [n+1] angle entries of each bead index
     const int2* ang_ent;      // (angle, role)
     const int4* ang_bead;     // (a, b, c, 0) per angle
     const double4* ang_par;   // (k, phi0, cos phi0, sin phi0)
+    double4* fbond;           // per bond: force on the head (the tail takes minus)
+    double4* fang;            // per angle: force on roles 0 (c), 1 (b), 2 (a)
+    int nbonds, nangles;
     int has_bonds, has_angles;
 };
 
@@ -75,16 +78,17 @@ __device__ inline void unwrapped_pos(int slot, const Buffers& S, const BoxDev& B
 }
 
 // CBondPair::AddForce (src/BondPair.cpp:138-286) for one angle in FP64.  first = bond
-// (b -> c), second = bond (a -> b); returns the force on the bead of `role`
-// (0: c = tail of first, 1: b = shared bead, 2: a = head of second).
-__device__ inline void angle_force(const double ua[3], const double ub[3], const double uc[3], const double4 par,
-                                   int role, double f[3], double* pe)
+// (b -> c), second = bond (a -> b); f[0] is the force on c (tail of first), f[1] on the
+// shared bead b, f[2] on a (head of second); pe (may be null) gets
+// CBondPair::AddPotentialEnergy (src/BondPair.cpp:320-378).
+__device__ inline void angle_forces(const double ua[3], const double ub[3], const double uc[3], const double4 par,
+                                    double f[3][3], double* pe)
 {
     double first[3], second[3];
     for (int d = 0; d < 3; ++d) { first[d] = ub[d] - uc[d]; second[d] = ua[d] - ub[d]; }
     const double l1 = sqrt(first[0] * first[0] + first[1] * first[1] + first[2] * first[2]);
     const double l2 = sqrt(second[0] * second[0] + second[1] * second[1] + second[2] * second[2]);
-    f[0] = f[1] = f[2] = 0.0;
+    for (int r = 0; r < 3; ++r) f[r][0] = f[r][1] = f[r][2] = 0.0;
     const double magProduct = l1 * l2;
     if (!(magProduct > 0.0001)) { if (pe) *pe = 0.0; return; }
     const double k = par.x, phi0 = par.y, cosPhi0 = par.z, sinPhi0 = par.w;
@@ -99,14 +103,14 @@ __device__ inline void angle_force(const double ua[3], const double ub[3], const
     } else {
         forceMag = k / magProduct;
     }
-    double ft[3], fh[3];
     for (int d = 0; d < 3; ++d) {
-        ft[d] = forceMag * (b1b2Overb1Sq * first[d] - second[d]);
-        fh[d] = forceMag * (first[d] - b1b2Overb2Sq * second[d]);
+        const double ft = forceMag * (b1b2Overb1Sq * first[d] - second[d]);
+        const double fh = forceMag * (first[d] - b1b2Overb2Sq * second[d]);
+        f[0][d] = ft;
+        f[2][d] = fh;
+        f[1][d] = -ft - fh;
     }
-    for (int d = 0; d < 3; ++d) f[d] = role == 0 ? ft[d] : (role == 2 ? fh[d] : -ft[d] - fh[d]);
     if (pe) {
-        // CBondPair::AddPotentialEnergy (src/BondPair.cpp:320-378)
         const double cps = (b1Dotb2 * b1Dotb2) / (magProduct * magProduct);
         const double sinPhi = sqrt(fmax(0.0, 1.0 - cps));
         double e;
@@ -117,58 +121,97 @@ __device__ inline void angle_force(const double ua[3], const double ub[3], const
     }
 }
 
-// Bonded force on the bead in `slot` (index q) in FP64, plus (obs) its share of the bond
-// virial (head bead only: CBond::AddForce adds the stress to the head) and the bonded
-// potential energies.  Bonds never cross boxes, so the partners share B.
-__device__ inline void bonded_force(int slot, int q, const Buffers& S, const int* slot_of, const Topo& T,
-                                    const BoxDev& B, double f[3], double vir[9], double* pe_b, double* pe_a,
-                                    bool obs)
+// Sum of the bonded forces on bead index q, gathered from the per-bond and per-angle
+// forces of this step (each bond and angle is evaluated once, by k_bonded).
+__device__ inline void bonded_gather(int q, const Topo& T, double f[3])
 {
     f[0] = f[1] = f[2] = 0.0;
     if (T.has_bonds) {
-        const int e0 = T.bond_ptr[q], e1 = T.bond_ptr[q + 1];
-        double us[3];
-        if (e0 < e1) unwrapped_pos(slot, S, B, us);
-        for (int e = e0; e < e1; ++e) {
-            const int4 ent = T.bond_ent[e];
-            const double2 par = T.bond_par[e];
-            double uo[3];
-            unwrapped_pos(slot_of[ent.x], S, B, uo);
-            const double d[3] = {us[0] - uo[0], us[1] - uo[1], us[2] - uo[2]};
-            const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-            const double s = par.x * (par.y - L) / L;
-            const double fb[3] = {s * d[0], s * d[1], s * d[2]};
-            f[0] += fb[0]; f[1] += fb[1]; f[2] += fb[2];
-            if (obs && ent.y) {
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) vir[3 * a + b] += d[b] * fb[a];
-                *pe_b += 0.5 * par.x * (L - par.y) * (L - par.y);
-            }
+        for (int e = T.bond_ptr[q], e1 = T.bond_ptr[q + 1]; e < e1; ++e) {
+            const int2 ent = T.bond_ent[e];
+            const double4 fb = T.fbond[ent.x];
+            const double sg = ent.y ? 1.0 : -1.0;
+            f[0] += sg * fb.x; f[1] += sg * fb.y; f[2] += sg * fb.z;
         }
     }
     if (T.has_angles) {
-        const int e0 = T.ang_ptr[q], e1 = T.ang_ptr[q + 1];
-        for (int e = e0; e < e1; ++e) {
+        for (int e = T.ang_ptr[q], e1 = T.ang_ptr[q + 1]; e < e1; ++e) {
             const int2 ent = T.ang_ent[e];
-            const int4 abc = T.ang_bead[ent.x];
-            double ua[3], ub[3], uc[3], fa[3];
-            unwrapped_pos(slot_of[abc.x], S, B, ua);
-            unwrapped_pos(slot_of[abc.y], S, B, ub);
-            unwrapped_pos(slot_of[abc.z], S, B, uc);
-            double e_ang = 0.0;
-            angle_force(ua, ub, uc, T.ang_par[ent.x], ent.y, fa, (obs && ent.y == 1) ? &e_ang : nullptr);
-            f[0] += fa[0]; f[1] += fa[1]; f[2] += fa[2];
-            if (obs && ent.y == 1) *pe_a += e_ang;
+            const double4 fa = T.fang[3 * ent.x + ent.y];
+            f[0] += fa.x; f[1] += fa.y; f[2] += fa.z;
         }
     }
 }
 
 // ---------------------------------------------------------------- kernels ------
 
+// CBond::AddForce (src/Bond.cpp:111-162) and CBondPair::AddForce for every bond and angle
+// of this step, in FP64: thread q < nbonds takes bond q, the rest take angles.  The force
+// goes to fbond / fang, from where k_advance / k_finish gather each bead's sum.  OBS adds
+// the bond virial (to the head bead's box, as CBond::AddForce adds its stress to the head)
+// and the bond and angle potential energies.  Bonds and angles never cross boxes.
+template <bool OBS, bool MULTI>
+__global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict__ slot_of, Topo T,
+                                                const BoxDev* boxes, BoxDev B0, double* __restrict__ acc)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    double ob[12];
+    if (OBS)
+        for (int i = 0; i < 12; ++i) ob[i] = 0.0;
+    int box = -1;
+    if (q < T.nbonds) {
+        const int2 hb = T.bond_bead[q];
+        const int sh = slot_of[hb.x], st = slot_of[hb.y];
+        const int w = __float_as_int(S.pos[sh].w);
+        box = bead_box(w);
+        const BoxDev& B = box_of<MULTI>(boxes, B0, w);
+        double uh[3], ut[3];
+        unwrapped_pos(sh, S, B, uh);
+        unwrapped_pos(st, S, B, ut);
+        const double2 par = T.bond_par[q];
+        const double d[3] = {uh[0] - ut[0], uh[1] - ut[1], uh[2] - ut[2]};
+        const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double s = par.x * (par.y - L) / L;
+        const double fb[3] = {s * d[0], s * d[1], s * d[2]};
+        T.fbond[q] = make_double4(fb[0], fb[1], fb[2], 0.0);
+        if (OBS) {
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) ob[3 * a + b] = d[b] * fb[a];
+            ob[10] = 0.5 * par.x * (L - par.y) * (L - par.y);
+        }
+    } else if (q < T.nbonds + T.nangles) {
+        const int g = q - T.nbonds;
+        const int4 abc = T.ang_bead[g];
+        const int sa = slot_of[abc.x], sb = slot_of[abc.y], sc = slot_of[abc.z];
+        const int w = __float_as_int(S.pos[sb].w);
+        box = bead_box(w);
+        const BoxDev& B = box_of<MULTI>(boxes, B0, w);
+        double ua[3], ub[3], uc[3], f[3][3], pe = 0.0;
+        unwrapped_pos(sa, S, B, ua);
+        unwrapped_pos(sb, S, B, ub);
+        unwrapped_pos(sc, S, B, uc);
+        angle_forces(ua, ub, uc, T.ang_par[g], f, OBS ? &pe : nullptr);
+        for (int r = 0; r < 3; ++r) T.fang[3 * g + r] = make_double4(f[r][0], f[r][1], f[r][2], 0.0);
+        if (OBS) ob[11] = pe;
+    }
+    if (OBS) {
+        // slots O_VIR .. O_PEA: virial (9), pair PE (untouched here), bond PE, angle PE
+        int box0 = 0;
+        if (MULTI) {
+            const int q0 = blockIdx.x * blockDim.x;
+            const int s0 = q0 < T.nbonds ? slot_of[T.bond_bead[q0].x] : slot_of[T.ang_bead[q0 - T.nbonds].y];
+            box0 = bead_box(__float_as_int(S.pos[s0].w));
+        } else {
+            box = box < 0 ? -1 : 0;
+        }
+        accumulate_boxes<12>(ob, box, box0, kObsVir, acc);
+    }
+}
+
 // Close the previous step (kEndBegin) and start the next one: new cell key and rank.
 template <int MODE, bool MULTI>
-__global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc,
-                                                 const int* __restrict__ slot_of, Topo T, const BoxDev* boxes,
+__global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc, Topo T,
+                                                 const BoxDev* boxes,
                                                  BoxDev B0, int* __restrict__ cnt, int* __restrict__ bcnt,
                                                  int* __restrict__ nkey, int* __restrict__ rank,
                                                  int* __restrict__ flags, int64_t* __restrict__ step)
@@ -184,8 +227,8 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
         float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
         const int4 ax = S.aux[s];
         if (MODE == kEndBegin) {
-            double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
-            bonded_force(s, ax.w, S, slot_of, T, B, fb, vir, &peb, &pea, false);
+            double fb[3];
+            bonded_gather(ax.w, T, fb);
             f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
             const float4 fo = S.fold[s];
             v.x = v.x + B.hdt * (f.x + fo.x);
@@ -321,16 +364,18 @@ __global__ void k_copy_force(int n, const float4* __restrict__ fold, float4* __r
     if (s < n) frc[s] = fold[s];
 }
 
-// Close the last step: bonded forces, UpdateMom (UPDATE_V) and FP64 observables.
+// Close the last step: bonded forces (gathered), UpdateMom (UPDATE_V) and the FP64 sums
+// of v^2 and momentum (the virial and energies come from the pair and bonded kernels).
 template <bool UPDATE_V, bool OBS, bool MULTI>
-__global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __restrict__ frc,
-                                                const int* __restrict__ slot_of, Topo T, const BoxDev* boxes,
-                                                BoxDev B0, int* __restrict__ flags, double* __restrict__ acc)
+__global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __restrict__ frc, Topo T,
+                                                const BoxDev* boxes, BoxDev B0, int* __restrict__ flags,
+                                                double* __restrict__ acc)
 {
+    static_assert(O_V2 == 0 && O_MOM == 3, "kinetic slots first");
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    double ob[O_N];
+    double ob[6];
     if (OBS)
-        for (int i = 0; i < O_N; ++i) ob[i] = 0.0;
+        for (int i = 0; i < 6; ++i) ob[i] = 0.0;
     int box = -1;
     if (s < n) {
         const int w = __float_as_int(S.pos[s].w);
@@ -338,8 +383,8 @@ __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __rest
         const BoxDev& B = box_of<MULTI>(boxes, B0, w);
         float4 f = frc[s];
         const int4 ax = S.aux[s];
-        double fb[3], vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, peb = 0.0, pea = 0.0;
-        bonded_force(s, ax.w, S, slot_of, T, B, fb, vir, &peb, &pea, OBS);
+        double fb[3];
+        bonded_gather(ax.w, T, fb);
         f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
         frc[s] = f;
         float4 v = S.vel[s];
@@ -359,14 +404,11 @@ __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __rest
             ob[O_MOM + 0] = v.x;
             ob[O_MOM + 1] = v.y;
             ob[O_MOM + 2] = v.z;
-            for (int i = 0; i < 9; ++i) ob[O_VIR + i] = vir[i];
-            ob[O_PEB] = peb;
-            ob[O_PEA] = pea;
         }
     }
     if (OBS) {
         const int box0 = MULTI ? bead_box(__float_as_int(S.pos[blockIdx.x * blockDim.x].w)) : 0;
-        accumulate_boxes<O_N>(ob, MULTI ? box : (s < n ? 0 : -1), box0, 0, acc);
+        accumulate_boxes<6>(ob, MULTI ? box : (s < n ? 0 : -1), box0, 0, acc);
     }
 }
 
@@ -457,11 +499,11 @@ struct dpd_engine {
     double* gather = nullptr;   // 12n doubles for downloads
     // topology
     int *bond_ptr = nullptr, *ang_ptr = nullptr;
-    int4* bond_ent = nullptr;
+    int2 *bond_ent = nullptr, *bond_bead = nullptr, *ang_ent = nullptr;
     double2* bond_par = nullptr;
-    int2* ang_ent = nullptr;
     int4* ang_bead = nullptr;
-    double4* ang_par = nullptr;
+    double4 *ang_par = nullptr, *fbond = nullptr, *fang = nullptr;
+    int nbonds = 0, nangles = 0;
     int has_bonds = 0, has_angles = 0;
     // observables
     std::vector<dpd_obs> obs;
@@ -575,8 +617,8 @@ Buffers bufs(dpd_engine* e, int i) { return Buffers{e->pos[i], e->vel[i], e->fol
 
 Topo topo(dpd_engine* e)
 {
-    return Topo{e->bond_ptr, e->bond_ent, e->bond_par, e->ang_ptr, e->ang_ent, e->ang_bead, e->ang_par,
-                e->has_bonds, e->has_angles};
+    return Topo{e->bond_ptr, e->bond_ent, e->bond_bead, e->bond_par, e->ang_ptr, e->ang_ent, e->ang_bead,
+                e->ang_par, e->fbond, e->fang, e->nbonds, e->nangles, e->has_bonds, e->has_angles};
 }
 
 inline int grid(int64_t n, int b) { return (int)((n + b - 1) / b); }
@@ -590,7 +632,7 @@ void sort_step_t(dpd_engine* e)
     Buffers S = bufs(e, e->cur), D = bufs(e, e->cur ^ 1);
     const BoxDev B0 = e->boxes[0].B;
     hipLaunchKernelGGL((k_advance<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
-                       e->slot_of, topo(e), e->dboxes, B0, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->dstep);
+                       topo(e), e->dboxes, B0, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->dstep);
     hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, e->stream, e->nbricks, e->bcnt, e->bstart);
     hipLaunchKernelGGL(k_cell_start, dim3(grid(e->nbricks, 4)), dim3(256), 0, e->stream, e->nbricks, n, e->bstart,
                        e->cnt, e->cstart);
@@ -635,10 +677,25 @@ void launch_finish(dpd_engine* e)
     const BoxDev B0 = e->boxes[0].B;
     if (multi(e))
         hipLaunchKernelGGL((k_finish<UPDATE_V, OBS, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n,
-                           bufs(e, e->cur), e->frc, e->slot_of, topo(e), e->dboxes, B0, e->flags, e->acc);
+                           bufs(e, e->cur), e->frc, topo(e), e->dboxes, B0, e->flags, e->acc);
     else
         hipLaunchKernelGGL((k_finish<UPDATE_V, OBS, false>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n,
-                           bufs(e, e->cur), e->frc, e->slot_of, topo(e), e->dboxes, B0, e->flags, e->acc);
+                           bufs(e, e->cur), e->frc, topo(e), e->dboxes, B0, e->flags, e->acc);
+}
+
+// bonds and angles of this step (before k_advance / k_finish gather them)
+template <bool OBS>
+void launch_bonded(dpd_engine* e)
+{
+    const int nq = e->nbonds + e->nangles;
+    if (nq == 0) return;
+    const BoxDev B0 = e->boxes[0].B;
+    if (multi(e))
+        hipLaunchKernelGGL((k_bonded<OBS, true>), dim3(grid(nq, 256)), dim3(256), 0, e->stream, bufs(e, e->cur),
+                           e->slot_of, topo(e), e->dboxes, B0, e->acc);
+    else
+        hipLaunchKernelGGL((k_bonded<OBS, false>), dim3(grid(nq, 256)), dim3(256), 0, e->stream, bufs(e, e->cur),
+                           e->slot_of, topo(e), e->dboxes, B0, e->acc);
 }
 
 // clear the observable accumulators (and, when `flags`, the error flags)
@@ -699,6 +756,8 @@ int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, bool timing)
         if (timing) HIPCHK(hipEventRecord(e->pev[2 * t], e->stream));
         if ((rc = launch_pair(e, obs && t == nsteps - 1))) return rc;
         if (timing) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
+        if (obs && t == nsteps - 1) launch_bonded<true>(e);
+        else launch_bonded<false>(e);
     }
     if (obs) launch_finish<true, true>(e);
     else launch_finish<true, false>(e);
@@ -829,14 +888,16 @@ int commit_topology(dpd_engine* e)
         ap[a + 1]++; ap[bm + 1]++; ap[c + 1]++;
     }
     for (int64_t i = 0; i < n; ++i) { bp[i + 1] += bp[i]; ap[i + 1] += ap[i]; }
-    std::vector<int4> bent(std::max<int>(bp[n], 1));
-    std::vector<double2> bpar(std::max<int>(bp[n], 1));
+    std::vector<int2> bent(std::max<int>(bp[n], 1)), bbead(std::max<int64_t>(nbonds, 1));
+    std::vector<double2> bpar(std::max<int64_t>(nbonds, 1));
     std::vector<int2> aent(std::max<int>(ap[n], 1));
     std::vector<int> fill(bp.begin(), bp.end() - 1);
     for (int64_t b = 0; b < nbonds; ++b) {
         const int h = H[b], t = Tl[b];
-        bent[fill[h]] = make_int4(t, 1, 0, 0); bpar[fill[h]++] = make_double2(K[b], L0[b]);
-        bent[fill[t]] = make_int4(h, 0, 0, 0); bpar[fill[t]++] = make_double2(K[b], L0[b]);
+        bbead[b] = make_int2(h, t);
+        bpar[b] = make_double2(K[b], L0[b]);
+        bent[fill[h]++] = make_int2((int)b, 1);
+        bent[fill[t]++] = make_int2((int)b, 0);
     }
     std::vector<int> afill(ap.begin(), ap.end() - 1);
     for (int64_t q = 0; q < npairs; ++q) {
@@ -849,17 +910,22 @@ int commit_topology(dpd_engine* e)
     int rc;
     if ((rc = dalloc(e, &e->bond_ptr, n + 1)) || (rc = dalloc(e, &e->ang_ptr, n + 1)) ||
         (rc = dalloc(e, &e->bond_ent, bent.size())) || (rc = dalloc(e, &e->bond_par, bpar.size())) ||
+        (rc = dalloc(e, &e->bond_bead, bbead.size())) || (rc = dalloc(e, &e->fbond, bbead.size())) ||
+        (rc = dalloc(e, &e->fang, 3 * abead.size())) ||
         (rc = dalloc(e, &e->ang_ent, aent.size())) || (rc = dalloc(e, &e->ang_bead, abead.size())) ||
         (rc = dalloc(e, &e->ang_par, apar.size())))
         return rc;
     HIPCHK(hipMemcpyAsync(e->bond_ptr, bp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_ptr, ap.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemcpyAsync(e->bond_ent, bent.data(), sizeof(int4) * bent.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->bond_ent, bent.data(), sizeof(int2) * bent.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->bond_bead, bbead.data(), sizeof(int2) * bbead.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->bond_par, bpar.data(), sizeof(double2) * bpar.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_ent, aent.data(), sizeof(int2) * aent.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_bead, abead.data(), sizeof(int4) * abead.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_par, apar.data(), sizeof(double4) * apar.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    e->nbonds = (int)nbonds;
+    e->nangles = (int)npairs;
     e->has_bonds = nbonds > 0;
     e->has_angles = npairs > 0;
     return DPD_OK;
@@ -984,7 +1050,8 @@ void dpd_destroy(dpd_engine* e)
     void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->aux[0], e->aux[1],
                     e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
                     e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
-                    e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->dboxes};
+                    e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
+                    e->fbond, e->fang, e->dboxes};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& ev : e->ev)
@@ -1006,6 +1073,7 @@ int dpd_upload_beads(dpd_engine* e, int64_t n, const int32_t* id, const int32_t*
     if ((rc = commit_beads(e, step0))) return rc;
     if (keep_topo) return commit_topology(e);
     e->has_bonds = e->has_angles = 0;
+    e->nbonds = e->nangles = 0;
     return DPD_OK;
 }
 
@@ -1143,6 +1211,7 @@ int dpd_compute_forces(dpd_engine* e, int64_t step, uint32_t flags, double* f_ou
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, step);
     // the uploaded state is sorted with vp = v: evaluate at it
     if ((rc = launch_pair(e, true))) return rc;
+    launch_bonded<true>(e);
     launch_finish<false, true>(e);
     HIPCHK(hipGetLastError());
     if ((rc = read_flags(e))) return rc;

@@ -6,15 +6,16 @@
 // a candidate with r < 1 is not evaluated on the spot -- where ~15% of the lanes would
 // run the pair force and its Philox draw while the rest idle -- but appended to a
 // per-wavefront LDS ring (ballot + mbcnt); whenever 64 pairs are queued they are
-// evaluated together, one per lane.  Each loop trip tests two candidates per lane, so
-// two independent loads are in flight per trip.  The force of a queued pair belongs to
+// evaluated together, one per lane.  Each loop trip tests four candidates per lane
+// (four independent loads in flight), and the next cell's slot range is fetched while
+// the current cell is tested, so each lane's chain of dependent loads stays short.  The force of a queued pair belongs to
 // bead i, the bead of the lane that found it; the evaluating lane adds it to i's LDS
 // accumulator.
 // Consecutive queue entries come from different lanes, so these LDS atomics rarely
 // collide.  Each pair is evaluated from both sides with the same operands (dx negated),
 // so the two forces are exact negatives and no force leaves its bead's lane; virial and
-// PE count half per side.  LDS use is ~24 KB per 256-thread workgroup (no staging), and
-// no bead count per cell or brick can overflow.  Workgroups are mapped XCD-aware
+// PE count half per side.  LDS use is ~19 KB per 256-thread workgroup (no staging: eight
+// workgroups, 32 wavefronts, per CU), and no bead count per cell or brick can overflow.  Workgroups are mapped XCD-aware
 // (workgroup b runs on XCD b % 8; it takes bead block (b % 8) * ceil(G/8) + b / 8), so each
 // XCD walks one contiguous slab of the box and neighbour beads hit its own L2.
 #include "pair_kernels.hpp"
@@ -24,7 +25,7 @@ namespace dpd {
 namespace {
 
 constexpr int kWaves = 4;   // 256 threads
-constexpr int kQueue = 256;   // ring; holds < 64 + 2 * 64 entries between drains
+constexpr int kQueue = 128;   // ring; holds < 64 + 64 entries between drains
 
 struct Wave {
     uint64_t queue[kQueue];   // i lane | neighbour offset << 6 | j << 32
@@ -125,50 +126,56 @@ __global__ __launch_bounds__(256) void k_pair_compact(PairArgs A, BoxDev B0, int
 
     volatile uint64_t* queue = W.queue;
     int head = 0, tail = 0;   // wave-uniform ring positions
-    for (int nb = 0; nb < 27; ++nb) {
-        const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - 1;
-        int j = 0, j1 = 0;
-        if (active) {
-            int cx = c[0] + ox, cy = c[1] + oy, cz = c[2] + oz;
-            cx += (cx < 0) ? n0 : ((cx >= n0) ? -n0 : 0);
-            cy += (cy < 0) ? n1 : ((cy >= n1) ? -n1 : 0);
-            cz += (cz < 0) ? n2 : ((cz >= n2) ? -n2 : 0);
-            const int brick = (cx >> 2) + nb0 * ((cy >> 2) + nb1 * (cz >> 2));
-            const int k = koff + brick * kBrickCells + (((cz & 3) * 4 + (cy & 3)) * 4 + (cx & 3));
-            j = A.cstart[k];
-            j1 = A.cstart[k + 1];
+    // the slot range of neighbour cell nb (0..26) of this lane's bead
+    auto cell_range = [&](int nb, int& j0, int& j1) {
+        j0 = j1 = 0;
+        if (!active) return;
+        int cx = c[0] + nb % 3 - 1, cy = c[1] + (nb / 3) % 3 - 1, cz = c[2] + nb / 9 - 1;
+        cx += (cx < 0) ? n0 : ((cx >= n0) ? -n0 : 0);
+        cy += (cy < 0) ? n1 : ((cy >= n1) ? -n1 : 0);
+        cz += (cz < 0) ? n2 : ((cz >= n2) ? -n2 : 0);
+        const int brick = (cx >> 2) + nb0 * ((cy >> 2) + nb1 * (cz >> 2));
+        const int k = koff + brick * kBrickCells + (((cz & 3) * 4 + (cy & 3)) * 4 + (cx & 3));
+        j0 = A.cstart[k];
+        j1 = A.cstart[k + 1];
+    };
+    uint64_t tag = 0;
+    // append a tested candidate; evaluate whenever 64 pairs are queued
+    auto push = [&](bool pass, int jj) {
+        const uint64_t mk = __ballot(pass);
+        if (pass) queue[(tail + lane_prefix(mk)) & (kQueue - 1)] = tag | ((uint64_t)(uint32_t)jj << 32);
+        tail += __popcll(mk);
+        if (tail - head >= 64) {
+            __builtin_amdgcn_wave_barrier();
+            eval_queued<OBS, MULTI>(W, queue[(head + lane) & (kQueue - 1)], A, B0, tab, vir_s[OBS ? wave : 0]);
+            head += 64;
+            __builtin_amdgcn_wave_barrier();
         }
-        const float sx = (float)ox * w0, sy = (float)oy * w1, sz = (float)oz * w2;
-        const uint64_t tag = (uint64_t)(lane | (nb << 6));
+    };
+    int nj0, nj1;
+    cell_range(0, nj0, nj1);
+    for (int nb = 0; nb < 27; ++nb) {
+        int j = nj0;
+        const int j1 = nj1;
+        if (nb < 26) cell_range(nb + 1, nj0, nj1);   // prefetch the next cell's range
+        const float sx = (float)(nb % 3 - 1) * w0, sy = (float)((nb / 3) % 3 - 1) * w1,
+                    sz = (float)(nb / 9 - 1) * w2;
+        tag = (uint64_t)(lane | (nb << 6));
+        // four candidates per trip: four independent loads in flight per lane
         while (__any(j < j1)) {
-            const bool ha = j < j1, hb = j + 1 < j1;
-            float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), pb = pa;
-            if (ha) pa = A.pos[j];
-            if (hb) pb = A.pos[j + 1];
-            bool passa = false, passb = false;
-            if (ha) {
-                const float dx = (pi.x - pa.x) - sx, dy = (pi.y - pa.y) - sy, dz = (pi.z - pa.z) - sz;
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                q[u] = make_float4(1e30f, 1e30f, 1e30f, 0.f);
+                if (j + u < j1) q[u] = A.pos[j + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float dx = (pi.x - q[u].x) - sx, dy = (pi.y - q[u].y) - sy, dz = (pi.z - q[u].z) - sz;
                 const float dr2 = dx * dx + dy * dy + dz * dz;
-                passa = dr2 < 1.0f && dr2 > 1e-18f && j != i;
+                push(j + u < j1 && dr2 < 1.0f && dr2 > 1e-18f && j + u != i, j + u);
             }
-            if (hb) {
-                const float dx = (pi.x - pb.x) - sx, dy = (pi.y - pb.y) - sy, dz = (pi.z - pb.z) - sz;
-                const float dr2 = dx * dx + dy * dy + dz * dz;
-                passb = dr2 < 1.0f && dr2 > 1e-18f && j + 1 != i;
-            }
-            const uint64_t ma = __ballot(passa);
-            if (passa) queue[(tail + lane_prefix(ma)) & (kQueue - 1)] = tag | ((uint64_t)(uint32_t)j << 32);
-            tail += __popcll(ma);
-            const uint64_t mb = __ballot(passb);
-            if (passb) queue[(tail + lane_prefix(mb)) & (kQueue - 1)] = tag | ((uint64_t)(uint32_t)(j + 1) << 32);
-            tail += __popcll(mb);
-            j += 2;
-            while (tail - head >= 64) {
-                __builtin_amdgcn_wave_barrier();
-                eval_queued<OBS, MULTI>(W, queue[(head + lane) & (kQueue - 1)], A, B0, tab, vir_s[OBS ? wave : 0]);
-                head += 64;
-                __builtin_amdgcn_wave_barrier();
-            }
+            j += 4;
         }
     }
     __builtin_amdgcn_wave_barrier();

@@ -9,9 +9,12 @@
 //     half of the home neighbour cells (staged cell index above its own) and its own
 //     cell; a pair of two home beads is thus listed once (half shell), a pair with a
 //     halo bead once per brick owning one of its beads;
-//   * a wavefront takes one home cell at a time (dynamic, from an LDS counter) and
-//     streams the flat index t over (home bead, list entry) = n_cell x M: bead t / M
-//     (float reciprocal, exact for t < 2^24), entry t % M, one LDS read for j;
+//   * a wavefront takes one home cell at a time (dynamic, from an LDS counter), holds up
+//     to 64 list entries in registers (one per lane; sequential, conflict-free LDS reads)
+//     and tests them against each bead of the cell in turn (its position broadcast), so
+//     LDS is read once per (home cell, candidate) rather than once per (bead, candidate):
+//     random 16-byte LDS gathers cost ~20 cycles per wavefront from bank conflicts and
+//     bound any scheme that gathers per pair;
 //   * candidates with r < 1 (and, in the own cell, j above i) are compacted into a
 //     per-wavefront LDS queue (ballot + mbcnt) and evaluated 64 at a time, so the pair
 //     force and its Philox draw run on full wavefronts;
@@ -318,7 +321,12 @@ void k_pair_stream(PairArgs A, int per_xcd)
     }
     __syncthreads();   // also retires the list offsets: the queues reuse their LDS
 
-    // ---- candidate stream + per-wavefront compaction + evaluation
+    // ---- candidates: a wavefront takes one home cell at a time (dynamic, from an LDS
+    // counter), loads up to 64 entries of its list into registers (one per lane, a
+    // sequential read of the list and of the staged beads it names) and then, for each
+    // bead i of the cell, tests all of them against i's broadcast position, so each staged
+    // position is read once per home cell rather than once per (i, j) candidate; pairs in
+    // range are compacted per wavefront
     float vir[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     volatile uint32_t* queue = S.u.queue[wave];
     int qn = 0;   // wave-uniform queue length
@@ -329,40 +337,38 @@ void k_pair_stream(PairArgs A, int per_xcd)
         if (h >= kBrickCells) break;
         const int hc = sidx(h & 3, (h >> 2) & 3, h >> 4);
         const int i0 = S.start[hc];
-        const int ni = S.start[hc + 1] - i0;
+        const int i1 = S.start[hc + 1];
         const int jb = S.jstart[h];
-        const int M = S.jstart[h + 1] - jb;
-        const int T = ni * M;   // 0 for empty and absent home cells
-        const float invM = 1.0f / (float)max(M, 1);
-        for (int t0 = 0; t0 < T; t0 += 64) {
-            const int t = t0 + lane;
-            bool pass = false;
-            int li = 0, j = 0;
-            if (t < T) {
-                const int q = (int)(((float)t + 0.5f) * invM);
-                const int m = t - (int)__umul24((uint32_t)q, (uint32_t)M);
+        const int M = S.jstart[h + 1] - jb;   // 0 for absent home cells
+        if (i1 == i0) continue;
+        for (int m0 = 0; m0 < M; m0 += 64) {
+            const int m = m0 + lane;
+            int j = 0;
+            float4 pj = make_float4(1e30f, 1e30f, 1e30f, 0.f);
+            if (m < M) {
                 j = S.jlist[jb + m];
-                li = i0 + q;
+                pj = S.pos[j];
+            }
+            // own cell: each pair once (j above i); the list's other home cells are forward
+            const bool own = j >= i0 && j < i1;
+            for (int li = i0; li < i1; ++li) {
                 const float4 pi = S.pos[li];
-                const float4 pj = S.pos[j];
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 const float dr2 = dx * dx + dy * dy + dz * dz;
-                // own cell: each pair once (j above i); other home cells are forward only
-                const bool order = j > li || (__float_as_int(pj.w) >> 8) == 0;
-                pass = dr2 < 1.0f && dr2 > 1e-18f && order;
-            }
-            const uint64_t mk = __ballot(pass);
-            if (pass) queue[qn + lane_prefix(mk)] = (uint32_t)li | ((uint32_t)j << 16);
-            qn += __popcll(mk);
-            if (qn >= 64) {
-                __builtin_amdgcn_wave_barrier();
-                const uint32_t pk = queue[lane];
-                eval_pair<OBS>(S, pk, B.invrootdt, K, vir);
-                const uint32_t rest = (lane + 64 < qn) ? queue[lane + 64] : 0u;
-                __builtin_amdgcn_wave_barrier();
-                if (lane + 64 < qn) queue[lane] = rest;
-                qn -= 64;
-                __builtin_amdgcn_wave_barrier();
+                const bool pass = dr2 < 1.0f && dr2 > 1e-18f && (!own || j > li);
+                const uint64_t mk = __ballot(pass);
+                if (pass) queue[qn + lane_prefix(mk)] = (uint32_t)li | ((uint32_t)j << 16);
+                qn += __popcll(mk);
+                if (qn >= 64) {
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t pk = queue[lane];
+                    eval_pair<OBS>(S, pk, B.invrootdt, K, vir);
+                    const uint32_t rest = (lane + 64 < qn) ? queue[lane + 64] : 0u;
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane + 64 < qn) queue[lane] = rest;
+                    qn -= 64;
+                    __builtin_amdgcn_wave_barrier();
+                }
             }
         }
     }
