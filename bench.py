@@ -92,7 +92,7 @@ def build_engine(wl, kernel):
     return e
 
 
-def bench_batched(nbox=256, steps=100, warmup=20, nsingle=16, seed=20261202):
+def bench_batched(nbox=256, steps=100, warmup=20, nsingle=16, seed=20261202, pair_kernel=D.PAIR_AUTO):
     """Batched replicas: `nbox` water/lipid boxes with side lengths L ~ U{8..24}, a_ij ~
     U[20,90], k ~ U[64,256], advanced together (one graph replay per 50 steps), against
     the same kind of box advanced alone, one box per launch sequence."""
@@ -101,7 +101,7 @@ def bench_batched(nbox=256, steps=100, warmup=20, nsingle=16, seed=20261202):
     params = S.sample_params(nbox, rng)
     sizes = rng.integers(8, 25, size=nbox)
     seeds = rng.integers(1, 2 ** 31, size=nbox)
-    e, meta = S.build_batch(params, sizes, seeds)
+    e, meta = S.build_batch(params, sizes, seeds, pair_kernel=pair_kernel)
     e.step(warmup, graph=True)
     t0 = time.perf_counter()
     e.step(steps, observe=True, graph=True)
@@ -112,7 +112,7 @@ def bench_batched(nbox=256, steps=100, warmup=20, nsingle=16, seed=20261202):
     # the same kind of boxes one at a time (a sample of nsingle boxes)
     tot_bs, tot_t = 0.0, 0.0
     for i in range(nsingle):
-        es, _ = S.build_batch(params[i:i + 1], sizes[i:i + 1], seeds[i:i + 1])
+        es, _ = S.build_batch(params[i:i + 1], sizes[i:i + 1], seeds[i:i + 1], pair_kernel=pair_kernel)
         es.step(warmup, graph=True)
         t0 = time.perf_counter()
         es.step(steps, observe=True, graph=True)
@@ -221,6 +221,18 @@ def cpu_baseline_port(wl, nsteps=3):
             "sample": f"FP64 oracle step, {wl.n} beads, {nsteps} steps after 1 warm-up step"}
 
 
+def copy_probe(nbytes=256 << 20):
+    """One plain device copy of known size: what FETCH_SIZE / WRITE_SIZE report for it
+    scales the counters of the pair kernel to bytes."""
+    import torch
+    a = torch.ones(nbytes // 4, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,7 +241,12 @@ def main():
     ap.add_argument("--pair-kernel", type=int, default=D.PAIR_AUTO)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run one 256 MiB device-to-device copy (calibrates the PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (multi-GPU "
                                    "sharding is designed, not built)"}))
@@ -255,7 +272,7 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
-    batched = None if args.no_batched else bench_batched()
+    batched = None if args.no_batched else bench_batched(pair_kernel=args.pair_kernel)
 
     cpu = None
     if not args.no_cpu_baseline:

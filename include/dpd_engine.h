@@ -66,7 +66,8 @@ typedef struct dpd_params {
 #define DPD_PAIR_STREAM    4    /* LDS cell-stream kernel, 512 threads per brick     */
 #define DPD_PAIR_STREAM1024 5   /* LDS cell-stream kernel, 1024 threads per brick    */
 #define DPD_PAIR_COMPACT   6    /* thread-per-bead 27-cell kernel, compacted pairs   */
-#define DPD_PAIR_KINDS     6    /* kernels are numbered 1..DPD_PAIR_KINDS            */
+#define DPD_PAIR_CELLS     7    /* wavefront-per-cell kernel, candidates loaded once */
+#define DPD_PAIR_KINDS     7    /* kernels are numbered 1..DPD_PAIR_KINDS            */
 
 /* FP64 observables of the last completed step (CMonitor::CalculateSingleBeadData,
  * src/Monitor.cpp:790-1041).  stress = (sum over pairs and bonds of r (x) F) / V,

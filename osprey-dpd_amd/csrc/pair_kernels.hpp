@@ -12,6 +12,8 @@
 //                       cell, so the candidate test runs on full wavefronts (pair_stream.hip).
 //   DPD_PAIR_COMPACT    the full-shell walk with pairs in range compacted per wavefront
 //                       and evaluated 64 at a time (pair_compact.hip).
+//   DPD_PAIR_CELLS      a wavefront per home cell: the 27-cell candidate list is loaded once
+//                       per cell and tested against each of its beads (pair_cells.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -121,6 +123,7 @@ inline int default_pair_kernel() { return DPD_PAIR_FULLSHELL; }
 hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s, int threads);
 hipError_t launch_pair_stream(const PairArgs& a, bool obs, hipStream_t s, int threads);
 hipError_t launch_pair_compact(const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s);
+hipError_t launch_pair_cells(const PairArgs& a, bool obs, hipStream_t s);
 
 inline hipError_t launch_pair_kernel(int which, const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s)
 {
@@ -129,6 +132,7 @@ inline hipError_t launch_pair_kernel(int which, const PairArgs& a, const BoxDev&
     if (which == DPD_PAIR_STREAM) return launch_pair_stream(a, obs, s, 512);
     if (which == DPD_PAIR_STREAM1024) return launch_pair_stream(a, obs, s, 1024);
     if (which == DPD_PAIR_COMPACT) return launch_pair_compact(a, B0, obs, s);
+    if (which == DPD_PAIR_CELLS) return launch_pair_cells(a, obs, s);
     const int g = (a.n + 255) / 256;
     const int per_xcd = (g + 7) / 8;
     const dim3 grid(8 * per_xcd);
