@@ -54,14 +54,16 @@ typedef struct dpd_params {
     const double* a;            /* ntypes*ntypes conservative a_ij, row-major        */
     const double* gamma;        /* ntypes*ntypes dissipative gamma_ij                */
     int32_t device;             /* HIP device ordinal                                */
-    int32_t pair_kernel;        /* DPD_PAIR_AUTO / _BRICK / _FULLSHELL               */
+    int32_t pair_kernel;        /* DPD_PAIR_AUTO or one of the kernels below          */
 } dpd_params;
 
 #define DPD_MAX_TYPES 16
 
-#define DPD_PAIR_AUTO      0    /* tuned choice                                      */
+#define DPD_PAIR_AUTO      0    /* by bead count, re-chosen at each commit: the LDS  */
+                                /* stream kernel (1024) below 400k beads, full shell */
+                                /* above (measured crossover on MI355X)              */
 #define DPD_PAIR_BRICK     1    /* LDS cell-brick kernel, 256 threads per brick      */
-#define DPD_PAIR_FULLSHELL 2    /* thread-per-bead 27-cell kernel (cross-check)      */
+#define DPD_PAIR_FULLSHELL 2    /* thread-per-bead 27-cell kernel                    */
 #define DPD_PAIR_BRICK512  3    /* LDS cell-brick kernel, 512 threads per brick      */
 #define DPD_PAIR_STREAM    4    /* LDS cell-stream kernel, 512 threads per brick     */
 #define DPD_PAIR_STREAM1024 5   /* LDS cell-stream kernel, 1024 threads per brick    */

@@ -4,13 +4,14 @@
 //
 // One step (the reference's CSimBox::Evolve, src/SimBox.cpp:977-1120) runs as:
 //   k_advance   UpdateMom of the previous step fused with UpdatePos of this one
-//               (src/CNTCell.cpp:2280-2310, :1704-1757), bonded forces of the previous
-//               step (src/Bond.cpp:111-162, src/BondPair.cpp:138-286) added first;
-//               computes each bead's new cell key and its rank in that cell
+//               (src/CNTCell.cpp:2280-2310, :1704-1757); computes each bead's new cell
+//               key and its rank in that cell
 //   k_brick_scan / k_cell_start   counting-sort prefix sums (bricks, then cells)
 //   k_permute   scatter into the new cell-sorted SoA buffers
 //   pair kernel DPD pair forces (src/CNTCell.cpp:1144-1555)
-// and a last k_finish closes the final step (UpdateMom + bonded + FP64 observables).
+//   k_bonded / k_bonded_sum   bonds and angles (src/Bond.cpp:111-162,
+//               src/BondPair.cpp:138-286), each once, summed into the bonded beads' forces
+// and a last k_finish closes the final step (UpdateMom + FP64 observables).
 // The simulation time lives in device memory (bumped by k_advance), so a captured
 // sequence of steps replays as a hipGraph without re-capture.
 #include <hip/hip_runtime.h>
@@ -48,7 +49,8 @@ struct Topo {
     const double4* ang_par;   // (k, phi0, cos phi0, sin phi0)
     double4* fbond;           // per bond: force on the head (the tail takes minus)
     double4* fang;            // per angle: force on roles 0 (c), 1 (b), 2 (a)
-    int nbonds, nangles;
+    const int* bonded;        // bead indices with at least one bond or angle
+    int nbonds, nangles, nbonded;
     int has_bonds, has_angles;
 };
 
@@ -208,9 +210,25 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
     }
 }
 
+// Add each bonded bead's bond and angle forces of this step to its pair force, in place
+// (one thread per bead that has bonds; its sum in FP64, rounded once).
+__global__ __launch_bounds__(256) void k_bonded_sum(Topo T, const int* __restrict__ slot_of,
+                                                    float4* __restrict__ frc)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T.nbonded) return;
+    const int q = T.bonded[t];
+    double fb[3];
+    bonded_gather(q, T, fb);
+    const int s = slot_of[q];
+    float4 f = frc[s];
+    f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
+    frc[s] = f;
+}
+
 // Close the previous step (kEndBegin) and start the next one: new cell key and rank.
 template <int MODE, bool MULTI>
-__global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc, Topo T,
+__global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc,
                                                  const BoxDev* boxes,
                                                  BoxDev B0, int* __restrict__ cnt, int* __restrict__ bcnt,
                                                  int* __restrict__ nkey, int* __restrict__ rank,
@@ -227,9 +245,6 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
         float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
         const int4 ax = S.aux[s];
         if (MODE == kEndBegin) {
-            double fb[3];
-            bonded_gather(ax.w, T, fb);
-            f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
             const float4 fo = S.fold[s];
             v.x = v.x + B.hdt * (f.x + fo.x);
             v.y = v.y + B.hdt * (f.y + fo.y);
@@ -270,24 +285,36 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
     if (active && lane == leader) atomicAdd(bcnt + (k >> 6), size);
 }
 
-// exclusive scan of brick counts (one workgroup), clears the counts
+// exclusive scan of brick counts (one workgroup: each thread a contiguous run of bricks,
+// wavefront shuffle scans, one cross-wavefront step), clears the counts
 __global__ __launch_bounds__(1024) void k_brick_scan(int nbricks, int* __restrict__ bcnt, int* __restrict__ bstart)
 {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int per = (nbricks + 1023) / 1024;
     const int b0 = min(nbricks, t * per), b1 = min(nbricks, b0 + per);
     int s = 0;
     for (int b = b0; b < b1; ++b) s += bcnt[b];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int add = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += add;
-        __syncthreads();
+    int incl = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
     }
-    int run = part[t] - s;
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        const int w = lane < 16 ? wsum[lane] : 0;
+        int wi = w;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            const int y = __shfl_up(wi, off, 64);
+            if (lane >= off) wi += y;
+        }
+        if (lane < 16) wsum[lane] = wi - w;
+    }
+    __syncthreads();
+    int run = wsum[wave] + incl - s;
     for (int b = b0; b < b1; ++b) {
         const int c = bcnt[b];
         bstart[b] = run;
@@ -364,10 +391,11 @@ __global__ void k_copy_force(int n, const float4* __restrict__ fold, float4* __r
     if (s < n) frc[s] = fold[s];
 }
 
-// Close the last step: bonded forces (gathered), UpdateMom (UPDATE_V) and the FP64 sums
-// of v^2 and momentum (the virial and energies come from the pair and bonded kernels).
+// Close the last step: UpdateMom (UPDATE_V) and the FP64 sums of v^2 and momentum (the
+// force already holds the bonded terms; the virial and energies come from the pair and
+// bonded kernels).
 template <bool UPDATE_V, bool OBS, bool MULTI>
-__global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __restrict__ frc, Topo T,
+__global__ __launch_bounds__(256) void k_finish(int n, Buffers S, const float4* __restrict__ frc,
                                                 const BoxDev* boxes, BoxDev B0, int* __restrict__ flags,
                                                 double* __restrict__ acc)
 {
@@ -381,12 +409,7 @@ __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, float4* __rest
         const int w = __float_as_int(S.pos[s].w);
         box = bead_box(w);
         const BoxDev& B = box_of<MULTI>(boxes, B0, w);
-        float4 f = frc[s];
-        const int4 ax = S.aux[s];
-        double fb[3];
-        bonded_gather(ax.w, T, fb);
-        f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
-        frc[s] = f;
+        const float4 f = frc[s];
         float4 v = S.vel[s];
         if (UPDATE_V) {
             const float4 fo = S.fold[s];
@@ -483,6 +506,7 @@ struct dpd_engine {
     int64_t step = 0;
     int cur = 0;
     int pair_kernel = DPD_PAIR_FULLSHELL;
+    bool pair_auto = false;   // choose by bead count at each commit
     bool committed = false;
     // device buffers
     BoxDev* dboxes = nullptr;
@@ -503,7 +527,8 @@ struct dpd_engine {
     double2* bond_par = nullptr;
     int4* ang_bead = nullptr;
     double4 *ang_par = nullptr, *fbond = nullptr, *fang = nullptr;
-    int nbonds = 0, nangles = 0;
+    int* bonded = nullptr;
+    int nbonds = 0, nangles = 0, nbonded = 0;
     int has_bonds = 0, has_angles = 0;
     // observables
     std::vector<dpd_obs> obs;
@@ -618,7 +643,8 @@ Buffers bufs(dpd_engine* e, int i) { return Buffers{e->pos[i], e->vel[i], e->fol
 Topo topo(dpd_engine* e)
 {
     return Topo{e->bond_ptr, e->bond_ent, e->bond_bead, e->bond_par, e->ang_ptr, e->ang_ent, e->ang_bead,
-                e->ang_par, e->fbond, e->fang, e->nbonds, e->nangles, e->has_bonds, e->has_angles};
+                e->ang_par, e->fbond, e->fang, e->bonded, e->nbonds, e->nangles, e->nbonded, e->has_bonds,
+                e->has_angles};
 }
 
 inline int grid(int64_t n, int b) { return (int)((n + b - 1) / b); }
@@ -632,7 +658,7 @@ void sort_step_t(dpd_engine* e)
     Buffers S = bufs(e, e->cur), D = bufs(e, e->cur ^ 1);
     const BoxDev B0 = e->boxes[0].B;
     hipLaunchKernelGGL((k_advance<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
-                       topo(e), e->dboxes, B0, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->dstep);
+                       e->dboxes, B0, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->dstep);
     hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, e->stream, e->nbricks, e->bcnt, e->bstart);
     hipLaunchKernelGGL(k_cell_start, dim3(grid(e->nbricks, 4)), dim3(256), 0, e->stream, e->nbricks, n, e->bstart,
                        e->cnt, e->cstart);
@@ -677,10 +703,10 @@ void launch_finish(dpd_engine* e)
     const BoxDev B0 = e->boxes[0].B;
     if (multi(e))
         hipLaunchKernelGGL((k_finish<UPDATE_V, OBS, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n,
-                           bufs(e, e->cur), e->frc, topo(e), e->dboxes, B0, e->flags, e->acc);
+                           bufs(e, e->cur), e->frc, e->dboxes, B0, e->flags, e->acc);
     else
         hipLaunchKernelGGL((k_finish<UPDATE_V, OBS, false>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n,
-                           bufs(e, e->cur), e->frc, topo(e), e->dboxes, B0, e->flags, e->acc);
+                           bufs(e, e->cur), e->frc, e->dboxes, B0, e->flags, e->acc);
 }
 
 // bonds and angles of this step (before k_advance / k_finish gather them)
@@ -696,6 +722,8 @@ void launch_bonded(dpd_engine* e)
     else
         hipLaunchKernelGGL((k_bonded<OBS, false>), dim3(grid(nq, 256)), dim3(256), 0, e->stream, bufs(e, e->cur),
                            e->slot_of, topo(e), e->dboxes, B0, e->acc);
+    hipLaunchKernelGGL(k_bonded_sum, dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, topo(e), e->slot_of,
+                       e->frc);
 }
 
 // clear the observable accumulators (and, when `flags`, the error flags)
@@ -854,6 +882,11 @@ int commit_beads(dpd_engine* e, int64_t step0)
     e->jumps = 0;
     e->obs_valid = false;
     e->committed = true;
+    if (e->pair_auto) {
+        const int k = default_pair_kernel(e->n);
+        if (k != e->pair_kernel) drop_graphs(e);
+        e->pair_kernel = k;
+    }
     HIPCHK(hipStreamSynchronize(e->stream));
     return read_flags(e);
 }
@@ -899,6 +932,9 @@ int commit_topology(dpd_engine* e)
         bent[fill[h]++] = make_int2((int)b, 1);
         bent[fill[t]++] = make_int2((int)b, 0);
     }
+    std::vector<int> bonded;
+    for (int64_t i = 0; i < n; ++i)
+        if (bp[i + 1] > bp[i] || ap[i + 1] > ap[i]) bonded.push_back((int)i);
     std::vector<int> afill(ap.begin(), ap.end() - 1);
     for (int64_t q = 0; q < npairs; ++q) {
         const int4 abc = abead[q];
@@ -911,7 +947,7 @@ int commit_topology(dpd_engine* e)
     if ((rc = dalloc(e, &e->bond_ptr, n + 1)) || (rc = dalloc(e, &e->ang_ptr, n + 1)) ||
         (rc = dalloc(e, &e->bond_ent, bent.size())) || (rc = dalloc(e, &e->bond_par, bpar.size())) ||
         (rc = dalloc(e, &e->bond_bead, bbead.size())) || (rc = dalloc(e, &e->fbond, bbead.size())) ||
-        (rc = dalloc(e, &e->fang, 3 * abead.size())) ||
+        (rc = dalloc(e, &e->fang, 3 * abead.size())) || (rc = dalloc(e, &e->bonded, bonded.size())) ||
         (rc = dalloc(e, &e->ang_ent, aent.size())) || (rc = dalloc(e, &e->ang_bead, abead.size())) ||
         (rc = dalloc(e, &e->ang_par, apar.size())))
         return rc;
@@ -919,6 +955,8 @@ int commit_topology(dpd_engine* e)
     HIPCHK(hipMemcpyAsync(e->ang_ptr, ap.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->bond_ent, bent.data(), sizeof(int2) * bent.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->bond_bead, bbead.data(), sizeof(int2) * bbead.size(), hipMemcpyHostToDevice, e->stream));
+    if (!bonded.empty())
+        HIPCHK(hipMemcpyAsync(e->bonded, bonded.data(), sizeof(int) * bonded.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->bond_par, bpar.data(), sizeof(double2) * bpar.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_ent, aent.data(), sizeof(int2) * aent.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_bead, abead.data(), sizeof(int4) * abead.size(), hipMemcpyHostToDevice, e->stream));
@@ -926,6 +964,7 @@ int commit_topology(dpd_engine* e)
     HIPCHK(hipStreamSynchronize(e->stream));
     e->nbonds = (int)nbonds;
     e->nangles = (int)npairs;
+    e->nbonded = (int)bonded.size();
     e->has_bonds = nbonds > 0;
     e->has_angles = npairs > 0;
     return DPD_OK;
@@ -1027,7 +1066,8 @@ int dpd_create_batch(int32_t nbox, const dpd_params* p, dpd_engine** out)
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (auto& ev : e->ev) HIPCHK(hipEventCreate(&ev));
-    e->pair_kernel = (p[0].pair_kernel == DPD_PAIR_AUTO) ? default_pair_kernel() : p[0].pair_kernel;
+    e->pair_auto = p[0].pair_kernel == DPD_PAIR_AUTO;
+    e->pair_kernel = e->pair_auto ? DPD_PAIR_FULLSHELL : p[0].pair_kernel;
     if (e->pair_kernel < 1 || e->pair_kernel > DPD_PAIR_KINDS) return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
     if ((rc = dalloc(e, &e->flags, 4)) || (rc = dalloc(e, &e->dstep, 1)) ||
         (rc = dalloc(e, &e->acc, (size_t)nbox * kObsStride)) || (rc = dalloc(e, &e->table, (size_t)nbox * kTableStride)) ||
@@ -1051,7 +1091,7 @@ void dpd_destroy(dpd_engine* e)
                     e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
                     e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
-                    e->fbond, e->fang, e->dboxes};
+                    e->fbond, e->fang, e->bonded, e->dboxes};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& ev : e->ev)
@@ -1073,7 +1113,7 @@ int dpd_upload_beads(dpd_engine* e, int64_t n, const int32_t* id, const int32_t*
     if ((rc = commit_beads(e, step0))) return rc;
     if (keep_topo) return commit_topology(e);
     e->has_bonds = e->has_angles = 0;
-    e->nbonds = e->nangles = 0;
+    e->nbonds = e->nangles = e->nbonded = 0;
     return DPD_OK;
 }
 
@@ -1342,7 +1382,8 @@ int64_t dpd_get_step(const dpd_engine* e) { return e ? e->step : -1; }
 int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel)
 {
     if (!e) return DPD_ERR_PARAM;
-    if (kernel == DPD_PAIR_AUTO) kernel = default_pair_kernel();
+    e->pair_auto = kernel == DPD_PAIR_AUTO;
+    if (e->pair_auto) kernel = e->committed ? default_pair_kernel(e->n) : DPD_PAIR_FULLSHELL;
     if (kernel < 1 || kernel > DPD_PAIR_KINDS) return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
     if (kernel != e->pair_kernel) drop_graphs(e);
     e->pair_kernel = kernel;
