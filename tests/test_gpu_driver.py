@@ -1,0 +1,110 @@
+"""The whole-run driver (dpd_run, the equivalent of the reference's `dpd <runId>`,
+src/dmpc.cpp:64-226 and CSimBox::Run, src/SimBox.cpp:1272-1355) on the GPU: history rows
+every SamplePeriod, restart snapshots every RestartPeriod that read back and restart a run,
+commands applied at their times, and the command-line front end."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from dpd_amd import host as H
+from helpers import ROOT
+
+pytestmark = pytest.mark.gpu
+
+WATER = """dpd
+
+Title	" water 8 "
+Date    02/12/26
+Comment	" driver test "
+
+State	{state}
+
+Bead  W
+      0.5
+      25
+      4.5
+
+Polymer	Water    1.0   " (W) "
+
+Box         8 8 8       1  1  1
+Density		3
+Temp        1
+RNGSeed		-4242
+Lambda		0.5
+Step		0.02
+Time		{time}
+SamplePeriod     20
+AnalysisPeriod	 {time}
+DensityPeriod    {time}
+DisplayPeriod    {time}
+RestartPeriod    200
+Grid		1  1  1
+{commands}"""
+
+
+def write(tmp, run_id, state="random", time=400, commands=""):
+    with open(os.path.join(tmp, "dmpci." + run_id), "w") as fh:
+        fh.write(WATER.format(state=state, time=time, commands=commands))
+
+
+def history(tmp, run_id):
+    return np.loadtxt(os.path.join(tmp, "dmpchs." + run_id), ndmin=2)
+
+
+def test_run_writes_history_and_restarts(tmp_path):
+    write(tmp_path, "w")
+    st = H.run(tmp_path, "w")
+    print(f"  driver: {st['steps']} steps, {st['nbeads']} beads, kT {st['last_kT']:.4f}, P {st['last_pressure']:.3f}",
+          flush=True)
+    assert st["steps"] == 400 and st["nbeads"] == 1536 and st["samples"] == 20
+    h = history(tmp_path, "w")
+    assert h.shape == (20, 3)
+    assert np.array_equal(h[:, 0], np.arange(20, 401, 20))
+    late = h[h[:, 0] > 200]
+    # water at a = 25, rho = 3, dt = 0.02: kT ~1.01 and P ~23.7 (reference dmpchs rows)
+    assert abs(late[:, 1].mean() - 1.0) < 0.05
+    assert abs(late[:, 2].mean() - 23.7) < 1.0
+    for t in (200, 400):
+        r = H.read_restart(os.path.join(tmp_path, f"dmpcrs.w.con.{t}.dat"), 1536)
+        assert np.all((r["x"] >= 0.0) & (r["x"] <= 8.0))
+        assert np.isfinite(r["v"]).all()
+        assert abs((r["v"] ** 2).sum() / (3 * 1536) - 1.0) < 0.15
+
+
+def test_restart_continues_the_state(tmp_path):
+    write(tmp_path, "w", time=200)
+    H.run(tmp_path, "w")
+    first = history(tmp_path, "w")
+    write(tmp_path, "r", state="restart RunId w StateId 200", time=200)
+    H.run(tmp_path, "r")
+    second = history(tmp_path, "r")
+    # the restarted run starts from the equilibrated snapshot: no start-up transient
+    assert abs(second[0, 1] - 1.0) < 0.1
+    assert abs(second[:, 2].mean() - first[first[:, 0] > 100][:, 2].mean()) < 1.0
+
+
+def test_commands_are_applied(tmp_path):
+    cmds = ("Command SetTimeStepSize 100 0.01\n"
+            "Command ToggleDPDBeadThermostat 200\n"
+            "Command SetDPDBeadConsInt 300 W W 30.0\n")
+    write(tmp_path, "c", commands=cmds)
+    st = H.run(tmp_path, "c")
+    assert st["commands_applied"] == 3
+    h = history(tmp_path, "c")
+    assert np.isfinite(h).all()
+    # stiffer repulsion after t = 300 raises the pressure
+    assert h[h[:, 0] > 320][:, 2].mean() > h[(h[:, 0] > 120) & (h[:, 0] <= 300)][:, 2].mean() + 1.0
+
+
+def test_command_line_front_end(tmp_path):
+    exe = os.path.join(ROOT, "osprey-dpd_amd", "bin", "dpd_amd")
+    if not os.path.exists(exe):
+        pytest.skip("command-line front end not built (make -C osprey-dpd_amd/csrc)")
+    write(tmp_path, "cli", time=200)
+    r = subprocess.run([exe, "cli"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print("  " + r.stdout.strip(), flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "run cli: 200 steps, 1536 beads" in r.stdout
+    assert os.path.exists(os.path.join(tmp_path, "dmpcrs.cli.con.200.dat"))

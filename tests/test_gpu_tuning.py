@@ -1,0 +1,61 @@
+"""Pair-kernel selection on the GPU: the automatic choice by bead count, dpd_autotune (times
+every kernel, restores the state after each trial, keeps the fastest), and that the chosen
+kernel still reproduces the oracle's forces."""
+import sys
+
+import numpy as np
+import pytest
+
+import dpd_amd as D
+import oracle as O
+from helpers import force_errors, random_state
+
+pytestmark = pytest.mark.gpu
+A2 = np.array([[25.0, 40.0], [40.0, 25.0]])
+G2 = np.full((2, 2), 4.5)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_automatic_choice_by_bead_count():
+    cells = (10, 10, 10)
+    ids, t, x, v = random_state(cells, 3000, 2, seed=8)
+    e = D.Engine(cells, A2, G2, seed=3, pair_kernel=D.PAIR_AUTO)
+    e.upload(ids, t, x, v)
+    assert e.pair_kernel == D.PAIR_STREAM1024   # 3000 beads: below the crossover
+    e.set_pair_kernel(D.PAIR_FULLSHELL)
+    assert e.pair_kernel == D.PAIR_FULLSHELL
+    e.set_pair_kernel(D.PAIR_AUTO)
+    assert e.pair_kernel == D.PAIR_STREAM1024
+    with pytest.raises(D.DpdError):
+        e.set_pair_kernel(D.PAIR_KINDS + 1)
+
+
+def test_autotune_restores_state_and_keeps_parity():
+    cells = (12, 12, 12)
+    ids, t, x, v = random_state(cells, 5184, 2, seed=9)
+    box = O.Box(cells, A2, G2, seed=4, rng_mode=O.RNG_PHILOX)
+    s = O.System(box, ids, t, x, v)
+    s.run(20)
+    x, u, v, f = s.state()
+    e = D.Engine(cells, A2, G2, seed=4, pair_kernel=D.PAIR_FULLSHELL)
+    e.upload(ids, t, x, v, f=f, unpbc=u, step0=20)
+    before = e.download()
+    chosen, ms = e.autotune(nsteps=3)
+    say(f"  autotune chose {chosen}: " + ", ".join(f"{k}: {m:.3f} ms" for k, m in ms.items()))
+    assert 1 <= chosen <= D.PAIR_KINDS and e.pair_kernel == chosen
+    assert all(m > 0 for m in ms.values()) and ms[chosen] == min(ms.values())
+    assert e.step_count == 20
+    after = e.download()
+    for b, a in zip(before, after):
+        assert np.array_equal(b, a)
+    # the tuned engine still steps like the oracle
+    e.step(1)
+    s.run(1, step0=20)
+    X, U, V, F = e.download()
+    norm, _ = force_errors(F, s.state()[3])
+    say(f"  force normwise after autotune {norm:.2e}")
+    assert norm < 1e-5
