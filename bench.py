@@ -9,9 +9,11 @@ warm-up steps.  One "step" is one full CSimBox::Evolve equivalent (sort, pair fo
 bonds, integration) of the whole box; inputs stay in HBM.
 
 Prints one JSON line.  `value` is bead-steps/s on the GPU; `roofline` is the pair kernel
-(the dominant kernel) against HBM bandwidth; `cpu_baseline` times the compiled reference
-(oracle/_ref/dpd) on one host core, on the same configuration, by differencing two run
-lengths so that set-up cancels.
+(the dominant kernel) against HBM bandwidth; `batched` is 256 replicas advanced together
+against the same boxes one at a time; `other_configs` the water 20^3, binary 40^3 and
+bilayer 48x48x32 boxes; `cpu_baseline` times the compiled reference (oracle/_ref/dpd) on
+one host core (and, under `all_cores`, 16 copies at once) on the condensate configuration,
+by differencing two run lengths so that set-up cancels.
 """
 import argparse
 import json
@@ -124,6 +126,24 @@ def bench_batched(nbox=256, steps=100, warmup=20, nsingle=16, seed=20261202, pai
             "ms_per_step": wall / steps * 1e3, "single_box_value": single, "single_box_sample": nsingle,
             "speedup_vs_single_box": batched / single,
             "workload": "256 water/lipid replicas, L~U{8..24}, rho=3, a_ij~U[20,90], k~U[64,256]"}
+
+
+def bench_configs(steps=100, warmup=20):
+    """The other single-GPU configurations (water 20^3, binary 40^3, bilayer 48x48x32):
+    bead-steps/s with the automatic pair-kernel choice, graph replay, inputs resident."""
+    out = {}
+    for wl in (W.water(), W.binary(), W.bilayer()):
+        e = build_engine(wl, D.PAIR_AUTO)
+        e.step(warmup, graph=True)
+        t0 = time.perf_counter()
+        e.step(steps, observe=True, graph=True)
+        wall = time.perf_counter() - t0
+        o = e.observables()
+        out[wl.name] = {"beads": wl.n, "bonds": wl.nbonds, "angles": wl.nangles, "steps": steps,
+                        "value": wl.n * steps / wall, "ms_per_step": wall / steps * 1e3,
+                        "pair_kernel": e.pair_kernel, "kT": o["kT"], "pressure": o["pressure"]}
+        e.close()
+    return out
 
 
 def cpu_baseline_reference(t1=2, t2=6):
@@ -241,6 +261,7 @@ def main():
     ap.add_argument("--pair-kernel", type=int, default=D.PAIR_AUTO)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the water, binary and bilayer boxes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run one 256 MiB device-to-device copy (calibrates the PMC byte counters)")
     args = ap.parse_args()
@@ -273,6 +294,7 @@ def main():
             traffic = json.load(fh).get("bytes_per_launch")
 
     batched = None if args.no_batched else bench_batched(pair_kernel=args.pair_kernel)
+    configs = None if (args.no_batched or args.no_configs) else bench_configs()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -291,6 +313,7 @@ def main():
                      "kernel": "pair", "pair_ms_per_launch": t_pair * 1e3,
                      "step_gbs": alg_bytes / (wall / args.steps) / 1e9},
         "batched": batched,
+        "other_configs": configs,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "speedup_vs_cpu_all_cores": (value / cpu["all_cores"]["value"]) if cpu and cpu.get("all_cores") else None,

@@ -83,3 +83,74 @@ def condensate(cells=(64, 64, 64), rho=3, length=16, polymer_fraction=0.2, k=128
     return Workload("condensate-%dx%dx%d" % tuple(cells), cells, a, np.full((3, 3), 4.5),
                     np.arange(1, n + 1, dtype=np.int32), types, x, u, v,
                     bonds=(head, tail, np.full(nb, k), np.full(nb, l0)), seed=seed)
+
+
+BILAYER_DMPCI = """dpd
+
+Title	" bilayer "
+Date    02/12/26
+Comment	" H3(T4)2 lipids in water "
+
+State	lamella
+        Polymer			Lipid
+        Normal			0 0 1
+        Centre			0.5
+        Thickness		5.0
+        Linearise		1
+        UpperFraction	0.5
+        Polymerise		0
+
+Bead  H
+      0.5
+      25
+      4.5
+
+Bead  T
+      0.5
+      35  25
+      4.5 4.5
+
+Bead  W
+      0.5
+      35  75  25
+      4.5 4.5 4.5
+
+Bond    H H  128  0.5
+Bond    H T  128  0.5
+Bond    T T  128  0.5
+
+BondPair  H T T  20.0  0.0
+BondPair  T T T  20.0  0.0
+
+Polymer	Water    {fw:.6f}   " (W) "
+Polymer	Lipid    {fl:.6f}   " (H H (* (T T T T)) H T T T T) "
+
+Box         {nx} {ny} {nz}       1  1  1
+Density		3
+Temp        1
+RNGSeed		-{seed}
+Lambda		0.5
+Step		0.02
+Time		1000
+SamplePeriod     10
+AnalysisPeriod	 1000
+DensityPeriod    1000
+DisplayPeriod    1000
+RestartPeriod    1000
+Grid		1  1  1
+"""
+
+
+def bilayer(cells=(48, 48, 32), lipid_fraction=0.0201, seed=20261202):
+    """H3(T4)2 lipid bilayer in water (a_HT 35, a_TW 75, a_HW 35, others 25; Hookean bonds
+    k 128, l0 0.5; bond pairs k 20, phi0 0), started as a lamella at the mid-plane; a
+    lipid number fraction of 0.0201 gives ~3.7k lipids in the 48x48x32 box."""
+    from .host import Input
+    text = BILAYER_DMPCI.format(fw=1.0 - lipid_fraction, fl=lipid_fraction, nx=cells[0], ny=cells[1],
+                                nz=cells[2], seed=seed)
+    inp = Input(text=text)
+    ids, types, pid, pt = inp.beads()
+    x, u, v = inp.build_state(seed=seed)
+    bonds, pairs = inp.topology()
+    return Workload("bilayer-%dx%dx%d" % tuple(cells), cells, inp.a, inp.gamma, ids, types, x, u, v,
+                    bonds=bonds, pairs=pairs, seed=seed)
