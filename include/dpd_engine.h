@@ -178,6 +178,24 @@ int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel);
 int32_t dpd_get_pair_kernel(const dpd_engine* e);
 int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step);
 
+/* Stress profile across `nslices` equal z-slices of box `box` (replaces the per-step
+ * tables of CMonitor::AddBeadStress, src/Monitor.cpp:1900-2104, and
+ * CMonitor::AddBondStress, src/Monitor.cpp:2283-2518).  Each interaction adds F (x) dx
+ * (tensor element 3a + b = F_a dx_b; F on the first bead, dx = first - second) to the
+ * slices its two beads span: the whole of it to their slice when both lie in one, else
+ * the share width / |dz| to each slice crossed and the fractions of that to the two end
+ * slices.  pair_out[nslices][nt][nt][9] (may be NULL) holds the pair forces of the
+ * current state -- positions, predicted velocities and pair stream of the last force
+ * evaluation, i.e. the last step's forces after dpd_step and the uploaded state at
+ * step0 after dpd_commit -- with each pair's share split equally between (ti, tj) and
+ * (tj, ti); the reference files a pair under the type of whichever bead its cell
+ * traversal meets first, so only the sum of the two entries is defined for ti != tj.
+ * bond_out[nslices][nt][nt][9] (may be NULL) holds the bonds under (head type, tail
+ * type) with dx = head - tail and F the force on the head.  Sums over slices are the
+ * pair and bond virials.  The tables are instantaneous, as the reference's are. */
+int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_out,
+                     double* bond_out);
+
 #ifdef __cplusplus
 }
 #endif

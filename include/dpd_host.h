@@ -50,6 +50,12 @@ int  dpd_input_beads(const dpd_input* in, int32_t* id, int32_t* type, int32_t* p
  * radians), in the reference's per-polymer order */
 int  dpd_input_topology(const dpd_input* in, int32_t* head, int32_t* tail, double* k, double* l0,
                         int32_t* first, int32_t* second, double* modulus, double* phi0);
+/* Bond types in declaration order (CInputData::AddBondType, src/InputData.cpp:556-598):
+ * *count of them, with the bead types named at head and tail (arrays of *count entries,
+ * may be NULL; call with both NULL to size them).  A bond's type is the declaration whose
+ * two names match its beads in either order (src/InputData.cpp:1489-1498); the bond
+ * slice-stress tables of the reference are keyed by it. */
+int  dpd_input_bond_types(const dpd_input* in, int32_t* count, int32_t* head_type, int32_t* tail_type);
 /* Engine parameters of this input (pointers stay valid while `in` lives). */
 int  dpd_input_params(const dpd_input* in, int32_t device, dpd_params* out);
 /* Starting state for State random / lamella: positions (wrapped and unwrapped) and

@@ -294,13 +294,23 @@ typedef struct pair_ctx {
 } pair_ctx;
 
 /* body of the pair loop, src/CNTCell.cpp:1191-1301 and 1409-1528 */
+static int pair_force_out(const pair_ctx* c, int t1, int t2, int id1, int id2, const double dx[3],
+                          const double dv[3], double* f1, double* f2, double* stress1, double* nf_out);
+
 static void pair_force(const pair_ctx* c, int t1, int t2, int id1, int id2, const double dx[3],
                        const double dv[3], double* f1, double* f2, double* stress1)
 {
+    pair_force_out(c, t1, t2, id1, id2, dx, dv, f1, f2, stress1, NULL);
+}
+
+/* as pair_force; returns 1 for an interacting pair and its force on bead 1 in nf_out */
+static int pair_force_out(const pair_ctx* c, int t1, int t2, int id1, int id2, const double dx[3],
+                          const double dv[3], double* f1, double* f2, double* stress1, double* nf_out)
+{
     double dr2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
-    if (!(dr2 < 1.0)) return;
+    if (!(dr2 < 1.0)) return 0;
     double dr = sqrt(dr2);
-    if (!(dr > 0.000000001)) return;
+    if (!(dr > 0.000000001)) return 0;
     double wr = (1.0 - dr);
     double wr2 = wr * wr;
     double conForce = c->a[t1 * c->nt + t2] * wr;
@@ -325,6 +335,8 @@ static void pair_force(const pair_ctx* c, int t1, int t2, int id1, int id2, cons
         stress1[6] += dx[0] * nf[2]; stress1[7] += dx[1] * nf[2]; stress1[8] += dx[2] * nf[2];
     }
     if (c->pe) *c->pe += 0.5 * c->a[t1 * c->nt + t2] * wr2;
+    if (nf_out) { nf_out[0] = nf[0]; nf_out[1] = nf[1]; nf_out[2] = nf[2]; }
+    return 1;
 }
 
 /* CCNTCell::UpdateForce over all cells in order (src/CNTCell.cpp:1144-1555) */
@@ -588,4 +600,112 @@ void orc_pair_forces_brute(const orc_params* p, int32_t n, const int32_t* id,
             pair_force(&ctx, type[i], type[j], id[i], id[j], dx, dv, f + 3 * i, f + 3 * j, st);
         }
     if (virial) memcpy(virial, st, sizeof(st));
+}
+
+/* ------------------------------------------------------- slice stress -- */
+
+/* CMonitor::AddBeadStress (src/Monitor.cpp:1900-2104), also the split of
+ * CMonitor::AddBondStress: fdx = F (x) dx of one interaction is shared between the
+ * z-slices spanned by the two beads (wrapped z coordinates z1 of the bead F acts on,
+ * z2 of the other; dx = x1 - x2 with the periodic correction).  Same slice: all of it,
+ * or half to each neighbour slice when both beads sit on the slice border; otherwise
+ * slice_width / |dz| to every slice strictly between and the fractional parts to the
+ * end slices, wrapping across the periodic boundary.  Adds wgt * share into
+ * tab[(slice * nidx + idx) * 9 + component]. */
+void orc_slice_add(double* tab, int32_t nslice, double Lz, int32_t idx, int32_t nidx, double z1, double z2,
+                   const double F[3], const double dx[3], double wgt)
+{
+    const double width = Lz / nslice, inv = nslice / Lz;
+    double fdx[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) fdx[3 * a + b] = F[a] * dx[b] * wgt;
+    double zd = dx[2], zu, zl;
+    if (zd > 0.00001) { zu = z1 * inv; zl = z2 * inv; }
+    else if (zd < -0.00001) { zd = -zd; zu = z2 * inv; zl = z1 * inv; }
+    else { zu = zl = z1 * inv; }
+    long iu = (long)zu, il = (long)zl;
+    const double ru = zu - (double)iu, rl = zl - (double)il;
+#define SLICE_ADD(sl, share) \
+    for (int q_ = 0; q_ < 9; ++q_) tab[((long)(sl) * nidx + idx) * 9 + q_] += fdx[q_] * (share)
+    if (iu == il) {
+        if (zu != zl || ru > 0.00001) {
+            SLICE_ADD(il, 1.0);
+        } else {
+            SLICE_ADD(iu, 0.5);
+            if (--il < 0) il += nslice;
+            SLICE_ADD(il, 0.5);
+        }
+    } else {
+        const double inner = width / zd;
+        if (iu > il) {
+            for (long sl = il + 1; sl <= iu - 1; ++sl) SLICE_ADD(sl, inner);
+        } else {
+            for (long sp = il + 1; sp <= nslice + iu - 1; ++sp) SLICE_ADD(sp >= nslice ? sp - nslice : sp, inner);
+        }
+        if (rl > 0.000001) SLICE_ADD(il, (1.0 - rl) * inner);
+        else SLICE_ADD(il, inner);
+        if (ru > 0.000001) SLICE_ADD(iu, ru * inner);
+    }
+#undef SLICE_ADD
+}
+
+void orc_slice_stress(const orc_params* p, int32_t n, const int32_t* id, const int32_t* type,
+                      const double* x, const double* unpbc, const double* v, const orc_topology* topo,
+                      int64_t step, int32_t thermostat, int32_t nslice, double* pair_tab, double* bond_tab)
+{
+    orc_params q = *p;
+    q.rng_mode = ORC_RNG_PHILOX;
+    orc_sys* s = orc_create(&q, n, id, type, x, unpbc, v, NULL, topo);
+    const int nt = s->p.ntypes, nidx = nt * nt;
+    const double Lz = s->L[2];
+    pair_ctx ctx = {s->a, s->g, nt, s->invrootdt, ORC_RNG_PHILOX, &s->lcg, s->p.seed, step, thermostat, NULL};
+    if (pair_tab) memset(pair_tab, 0, sizeof(double) * (size_t)nslice * nidx * 9);
+    if (bond_tab) memset(bond_tab, 0, sizeof(double) * (size_t)nslice * nidx * 9);
+    memset(s->f, 0, sizeof(double) * 3 * n);
+    /* the reference's traversal: bead1 in the current cell, bead2 later in it or in
+     * the 13-cell half shell (CCNTCell::UpdateForce) */
+    for (int c = 0; c < s->ncell && pair_tab; ++c) {
+        for (int b1 = s->head[c]; b1 >= 0; b1 = s->next[b1]) {
+            const double* x1 = s->x + 3 * b1;
+            const double* v1 = s->v + 3 * b1;
+            for (int b2 = s->tail[c]; b2 != b1; b2 = s->prev[b2]) {
+                double dx[3], dv[3], nf[3];
+                for (int d = 0; d < 3; ++d) { dx[d] = x1[d] - s->x[3 * b2 + d]; dv[d] = v1[d] - s->v[3 * b2 + d]; }
+                if (pair_force_out(&ctx, s->type[b1], s->type[b2], s->id[b1], s->id[b2], dx, dv, s->f + 3 * b1,
+                                   s->f + 3 * b2, NULL, nf))
+                    orc_slice_add(pair_tab, nslice, Lz, s->type[b1] * nt + s->type[b2], nidx, x1[2],
+                                  s->x[3 * b2 + 2], nf, dx, 1.0);
+            }
+            for (int h = 0; h < 13; ++h) {
+                int nc = s->nn27[27 * c + 13 + HALF[h][0] + 3 * HALF[h][1] + 9 * HALF[h][2]];
+                int pbc = s->ext[c] && s->ext[nc];
+                for (int b2 = s->head[nc]; b2 >= 0; b2 = s->next[b2]) {
+                    double dx[3], dv[3], nf[3];
+                    for (int d = 0; d < 3; ++d) { dx[d] = x1[d] - s->x[3 * b2 + d]; dv[d] = v1[d] - s->v[3 * b2 + d]; }
+                    if (pbc)
+                        for (int d = 0; d < 3; ++d) {
+                            if (dx[d] > s->halfL[d]) dx[d] -= s->L[d];
+                            else if (dx[d] < -s->halfL[d]) dx[d] += s->L[d];
+                        }
+                    if (pair_force_out(&ctx, s->type[b1], s->type[b2], s->id[b1], s->id[b2], dx, dv, s->f + 3 * b1,
+                                       s->f + 3 * b2, NULL, nf))
+                        orc_slice_add(pair_tab, nslice, Lz, s->type[b1] * nt + s->type[b2], nidx, x1[2],
+                                      s->x[3 * b2 + 2], nf, dx, 1.0);
+                }
+            }
+        }
+    }
+    /* bonds: head - tail (unwrapped) and the force on the head, keyed by the ordered
+     * (head type, tail type) pair */
+    for (int i = 0; i < s->nb && bond_tab; ++i) {
+        const int h = s->bh[i], t = s->bt[i];
+        double d[3];
+        for (int k = 0; k < 3; ++k) d[k] = s->u[3 * h + k] - s->u[3 * t + k];
+        const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double sc = s->bk[i] * (s->bl0[i] - L) / L;
+        const double F[3] = {sc * d[0], sc * d[1], sc * d[2]};
+        orc_slice_add(bond_tab, nslice, Lz, s->type[h] * nt + s->type[t], nidx, s->x[3 * h + 2], s->x[3 * t + 2],
+                      F, d, 1.0);
+    }
+    orc_destroy(s);
 }

@@ -15,6 +15,7 @@
  *   CBond::AddForce          src/Bond.cpp:111-162
  *   CBondPair::AddForce      src/BondPair.cpp:138-286
  *   CMonitor::CalculateSingleBeadData src/Monitor.cpp:790-1041 (kT, P, stress)
+ *   CMonitor::AddBeadStress / AddBondStress src/Monitor.cpp:1900-2104, :2283-2518 (slice stress)
  *
  * Two random-force modes:
  *   ORC_RNG_LCG     the reference's serial LCG, drawn in the reference's linked-list
@@ -100,6 +101,19 @@ void orc_forces(const orc_params* p, int32_t n, const int32_t* id, const int32_t
 void orc_pair_forces_brute(const orc_params* p, int32_t n, const int32_t* id,
                            const int32_t* type, const double* x, const double* v,
                            int64_t step, int32_t thermostat, double* f, double* virial);
+
+/* z-slice split of one interaction (CMonitor::AddBeadStress, src/Monitor.cpp:1900-2104):
+ * adds wgt * share of F (x) dx into tab[(slice * nidx + idx) * 9 + c]; z1 is the wrapped
+ * z of the bead F acts on, z2 of the other, dx = x1 - x2 (minimum image). */
+void orc_slice_add(double* tab, int32_t nslice, double Lz, int32_t idx, int32_t nidx, double z1, double z2,
+                   const double F[3], const double dx[3], double wgt);
+/* The per-step slice stress tables of one force evaluation at (x, v, step) (Philox
+ * stream): pair_tab[nslice][ntypes*ntypes][9] with the reference's half-shell order
+ * (bead1 type first), bond_tab[nslice][ntypes*ntypes][9] keyed by (head type, tail
+ * type); either may be NULL. */
+void orc_slice_stress(const orc_params* p, int32_t n, const int32_t* id, const int32_t* type,
+                      const double* x, const double* unpbc, const double* v, const orc_topology* topo,
+                      int64_t step, int32_t thermostat, int32_t nslice, double* pair_tab, double* bond_tab);
 
 /* reference LCG (src/CNTCell.cpp:2454-2465): advances *state, returns U in [0,1) */
 double   orc_lcg_uniform(uint64_t* state);

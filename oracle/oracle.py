@@ -63,6 +63,11 @@ def lib():
         L.orc_pair_forces_brute.argtypes = [C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
                                             C.c_void_p, C.c_void_p]
+        L.orc_slice_add.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_double,
+                                    C.c_double, C.c_void_p, C.c_void_p, C.c_double]
+        L.orc_slice_stress.argtypes = [C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.POINTER(Topology), C.c_int64, C.c_int32,
+                                       C.c_int32, C.c_void_p, C.c_void_p]
         L.orc_lcg_uniform.restype = C.c_double
         L.orc_lcg_uniform.argtypes = [C.POINTER(C.c_uint64)]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -194,6 +199,38 @@ def forces(box, ids, types, x, v, step, thermostat=True, unpbc=None, topo=None):
     L.orc_forces(C.byref(p), n, _p(ids), _p(types), _p(x), _p(u), _p(v), C.byref(ts), int(step),
                  int(thermostat), _p(f), _p(vir), _p(pe))
     return f, vir, pe
+
+
+def slice_stress(box, ids, types, x, v, step, nslice, thermostat=True, unpbc=None, topo=None):
+    """The slice stress tables of one force evaluation (CMonitor::AddBeadStress /
+    AddBondStress): pair[nslice, ntypes*ntypes, 9] in the reference's half-shell order
+    (type of the bead the force acts on first), bond[nslice, ntypes*ntypes, 9] keyed by
+    (head type, tail type)."""
+    L = lib()
+    n = len(ids)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    u = np.ascontiguousarray(unpbc, dtype=np.float64) if unpbc is not None else None
+    p = box.params()
+    t = (topo or Topo())
+    ts = t.struct()
+    nt = box.ntypes
+    pair = np.zeros((nslice, nt * nt, 9))
+    bond = np.zeros((nslice, nt * nt, 9))
+    L.orc_slice_stress(C.byref(p), n, _p(ids), _p(types), _p(x), _p(u), _p(v), C.byref(ts), int(step),
+                       int(thermostat), int(nslice), _p(pair), _p(bond))
+    return pair, bond
+
+
+def slice_add(tab, Lz, idx, z1, z2, F, dx, wgt=1.0):
+    """One interaction's z-slice split into tab[nslice, nidx, 9] (in place)."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    lib().orc_slice_add(_p(tab), tab.shape[0], float(Lz), int(idx), tab.shape[1], float(z1), float(z2), _p(F),
+                        _p(dx), float(wgt))
+    return tab
 
 
 def pair_forces_brute(box, ids, types, x, v, step, thermostat=True):

@@ -27,6 +27,7 @@
 #include "dpd_engine.h"
 #include "engine_common.hpp"
 #include "pair_kernels.hpp"
+#include "slice_stress.hpp"
 
 using namespace dpd;
 
@@ -521,6 +522,8 @@ struct dpd_engine {
     double* acc = nullptr;
     float* table = nullptr;
     double* gather = nullptr;   // 12n doubles for downloads
+    double* slice = nullptr;    // slice stress tables (pair | bond)
+    size_t slice_cap = 0;
     // topology
     int *bond_ptr = nullptr, *ang_ptr = nullptr;
     int2 *bond_ent = nullptr, *bond_bead = nullptr, *ang_ent = nullptr;
@@ -1091,7 +1094,7 @@ void dpd_destroy(dpd_engine* e)
                     e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
                     e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
-                    e->fbond, e->fang, e->bonded, e->dboxes};
+                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& ev : e->ev)
@@ -1459,6 +1462,48 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
     e->obs_valid = false;
     if (chosen) *chosen = bestk;
     return dpd_set_pair_kernel(e, bestk);
+}
+
+int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_out, double* bond_out)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (box < 0 || box >= (int32_t)e->boxes.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    if (nslices < 1 || nslices > 65536) return fail(e, DPD_ERR_PARAM, "nslices must be in [1, 65536]");
+    HIPCHK(hipSetDevice(e->device));
+    const BoxHost& h = e->boxes[box];
+    const int nt = h.B.ntypes;
+    const size_t ntab = (size_t)nslices * nt * nt * 9;
+    if (2 * ntab > e->slice_cap) {
+        int rc;
+        if ((rc = dalloc(e, &e->slice, 2 * ntab))) { e->slice_cap = 0; return rc; }
+        e->slice_cap = 2 * ntab;
+    }
+    HIPCHK(hipMemsetAsync(e->slice, 0, sizeof(double) * 2 * ntab, e->stream));
+    SliceArgs a;
+    a.B = h.B;
+    a.pos = e->pos[e->cur];
+    a.vp = e->vp;
+    a.aux = e->aux[e->cur];
+    a.key = e->key[e->cur];
+    a.cstart = e->cstart;
+    a.table = e->table + (size_t)box * kTableStride;
+    a.step = e->dstep;
+    a.slot_of = e->slot_of;
+    a.bond_bead = e->bond_bead;
+    a.bond_par = e->bond_par;
+    a.b0 = 0;
+    for (int32_t b = 0; b < box; ++b) a.b0 += (int)e->boxes[b].bh.size();
+    a.b1 = a.b0 + (int)h.bh.size();
+    a.nslice = nslices;
+    a.ntypes = nt;
+    a.nidx = nt * nt;
+    HIPCHK(launch_slice_stress(a, pair_out ? e->slice : nullptr, bond_out ? e->slice + ntab : nullptr, e->stream));
+    if (pair_out) HIPCHK(hipMemcpyAsync(pair_out, e->slice, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
+    if (bond_out)
+        HIPCHK(hipMemcpyAsync(bond_out, e->slice + ntab, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DPD_OK;
 }
 
 }  // extern "C"

@@ -283,6 +283,18 @@ int dpd_input_beads(const dpd_input* in, int32_t* id, int32_t* type, int32_t* po
     return DPD_OK;
 }
 
+int dpd_input_bond_types(const dpd_input* in, int32_t* count, int32_t* head_type, int32_t* tail_type)
+{
+    if (!in) return DPD_ERR_PARAM;
+    const auto& b = in->d.bonds;
+    if (count) *count = (int32_t)b.size();
+    for (size_t i = 0; i < b.size(); ++i) {
+        if (head_type) head_type[i] = in->d.type_of(b[i].head);
+        if (tail_type) tail_type[i] = in->d.type_of(b[i].tail);
+    }
+    return DPD_OK;
+}
+
 int dpd_input_topology(const dpd_input* in, int32_t* head, int32_t* tail, double* k, double* l0, int32_t* first,
                        int32_t* second, double* modulus, double* phi0)
 {

@@ -76,6 +76,7 @@ SIGNATURES = {
     "dpd_observables_box": (C.c_int, [_P, C.c_int32, C.POINTER(Obs)]),
     "dpd_download_box": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     "dpd_set_box_interactions": (C.c_int, [_P, C.c_int32, _P, _P]),
+    "dpd_slice_stress": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P]),
 }
 
 _lib = None
@@ -232,6 +233,19 @@ class Engine:
         self._check(lib().dpd_autotune(self.h, int(nsteps), C.byref(chosen), _ptr(ms)))
         return chosen.value, {k + 1: float(ms[k]) for k in range(PAIR_KINDS)}
 
+    def _ntypes(self, box):
+        return self._a.shape[0]
+
+    def slice_stress(self, nslices, box=0, bonds=True):
+        """Stress profile over `nslices` z-slices of the current state (CMonitor::AddBeadStress
+        / AddBondStress, src/Monitor.cpp:1900-2104, :2283-2518): (pair, bond) tables of shape
+        [nslices, ntypes*ntypes, 9], element 3a + b = F_a dx_b; bond is None when not asked."""
+        nt = self._ntypes(box)
+        pair = np.zeros((int(nslices), nt * nt, 9))
+        bond = np.zeros((int(nslices), nt * nt, 9)) if bonds else None
+        self._check(lib().dpd_slice_stress(self.h, int(box), int(nslices), _ptr(pair), _ptr(bond)))
+        return pair, bond
+
 
 def _params(cells, a, gamma, dt, lam, kT, seed, widths, device, pair_kernel, keep):
     a = np.ascontiguousarray(a, dtype=np.float64)
@@ -272,8 +286,12 @@ class Batch(Engine):
             self.close()
             raise DpdError(rc, msg)
         self.nbox = len(boxes)
+        self.box_ntypes = [np.asarray(b["a"]).shape[0] for b in boxes]
         self.box_n = [0] * self.nbox
         self.n = 0
+
+    def _ntypes(self, box):
+        return self.box_ntypes[box]
 
     def upload_box(self, box, ids, types, x, v, f=None, unpbc=None):
         n = len(ids)

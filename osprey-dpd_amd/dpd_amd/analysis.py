@@ -97,3 +97,18 @@ def area_per_lipid(x, types, L, head_type, tail_type, nlipids, grid=4):
             area += 0.5 * np.linalg.norm(np.cross(p10 - p00, p11 - p00))
             area += 0.5 * np.linalg.norm(np.cross(p11 - p00, p01 - p00))
     return area / (0.5 * nlipids)
+
+
+def bond_type_slices(bond_tab, bond_types, ntypes):
+    """The reference's bond slice-stress table, keyed by bond type, from a table keyed by
+    (head type, tail type) such as dpd_slice_stress returns.  bond_types lists (head type,
+    tail type) per Bond declaration (Input.bond_types()); a bond's type is the declaration
+    that names its two bead types in either order (src/InputData.cpp:556-598, :1489-1498),
+    so its entry is the sum of the (h, t) and (t, h) entries."""
+    bond_tab = np.asarray(bond_tab)
+    out = np.zeros((bond_tab.shape[0], len(bond_types), bond_tab.shape[2]))
+    for b, (h, t) in enumerate(bond_types):
+        out[:, b] = bond_tab[:, h * ntypes + t]
+        if h != t:
+            out[:, b] += bond_tab[:, t * ntypes + h]
+    return out

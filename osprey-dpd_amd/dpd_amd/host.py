@@ -36,6 +36,7 @@ HOST_SIGNATURES = {
     "dpd_input_types": (C.c_int, [_P, C.c_char_p, C.c_int32, _P, _P]),
     "dpd_input_beads": (C.c_int, [_P, _P, _P, _P, _P]),
     "dpd_input_topology": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "dpd_input_bond_types": (C.c_int, [_P, C.POINTER(C.c_int32), _P, _P]),
     "dpd_input_params": (C.c_int, [_P, C.c_int32, _P]),
     "dpd_input_build_state": (C.c_int, [_P, C.c_uint64, _P, _P, _P]),
     "dpd_input_command": (C.c_int, [_P, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int64),
@@ -119,6 +120,14 @@ class Input:
         hlib().dpd_input_topology(self.h, _ptr(h), _ptr(t), _ptr(k), _ptr(l0), _ptr(fi), _ptr(se),
                                   _ptr(mo), _ptr(ph))
         return (h, t, k, l0), (fi, se, mo, ph)
+
+    def bond_types(self):
+        """(head type, tail type) of each Bond declaration, in declaration order."""
+        n = C.c_int32()
+        hlib().dpd_input_bond_types(self.h, C.byref(n), None, None)
+        h, t = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        hlib().dpd_input_bond_types(self.h, C.byref(n), _ptr(h), _ptr(t))
+        return list(zip(h.tolist(), t.tolist()))
 
     def build_state(self, seed=None):
         n = self.counts["nbeads"]

@@ -1,0 +1,118 @@
+"""GPU tests of dpd_slice_stress, the z-slice stress profile (CMonitor::AddBeadStress /
+AddBondStress, src/Monitor.cpp:1900-2104, :2283-2518), against the oracle's restatement
+of the reference's half-shell traversal, and at larger sizes through the sum rule: summed
+over slices, the pair and bond tables are the virial the step's observables report."""
+import sys
+
+import numpy as np
+import pytest
+
+import dpd_amd as D
+import oracle as O
+from helpers import chain_state
+
+pytestmark = pytest.mark.gpu
+A3 = np.array([[25.0, 40.0, 30.0], [40.0, 25.0, 35.0], [30.0, 35.0, 25.0]])
+G3 = np.full((3, 3), 4.5)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def sym(t):
+    """pair table [ns, nt*nt, 9] -> [ns, nt, nt, 9] with (ti, tj) + (tj, ti): the reference
+    files a pair under the type of the bead its traversal meets first, so only this sum is
+    fixed for ti != tj."""
+    nt = int(round(np.sqrt(t.shape[1])))
+    t = t.reshape(t.shape[0], nt, nt, 9)
+    return t + t.swapaxes(1, 2)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def chains(cells, seed):
+    n = 3 * int(np.prod(cells))
+    nch = n // 60
+    ids, t, x, u, v, (h, tl, fi, se) = chain_state(cells, nch, 10, n - 10 * nch, seed=seed)
+    # bonds are built at length 0.5; unstretched lengths away from it give every bond a force
+    k = np.full(len(h), 128.0)
+    l0 = np.random.default_rng(seed).uniform(0.3, 0.8, size=len(h))
+    return ids, t, x, u, v, (h, tl, k, l0, fi, se, np.full(len(fi), 20.0), np.zeros(len(fi)))
+
+
+# (nslices, which path): 10 slices of a 3-type box keep the table in LDS (3.2 KB); 150
+# slices (48.6 KB) accumulate with global atomics; 37 slices of width 0.30 make most pairs
+# span several slices
+@pytest.mark.parametrize("nslices", [1, 10, 37, 150])
+def test_tables_match_oracle(nslices):
+    cells = (8, 9, 11)
+    ids, t, x, u, v, topo = chains(cells, seed=31)
+    box = O.Box(cells, A3, G3, seed=77)
+    step = 13
+    pr, br = O.slice_stress(box, ids, t, x, v, step, nslices, unpbc=u, topo=O.Topo(*topo))
+    e = D.Engine(cells, A3, G3, seed=77, pair_kernel=D.PAIR_FULLSHELL)
+    e.upload(ids, t, x, v, unpbc=u, step0=step)
+    e.upload_topology(*topo)
+    pg, bg = e.slice_stress(nslices)
+    ep, eb = rel(sym(pg), sym(pr)), rel(bg, br)
+    say(f"  {nslices} slices: pair {ep:.2e} bond {eb:.2e}")
+    assert pg.shape == pr.shape and bg.shape == br.shape
+    assert ep < 1e-5
+    assert eb < 1e-5
+    # the reference's ordering convention for bonds is exact: (head type, tail type)
+    nt = 3
+    for ti in range(nt):
+        for tj in range(nt):
+            if not np.any(br[:, ti * nt + tj]):
+                assert not np.any(bg[:, ti * nt + tj])
+    # a pair-only call gives the same pair table (FP64 atomics sum in any order)
+    p2, b2 = e.slice_stress(nslices, bonds=False)
+    assert b2 is None and np.allclose(p2, pg, rtol=1e-12, atol=1e-9)
+
+
+def test_tables_match_oracle_for_one_box_of_a_batch():
+    """Box 1 of a two-box batch: its slots and bonds are found inside the batch's arrays."""
+    specs = [((7, 8, 9), 5), ((9, 8, 10), 6)]
+    boxes = [{"cells": c, "a": A3, "gamma": G3, "seed": s} for c, s in specs]
+    e = D.Batch(boxes, pair_kernel=D.PAIR_FULLSHELL)
+    states = []
+    for i, (cells, _) in enumerate(specs):
+        ids, t, x, u, v, topo = chains(cells, seed=40 + i)
+        e.upload_box(i, ids, t, x, v, unpbc=u)
+        e.upload_box_topology(i, *topo)
+        states.append((ids, t, x, u, v, topo))
+    e.commit(step0=5)
+    for i, (cells, seed) in enumerate(specs):
+        ids, t, x, u, v, topo = states[i]
+        pr, br = O.slice_stress(O.Box(cells, A3, G3, seed=seed), ids, t, x, v, 5, 12, unpbc=u,
+                                topo=O.Topo(*topo))
+        pg, bg = e.slice_stress(12, box=i)
+        ep, eb = rel(sym(pg), sym(pr)), rel(bg, br)
+        say(f"  box {i}: pair {ep:.2e} bond {eb:.2e}")
+        assert ep < 1e-5 and eb < 1e-5
+
+
+@pytest.mark.parametrize("cells,nslices", [((16, 16, 24), 24), ((48, 48, 32), 64)])
+def test_sum_rule_after_steps(cells, nslices):
+    """After dpd_step the tables hold the last step's forces: summed over slices and type
+    pairs they equal V * stress of the step's observables (pairs and bonds)."""
+    ids, t, x, u, v, topo = chains(cells, seed=50)
+    e = D.Engine(cells, A3, G3, seed=9)
+    e.upload(ids, t, x, v, unpbc=u, step0=0)
+    e.upload_topology(*topo)
+    e.step(6, observe=True)
+    pg, bg = e.slice_stress(nslices)
+    ob = e.observables()
+    V = float(np.prod(cells))
+    tot = pg.sum(axis=(0, 1)) + bg.sum(axis=(0, 1))
+    err = np.linalg.norm(tot - V * ob["stress"]) / np.linalg.norm(V * ob["stress"])
+    say(f"  {len(ids)} beads, {nslices} slices: sum rule {err:.2e}")
+    assert err < 1e-5
+    # a homogeneous melt has a flat normal-stress profile
+    pzz = (pg[..., 8] + bg[..., 8]).sum(axis=1)
+    say(f"  P_zz per slice: min {pzz.min():.1f} max {pzz.max():.1f}")
+    assert np.all(np.abs(pzz - pzz.mean()) < 0.2 * pzz.mean())

@@ -259,3 +259,79 @@ def test_oracle_bonded_matches_reference(case, name, nsteps):
     assert six_digit_close(u, ref["u"]).all()
     assert six_digit_close(v, ref["v"]).all()
     assert six_digit_close(f, ref["f"]).all()
+
+
+# ---- slice stress (CMonitor::AddBeadStress, src/Monitor.cpp:1900-2104) ----
+
+def _shares(nslice, Lz, z1, z2, dz):
+    """Per-slice share of one interaction, from the oracle's split (F = dx = unit z-ish)."""
+    tab = np.zeros((nslice, 1, 9))
+    F = np.array([0.0, 0.0, 1.0])
+    dx = np.array([0.0, 0.0, dz])
+    O.slice_add(tab, Lz, 0, z1, z2, F, dx)
+    return tab[:, 0, 8] / dz if abs(dz) > 1e-12 else tab[:, 0, 8]
+
+
+def test_slice_split_closed_forms():
+    # same slice: all of it
+    s = _shares(10, 10.0, 3.7, 3.4, 0.3)
+    assert np.allclose(s, np.eye(10)[3])
+    # adjacent slices, width 1: lower (1 - 0.6)/0.7, upper 0.3/0.7
+    s = _shares(10, 10.0, 4.3, 3.6, 0.7)
+    want = np.zeros(10); want[3] = 0.4 / 0.7; want[4] = 0.3 / 0.7
+    assert np.allclose(s, want, rtol=1e-12)
+    # four slices of width 0.25 spanned: inner 0.25/0.7 each, ends by their fractions
+    s = _shares(40, 10.0, 4.3, 3.6, 0.7)
+    want = np.zeros(40); want[14] = 0.6 * 0.25 / 0.7; want[15] = want[16] = 0.25 / 0.7; want[17] = 0.2 * 0.25 / 0.7
+    assert np.allclose(s, want, rtol=1e-12)
+    # bead 2 above bead 1 (dz < 0): the same split with the roles swapped
+    s2 = _shares(40, 10.0, 3.6, 4.3, -0.7)
+    assert np.allclose(s2, want, rtol=1e-12)
+    # across the periodic boundary: bead 1 at 0.3 sits above bead 2 at 9.6
+    s = _shares(10, 10.0, 0.3, 9.6, 0.7)
+    want = np.zeros(10); want[9] = 0.4 / 0.7; want[0] = 0.3 / 0.7
+    assert np.allclose(s, want, rtol=1e-12)
+    s = _shares(40, 10.0, 0.3, 9.6, 0.7)
+    want = np.zeros(40); want[38] = 0.6 * 0.25 / 0.7; want[39] = want[0] = 0.25 / 0.7; want[1] = 0.2 * 0.25 / 0.7
+    assert np.allclose(s, want, rtol=1e-12)
+    # equal heights on a slice border: half to each side of the border
+    tab = np.zeros((10, 1, 9))
+    O.slice_add(tab, 10.0, 0, 3.0, 3.0, [1.0, 0.0, 0.0], [0.5, 0.0, 0.0])
+    assert np.isclose(tab[3, 0, 0], 0.25) and np.isclose(tab[2, 0, 0], 0.25) and np.isclose(tab.sum(), 0.5)
+    # every split sums to one
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        z1, z2 = rng.uniform(0, 10, 2)
+        dz = z1 - z2
+        dz -= 10.0 * np.round(dz / 10.0)
+        if abs(dz) >= 1.0:
+            continue
+        for ns in (1, 7, 10, 40):
+            assert np.isclose(_shares(ns, 10.0, z1, z2, dz).sum(), 1.0)
+
+
+def test_slice_stress_sums_to_the_virial():
+    """Summed over slices and type pairs, the slice tables are the pair (and bond)
+    virial of the same force evaluation: every pair counted once with F (x) dx."""
+    cells = (6, 5, 7)
+    ids, t, x, v = _random_state(cells, 630, 2, 11)
+    box = O.Box(cells, np.array(BINARY_A), np.array(BINARY_G), seed=7)
+    pair, bond = O.slice_stress(box, ids, t, x, v, step=4, nslice=14)
+    _, vir, _ = O.forces(box, ids, t, x, v, step=4)
+    assert np.allclose(pair.sum(axis=(0, 1)), vir, rtol=1e-10, atol=1e-9)
+    assert np.abs(bond).max() == 0.0
+    # the same system with chains: the bond table carries the bond virial
+    n = len(ids)
+    head = np.arange(0, 60, 2, dtype=np.int32)
+    tail = head + 1
+    topo = O.Topo(head=head, tail=tail, k=np.full(30, 64.0), l0=np.full(30, 0.5))
+    xb = x.copy()
+    xb[tail] = np.mod(x[head] + 0.6, np.array(cells, float))
+    pair_b, bond_b = O.slice_stress(box, ids, t, xb, v, step=4, nslice=14, topo=topo)
+    _, vir_b, _ = O.forces(box, ids, t, xb, v, step=4, topo=topo)
+    _, vir_nb, _ = O.forces(box, ids, t, xb, v, step=4)
+    assert np.allclose(bond_b.sum(axis=(0, 1)), vir_b - vir_nb, rtol=1e-10, atol=1e-9)
+    assert np.allclose(pair_b.sum(axis=(0, 1)), vir_nb, rtol=1e-10, atol=1e-9)
+    # slices see where the pairs are: a profile along z of a uniform fluid is flat
+    zz = pair.sum(axis=1)[:, [0, 4, 8]].sum(axis=1)
+    assert zz.std() < 0.25 * zz.mean()

@@ -133,3 +133,39 @@ def test_lamella_state_builder():
     assert (np.abs(np.mod(u[water, 2] - zc + 4.0, 8.0) - 4.0) >= 2.5 - 1e-9).all()
     (h, tl, k, l0), _ = inp.topology()
     assert np.linalg.norm(u[h] - u[tl], axis=1).max() < 1.0
+
+
+def test_bond_type_keying_of_the_slice_stress():
+    """The reference keys its bond slice-stress table by bond type, the Bond declaration
+    whose names match the bond's beads in either order (src/InputData.cpp:556-598,
+    :1489-1498; src/Monitor.cpp:2305).  Folding the (head type, tail type) table into
+    declarations must equal accumulating each bond directly under its declaration."""
+    import oracle as O
+    from dpd_amd import analysis as A
+    text = open(os.path.join(GOLDEN, "lipid8", "case.dmpci")).read().replace(
+        "restart RunId golden StateId 0",
+        "lamella Polymer Lipid Normal 0 0 1 Centre 0.5 Thickness 5.0 Linearise 1 UpperFraction 0.5 Polymerise 0")
+    inp = H.Input(text=text)
+    assert inp.bond_types() == [(0, 0), (0, 1), (1, 1)]
+    x, u, v = inp.build_state(seed=4)
+    ids, t, pid, pt = inp.beads()
+    (h, tl, k, l0), (fi, se, mo, ph) = inp.topology()
+    # stretch the bonds away from rest so every one carries a force
+    l0 = np.random.default_rng(0).uniform(0.3, 0.8, size=len(h))
+    box = O.Box(inp.cells, inp.a, inp.gamma, seed=3)
+    nt, ns = len(inp.bead_names), 16
+    _, bond = O.slice_stress(box, ids, t, x, v, 0, ns, unpbc=u, topo=O.Topo(h, tl, k, l0, fi, se, mo, ph))
+    got = A.bond_type_slices(bond, inp.bond_types(), nt)
+    names = inp.bead_names
+    decl = {}
+    for b, (bh, bt) in enumerate(inp.bond_types()):
+        decl[names[bh] + names[bt]] = b
+        decl[names[bt] + names[bh]] = b
+    want = np.zeros((ns, 3, 9))
+    for q in range(len(h)):
+        d = u[h[q]] - u[tl[q]]
+        L = np.linalg.norm(d)
+        F = k[q] * (l0[q] - L) * d / L
+        O.slice_add(want, inp.cells[2], decl[names[t[h[q]]] + names[t[tl[q]]]], x[h[q], 2], x[tl[q], 2], F, d)
+    assert np.abs(want).max() > 1.0
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-9)
