@@ -60,8 +60,8 @@ typedef struct dpd_params {
 #define DPD_MAX_TYPES 16
 
 #define DPD_PAIR_AUTO      0    /* by bead count, re-chosen at each commit: the LDS  */
-                                /* stream kernel (1024) below 400k beads, full shell */
-                                /* above (measured crossover on MI355X)              */
+                                /* stream kernel (1024) below 70k beads, the hit-   */
+                                /* list kernel above (measured crossover on MI355X)  */
 #define DPD_PAIR_BRICK     1    /* LDS cell-brick kernel, 256 threads per brick      */
 #define DPD_PAIR_FULLSHELL 2    /* thread-per-bead 27-cell kernel                    */
 #define DPD_PAIR_BRICK512  3    /* LDS cell-brick kernel, 512 threads per brick      */
@@ -69,7 +69,9 @@ typedef struct dpd_params {
 #define DPD_PAIR_STREAM1024 5   /* LDS cell-stream kernel, 1024 threads per brick    */
 #define DPD_PAIR_COMPACT   6    /* thread-per-bead 27-cell kernel, compacted pairs   */
 #define DPD_PAIR_CELLS     7    /* wavefront-per-cell kernel, candidates loaded once */
-#define DPD_PAIR_KINDS     7    /* kernels are numbered 1..DPD_PAIR_KINDS            */
+#define DPD_PAIR_HITLIST   8    /* thread-per-bead 27-cell scan into an LDS list of  */
+                                /* partners in range, then one dense evaluation loop */
+#define DPD_PAIR_KINDS     8    /* kernels are numbered 1..DPD_PAIR_KINDS            */
 
 /* FP64 observables of the last completed step (CMonitor::CalculateSingleBeadData,
  * src/Monitor.cpp:790-1041).  stress = (sum over pairs and bonds of r (x) F) / V,

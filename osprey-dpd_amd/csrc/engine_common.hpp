@@ -213,10 +213,10 @@ __device__ inline void wave_groups(int key, bool active, int& rank, int& size, i
 // Add per-thread FP64 values v[0..N) into acc[box * kObsStride + base + i]: one
 // block-wide reduction for the threads of the block's first box, direct atomics for
 // threads of another box (only blocks straddling a box boundary have any).
-template <int N>
+template <int N, int WAVES = 4>
 __device__ inline void accumulate_boxes(const double (&v)[N], int box, int box0, int base, double* acc)
 {
-    __shared__ double red[N][4];
+    __shared__ double red[N][WAVES];   // workgroups of up to 64 * WAVES threads
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const bool mine = box == box0;
 #pragma unroll

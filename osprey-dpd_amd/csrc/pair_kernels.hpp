@@ -1,6 +1,6 @@
 // pair_kernels.hpp -- DPD pair-force kernels (CCNTCell::UpdateForce, src/CNTCell.cpp:877-1572).
 //
-// Two kernels over the same cell-sorted SoA layout:
+// Pair kernels over the same cell-sorted SoA layout (all produce the same forces):
 //   DPD_PAIR_FULLSHELL  one thread per bead visits all 27 neighbour cells and keeps only
 //                       its own force (no scatter, no atomics).  Simple and exact; the
 //                       cross-check for the brick kernel.
@@ -14,6 +14,8 @@
 //                       and evaluated 64 at a time (pair_compact.hip).
 //   DPD_PAIR_CELLS      a wavefront per home cell: the 27-cell candidate list is loaded once
 //                       per cell and tested against each of its beads (pair_cells.hip).
+//   DPD_PAIR_HITLIST    the full-shell walk in two passes: partners in range are listed in
+//                       LDS, then evaluated in one dense loop (below).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -118,14 +120,159 @@ __global__ __launch_bounds__(256) void k_pair_fullshell(PairArgs A, BoxDev B0, i
     }
 }
 
-// The automatic choice by the number of beads the engine holds (all boxes): below
-// ~400k beads a thread per bead leaves most of the 256 CUs idle (a 10k-40k-bead box is
-// 160-640 wavefronts) and the LDS brick kernel with 1024-thread workgroups is ~1.8x faster;
-// above it the full-shell kernel is fastest (786k-bead box: 223 us against 290 us; a
-// 4M-bead batch of 256 replicas: 2.37e9 against 1.89e9 bead-steps/s).
+// Hit-list kernel: thread per bead in two passes.  The scan of the 27 neighbour cells
+// appends every partner in range to a per-thread list in LDS (slot | neighbour-cell code
+// << 26); the list is then evaluated in one dense loop.  In the one-pass kernel a
+// wavefront runs the whole pair evaluation (random stream, force) whenever any of its
+// lanes has a partner in range -- on nearly every candidate iteration, with ~8% of the
+// lanes busy -- while here the evaluation loop runs max(partners per lane) ~20 times per
+// wavefront.  Partners are evaluated in candidate order with the same arithmetic, so
+// the forces are bitwise those of k_pair_fullshell.  Slots must fit in 26 bits.
+constexpr int kHitThreads = 512;
+constexpr int kHitCap = 24;     // list entries per thread (96 B of LDS each: 48 KB per workgroup)
+constexpr int kHitFlush = 24;   // after each neighbour cell, the wavefront evaluates its
+                                // lists together once any of them holds this many
+constexpr int kHitSlotBits = 26;
+constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
+
+template <bool OBS, bool MULTI>
+__global__ __launch_bounds__(kHitThreads) void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
+{
+    __shared__ uint32_t hits[kHitCap][kHitThreads];
+    __shared__ float stab[MULTI ? 1 : kTableStride];
+    if (!MULTI) {
+        // one box: its interaction table in LDS
+        for (int q = threadIdx.x; q < kTableStride; q += kHitThreads) stab[q] = A.table[q];
+        __syncthreads();
+    }
+    const int blk = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (blk >= nblocks) return;
+    const int t = threadIdx.x;
+    const int i = blk * kHitThreads + t;
+    double vir[10];
+    if (OBS)
+        for (int q = 0; q < 10; ++q) vir[q] = 0.0;
+    int box = -1;
+    if (i < A.n) {
+        const float4 pi = A.pos[i];
+        const int wi = __float_as_int(pi.w);
+        box = bead_box(wi);
+        const BoxDev& B = MULTI ? A.boxes[box] : B0;
+        const RngKey K = rng_key(B, *A.step);
+        const float* tab = MULTI ? A.table + box * kTableStride : stab;
+        const float4 vi = A.vp[i];
+        const int ti = bead_type(wi);
+        const uint32_t idi = (uint32_t)__float_as_int(vi.w);
+        int c[3];
+        key_cell(A.key[i] - B.koff, B, c);
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        float sv[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int nh = 0;
+        // one listed partner: slot | neighbour-cell code, its position and velocity
+        auto eval = [&](uint32_t e, const float4& pj, const float4& vj) {
+            const int code = (int)(e >> kHitSlotBits);
+            const float sx = (float)(1 - (code & 3)) * B.w[0];
+            const float sy = (float)(1 - ((code >> 2) & 3)) * B.w[1];
+            const float sz = (float)(1 - (code >> 4)) * B.w[2];
+            const float ddx = (pi.x - pj.x) + sx;
+            const float ddy = (pi.y - pj.y) + sy;
+            const float ddz = (pi.z - pj.z) + sz;
+            const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const int tt = ti * kMaxTypes + bead_type(__float_as_int(pj.w));
+            const float dxdv = ddx * (vi.x - vj.x) + ddy * (vi.y - vj.y) + ddz * (vi.z - vj.z);
+            float pe;
+            const float s = pair_scalar(dr2, dxdv, tab[tt], tab[2 * kTypeStride + tt], tab[kTypeStride + tt], idi,
+                                        (uint32_t)__float_as_int(vj.w), B.invrootdt, K, OBS ? &pe : nullptr);
+            const float Fx = s * ddx, Fy = s * ddy, Fz = s * ddz;
+            fx += Fx; fy += Fy; fz += Fz;
+            if (OBS) {
+                sv[0] += ddx * Fx; sv[1] += ddy * Fx; sv[2] += ddz * Fx;
+                sv[3] += ddx * Fy; sv[4] += ddy * Fy; sv[5] += ddz * Fy;
+                sv[6] += ddx * Fz; sv[7] += ddy * Fz; sv[8] += ddz * Fz;
+                sv[9] += pe;
+            }
+        };
+        // evaluate and clear this thread's list, two entries per pass (both loads in
+        // flight, two independent random-stream chains), in list order
+        auto drain = [&]() {
+            int h = 0;
+            for (; h + 1 < nh; h += 2) {
+                const uint32_t ea = hits[h][t], eb = hits[h + 1][t];
+                const int ja = (int)(ea & kHitSlotMask), jb = (int)(eb & kHitSlotMask);
+                const float4 pa = A.pos[ja], va = A.vp[ja];
+                const float4 pb = A.pos[jb], vb = A.vp[jb];
+                eval(ea, pa, va);
+                eval(eb, pb, vb);
+            }
+            if (h < nh) {
+                const uint32_t ea = hits[h][t];
+                const int ja = (int)(ea & kHitSlotMask);
+                eval(ea, A.pos[ja], A.vp[ja]);
+            }
+            nh = 0;
+        };
+        // candidate test; the bead itself has dr2 = 0 and is never listed
+        auto test = [&](int j, const float4& pj, float sx, float sy, float sz, uint32_t code) {
+            const float ddx = (pi.x - pj.x) + sx;
+            const float ddy = (pi.y - pj.y) + sy;
+            const float ddz = (pi.z - pj.z) + sz;
+            const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (dr2 < 1.0f && dr2 > 1e-18f) {
+                if (nh == kHitCap) drain();   // overflow inside a cell: this lane alone
+                hits[nh][t] = (uint32_t)j | code;
+                ++nh;
+            }
+        };
+        for (int dz = -1; dz <= 1; ++dz) {
+            int cz = c[2] + dz;
+            cz += (cz < 0) ? B.n[2] : ((cz >= B.n[2]) ? -B.n[2] : 0);
+            for (int dy = -1; dy <= 1; ++dy) {
+                int cy = c[1] + dy;
+                cy += (cy < 0) ? B.n[1] : ((cy >= B.n[1]) ? -B.n[1] : 0);
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int cx = c[0] + dx;
+                    cx += (cx < 0) ? B.n[0] : ((cx >= B.n[0]) ? -B.n[0] : 0);
+                    const int k = B.koff + cell_key(cx, cy, cz, B);
+                    const int j0 = A.cstart[k], j1 = A.cstart[k + 1];
+                    const float sx = -(float)dx * B.w[0], sy = -(float)dy * B.w[1], sz = -(float)dz * B.w[2];
+                    const uint32_t code = (uint32_t)((dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4)) << kHitSlotBits;
+                    // four candidates per pass, all loads in flight; the extra loads of a
+                    // short pass are clamped into the cell
+                    for (int j = j0; j < j1; j += 4) {
+                        const int jb = min(j + 1, j1 - 1), jc = min(j + 2, j1 - 1), jd = min(j + 3, j1 - 1);
+                        const float4 pa = A.pos[j], pb = A.pos[jb], pc = A.pos[jc], pd = A.pos[jd];
+                        test(j, pa, sx, sy, sz, code);
+                        if (j + 1 < j1) test(j + 1, pb, sx, sy, sz, code);
+                        if (j + 2 < j1) test(j + 2, pc, sx, sy, sz, code);
+                        if (j + 3 < j1) test(j + 3, pd, sx, sy, sz, code);
+                    }
+                    if (kHitFlush < kHitCap && __ballot(nh >= kHitFlush) != 0) drain();
+                }
+            }
+        }
+        drain();
+        A.frc[i] = make_float4(fx, fy, fz, 0.f);
+        if (OBS)
+            for (int q = 0; q < 10; ++q) vir[q] = 0.5 * (double)sv[q];
+    }
+    if (OBS) {
+        const int box0 = MULTI ? bead_box(__float_as_int(A.pos[blk * kHitThreads].w)) : 0;
+        accumulate_boxes<10, kHitThreads / 64>(vir, box, box0, kObsVir, A.acc);
+    }
+}
+
+// The automatic choice by the number of beads the engine holds (all boxes).  A thread
+// per bead leaves most of the 256 CUs idle for small boxes (24k beads are 375 wavefronts),
+// where the LDS stream kernel with 1024-thread workgroups is faster (1.4x at 24k, 1.3x at
+// 41k beads); the two break even at ~66k beads, and above that the hit-list kernel is
+// fastest (192k-bead box 1.28x, 786k-bead box 1.48x, 4M-bead batch of 256 replicas 1.85x
+// the next best kernel, measured on MI355X).
+constexpr int64_t kHitListMinBeads = 70000;
+
 inline int default_pair_kernel(int64_t nbeads)
 {
-    return nbeads < 400000 ? DPD_PAIR_STREAM1024 : DPD_PAIR_FULLSHELL;
+    if (nbeads < kHitListMinBeads) return DPD_PAIR_STREAM1024;
+    return nbeads <= (int64_t)kHitSlotMask ? DPD_PAIR_HITLIST : DPD_PAIR_FULLSHELL;
 }
 
 hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s, int threads);
@@ -145,6 +292,20 @@ inline hipError_t launch_pair_kernel(int which, const PairArgs& a, const BoxDev&
     const int per_xcd = (g + 7) / 8;
     const dim3 grid(8 * per_xcd);
     const bool multi = a.nbox > 1;
+    if (which == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask) {
+        const int gh = (a.n + kHitThreads - 1) / kHitThreads;
+        const int ph = (gh + 7) / 8;
+        const dim3 gr(8 * ph), bl(kHitThreads);
+        if (obs) {
+            if (multi) hipLaunchKernelGGL((k_pair_hitlist<true, true>), gr, bl, 0, s, a, B0, gh, ph);
+            else hipLaunchKernelGGL((k_pair_hitlist<true, false>), gr, bl, 0, s, a, B0, gh, ph);
+        } else {
+            if (multi) hipLaunchKernelGGL((k_pair_hitlist<false, true>), gr, bl, 0, s, a, B0, gh, ph);
+            else hipLaunchKernelGGL((k_pair_hitlist<false, false>), gr, bl, 0, s, a, B0, gh, ph);
+        }
+        return hipGetLastError();
+    }
+    // the full-shell kernel (also for hit lists beyond 2^26 beads)
     if (obs) {
         if (multi) hipLaunchKernelGGL((k_pair_fullshell<true, true>), grid, dim3(256), 0, s, a, B0, g, per_xcd);
         else hipLaunchKernelGGL((k_pair_fullshell<true, false>), grid, dim3(256), 0, s, a, B0, g, per_xcd);

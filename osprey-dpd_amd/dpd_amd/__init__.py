@@ -14,9 +14,9 @@ LIB_PATH = os.path.join(PKG, "lib", "libospreydpd_amd.so")
 OK = 0
 ERR_NAMES = {1: "PARAM", 2: "OOM", 3: "HIP", 4: "NAN", 5: "OUT_OF_BOX", 6: "STATE", 7: "IO",
              8: "UNSUPPORTED"}
-PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL, PAIR_BRICK512, PAIR_STREAM, PAIR_STREAM1024, PAIR_COMPACT, PAIR_CELLS = \
-    0, 1, 2, 3, 4, 5, 6, 7
-PAIR_KINDS = 7
+(PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL, PAIR_BRICK512, PAIR_STREAM, PAIR_STREAM1024, PAIR_COMPACT, PAIR_CELLS,
+ PAIR_HITLIST) = range(9)
+PAIR_KINDS = 8
 STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH = 1, 2, 4, 8
 TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE = 1, 2
 

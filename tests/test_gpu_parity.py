@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 KERNELS = [pytest.param(D.PAIR_FULLSHELL, id="fullshell"), pytest.param(D.PAIR_BRICK, id="brick"),
            pytest.param(D.PAIR_STREAM, id="stream"), pytest.param(D.PAIR_STREAM1024, id="stream1024"),
-           pytest.param(D.PAIR_COMPACT, id="compact"), pytest.param(D.PAIR_CELLS, id="cells")]
+           pytest.param(D.PAIR_COMPACT, id="compact"), pytest.param(D.PAIR_CELLS, id="cells"),
+           pytest.param(D.PAIR_HITLIST, id="hitlist")]
 BINARY_A = np.array([[25.0, 80.0], [80.0, 25.0]])
 BINARY_G = np.array([[4.5, 4.5], [4.5, 4.5]])
 C4_A = np.array([[25.0, 25.0, 25.0], [25.0, 10.0, 25.0], [25.0, 25.0, 25.0]])
