@@ -287,11 +287,16 @@ def main():
     alg_bytes = BYTES_PER_BEAD_STEP * wl.n + BYTES_PER_BOND_STEP * wl.nbonds + BYTES_PER_ANGLE_STEP * wl.nangles
     t_pair = ms3[1] / max(nlaunch, 1) * 1e-3
     achieved = alg_bytes / t_pair / 1e9
+    kname = D.PAIR_KERNEL_NAMES[e.pair_kernel]
+    # HBM bytes per launch from the PMC runs summarised under profiles/, if they are of
+    # this kernel on this workload
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pair_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            tj = json.load(fh)
+        if tj.get("kernel") == kname and tj.get("workload") == wl.name:
+            traffic = tj.get("bytes_per_launch")
 
     batched = None if args.no_batched else bench_batched(pair_kernel=args.pair_kernel)
     configs = None if (args.no_batched or args.no_configs) else bench_configs()
@@ -310,7 +315,7 @@ def main():
                    "types": "W,S,P a_SS=10 a=25 gamma=4.5", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "pair", "pair_ms_per_launch": t_pair * 1e3,
+                     "kernel": kname, "pair_kernel": e.pair_kernel, "pair_ms_per_launch": t_pair * 1e3,
                      "step_gbs": alg_bytes / (wall / args.steps) / 1e9},
         "batched": batched,
         "other_configs": configs,

@@ -17,6 +17,10 @@ ERR_NAMES = {1: "PARAM", 2: "OOM", 3: "HIP", 4: "NAN", 5: "OUT_OF_BOX", 6: "STAT
 (PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL, PAIR_BRICK512, PAIR_STREAM, PAIR_STREAM1024, PAIR_COMPACT, PAIR_CELLS,
  PAIR_HITLIST) = range(9)
 PAIR_KINDS = 8
+# device kernel of each pair-kernel id (BRICK512 and STREAM1024 are workgroup sizes of two)
+PAIR_KERNEL_NAMES = {PAIR_BRICK: "k_pair_brick", PAIR_FULLSHELL: "k_pair_fullshell", PAIR_BRICK512: "k_pair_brick",
+                     PAIR_STREAM: "k_pair_stream", PAIR_STREAM1024: "k_pair_stream", PAIR_COMPACT: "k_pair_compact",
+                     PAIR_CELLS: "k_pair_cells", PAIR_HITLIST: "k_pair_hitlist"}
 STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH = 1, 2, 4, 8
 TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE = 1, 2
 
