@@ -100,6 +100,10 @@ typedef struct dpd_obs {
 /* toggles of dpd_toggle (command handlers of src/CNTCell.cpp:223-346) */
 #define DPD_TOGGLE_THERMOSTAT     1   /* ToggleDPDBeadThermostat: on=0 zeroes gamma  */
 #define DPD_TOGGLE_CONSERVATIVE   2   /* ToggleDPDBeadConservativeForces              */
+#define DPD_TOGGLE_FORCES         3   /* ToggleDPDBeadForces (src/CNTCell.cpp:223-227, */
+                                      /* src/SimState.cpp:599-618): off zeroes the     */
+                                      /* half-step constants, so no force moves a bead  */
+                                      /* or changes a velocity (ballistic motion)       */
 
 /* Create an engine on p->device.  Replaces the CCNTCell static set-up. */
 int  dpd_create(const dpd_params* p, dpd_engine** out);
@@ -179,6 +183,16 @@ int64_t dpd_get_step(const dpd_engine* e);
 int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel);
 int32_t dpd_get_pair_kernel(const dpd_engine* e);
 int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step);
+
+/* Momentum commands on the current (closed) state of box `box` (-1: every box, each on
+ * its own), executed between dpd_step calls as the reference executes commands before
+ * Evolve.  dpd_zero_total_momentum subtracts the mean velocity (CSimBox::ZeroTotalMom,
+ * src/SimBox.cpp:1919-1947); dpd_renormalise_momenta scales all velocities so that
+ * sum v^2 / (3 N) = kT (CSimBox::UpdateRenormalisedMom, src/SimBox.cpp:3177-3215, which
+ * the RenormaliseMomenta command schedules every Period steps, src/SimBox.cpp:1313-1317).
+ * Sums in FP64. */
+int dpd_zero_total_momentum(dpd_engine* e, int32_t box);
+int dpd_renormalise_momenta(dpd_engine* e, int32_t box);
 
 /* Stress profile across `nslices` equal z-slices of box `box` (replaces the per-step
  * tables of CMonitor::AddBeadStress, src/Monitor.cpp:1900-2104, and

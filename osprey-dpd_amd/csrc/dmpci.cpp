@@ -421,12 +421,14 @@ bool build_system(const InputData& d, System& s, std::string& err)
                 s.bond_tail.push_back(first_bead + p.bond_tail[b]);
                 s.bond_k.push_back(d.bonds[p.bond_type[b]].k);
                 s.bond_l0.push_back(d.bonds[p.bond_type[b]].l0);
+                s.bond_type.push_back(p.bond_type[b]);
             }
             for (size_t b = 0; b < p.pair_first.size(); ++b) {
                 s.pair_first.push_back(first_bond + p.pair_first[b]);
                 s.pair_second.push_back(first_bond + p.pair_second[b]);
                 const BondPairType& bp = d.bondpairs[p.pair_type[b]];
                 s.pair_k.push_back(bp.k);
+                s.pair_type.push_back(p.pair_type[b]);
                 // CBondPairType: degrees to radians with the reference's pi
                 s.pair_phi0.push_back(bp.phi0_deg * 3.141592653589793238 / 180.0);
             }

@@ -78,6 +78,7 @@ struct System {
     std::vector<double> bond_k, bond_l0;
     std::vector<int32_t> pair_first, pair_second;
     std::vector<double> pair_k, pair_phi0;   // radians (src/BondPairType.cpp:85)
+    std::vector<int32_t> bond_type, pair_type;   // declaration index of each bond / bond pair
     std::vector<int64_t> polymers_per_type;
     std::vector<PolymerType> types;          // expanded polymer templates
     int64_t npolymers = 0;

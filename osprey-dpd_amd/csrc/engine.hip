@@ -436,6 +436,39 @@ __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, const float4* 
     }
 }
 
+// FP64 sums per box of v and v^2 and the bead count: acc[box][0..4]
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_mom_sums(int n, Buffers S, double* __restrict__ acc)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double ob[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int box = -1;
+    if (s < n) {
+        box = MULTI ? bead_box(__float_as_int(S.pos[s].w)) : 0;
+        const float4 v = S.vel[s];
+        ob[0] = v.x; ob[1] = v.y; ob[2] = v.z;
+        ob[3] = (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z;
+        ob[4] = 1.0;
+    }
+    const int box0 = MULTI ? bead_box(__float_as_int(S.pos[blockIdx.x * blockDim.x].w)) : 0;
+    accumulate_boxes<5>(ob, box, box0, 0, acc);
+}
+
+// v <- (v - shift) * scale with adj[box] = (shift x, y, z, scale), in FP64
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_mom_apply(int n, Buffers S, const double4* __restrict__ adj)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int box = MULTI ? bead_box(__float_as_int(S.pos[s].w)) : 0;
+    const double4 a = adj[box];
+    float4 v = S.vel[s];
+    v.x = (float)(((double)v.x - a.x) * a.w);
+    v.y = (float)(((double)v.y - a.y) * a.w);
+    v.z = (float)(((double)v.z - a.z) * a.w);
+    S.vel[s] = v;
+}
+
 // state back to upload order in FP64
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_gather(int n, Buffers S, const float4* __restrict__ frc,
@@ -486,6 +519,7 @@ struct BoxHost {
     dpd_params p{};
     std::vector<double> a, g;
     bool thermostat_on = true, conservative_on = true;
+    bool forces_on = true;   // ToggleDPDBeadForces: off zeroes the half-step constants
     double dt = 0, lambda = 0, kT = 0;
     BoxDev B{};
     // staged beads and topology (box-local indices)
@@ -523,6 +557,7 @@ struct dpd_engine {
     float* table = nullptr;
     double* gather = nullptr;   // 12n doubles for downloads
     double* slice = nullptr;    // slice stress tables (pair | bond)
+    double4* madj = nullptr;    // per-box velocity shift and scale
     size_t slice_cap = 0;
     // topology
     int *bond_ptr = nullptr, *ang_ptr = nullptr;
@@ -584,9 +619,12 @@ void drop_graphs(dpd_engine* e)
 void set_step_constants(BoxHost& b)
 {
     // CCNTCell::SetTimeStepConstants / ChangeTimeStepConstants (src/CNTCell.cpp:180-203, :357-377)
+    // CCNTCell::ToggleDPDBeadForces (src/CNTCell.cpp:223-227) passes dt = 0 for the two
+    // half-step constants: forces no longer move beads or change velocities
+    const double hdt = b.forces_on ? b.dt : 0.0;
     b.B.dt = (float)b.dt;
-    b.B.hdt = (float)(0.5 * b.dt);
-    b.B.hdt2 = (float)(0.5 * b.dt * b.dt);
+    b.B.hdt = (float)(0.5 * hdt);
+    b.B.hdt2 = (float)(0.5 * hdt * hdt);
     b.B.lamdt = (float)(b.lambda * b.dt);
     b.B.invrootdt = (float)std::sqrt(24.0 * b.kT / b.dt);
 }
@@ -1094,7 +1132,7 @@ void dpd_destroy(dpd_engine* e)
                     e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
                     e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
-                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice};
+                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& ev : e->ev)
@@ -1365,6 +1403,12 @@ int dpd_set_timestep(dpd_engine* e, double dt)
 int dpd_toggle(dpd_engine* e, int32_t what, int32_t on)
 {
     if (!e) return DPD_ERR_PARAM;
+    if (what == DPD_TOGGLE_FORCES) {
+        // the step constants are kernel arguments of captured graphs: drop them
+        for (auto& b : e->boxes) { b.forces_on = on != 0; set_step_constants(b); }
+        drop_graphs(e);
+        return e->committed ? upload_boxes(e) : DPD_OK;
+    }
     for (auto& b : e->boxes) {
         if (what == DPD_TOGGLE_THERMOSTAT) b.thermostat_on = on != 0;
         else if (what == DPD_TOGGLE_CONSERVATIVE) b.conservative_on = on != 0;
@@ -1463,6 +1507,57 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
     if (chosen) *chosen = bestk;
     return dpd_set_pair_kernel(e, bestk);
 }
+
+namespace {
+
+// shared body of dpd_zero_total_momentum / dpd_renormalise_momenta
+int adjust_momenta(dpd_engine* e, int32_t box, bool renormalise)
+{
+    if (!e) return DPD_ERR_PARAM;
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    const int nbox = (int)e->boxes.size();
+    if (box < -1 || box >= nbox) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    HIPCHK(hipSetDevice(e->device));
+    const int n = (int)e->n;
+    const int nacc = nbox * kObsStride;
+    hipLaunchKernelGGL(k_zero, dim3(grid(std::max(nacc, 4), 256)), dim3(256), 0, e->stream, e->acc, nacc, e->flags, 0);
+    if (multi(e)) hipLaunchKernelGGL(k_mom_sums<true>, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->acc);
+    else hipLaunchKernelGGL(k_mom_sums<false>, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->acc);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(nacc);
+    HIPCHK(hipMemcpyAsync(h.data(), e->acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<double4> adj(nbox, make_double4(0.0, 0.0, 0.0, 1.0));
+    for (int b = 0; b < nbox; ++b) {
+        if (box >= 0 && b != box) continue;
+        const double* a = h.data() + (size_t)b * kObsStride;
+        const double cnt = a[4];
+        if (!(cnt > 0.0)) continue;
+        if (renormalise) {
+            // CSimBox::UpdateRenormalisedMom (src/SimBox.cpp:3177-3215): v / sqrt(sum v^2 / (3 N kT))
+            const double norm = std::sqrt(a[3] / (3.0 * cnt * e->boxes[b].kT));
+            if (norm > 0.0) adj[b].w = 1.0 / norm;
+        } else {
+            // CSimBox::ZeroTotalMom (src/SimBox.cpp:1919-1947): v - sum v / N
+            adj[b] = make_double4(a[0] / cnt, a[1] / cnt, a[2] / cnt, 1.0);
+        }
+    }
+    int rc;
+    if (!e->madj && (rc = dalloc(e, &e->madj, (size_t)nbox))) return rc;
+    HIPCHK(hipMemcpyAsync(e->madj, adj.data(), sizeof(double4) * nbox, hipMemcpyHostToDevice, e->stream));
+    if (multi(e)) hipLaunchKernelGGL(k_mom_apply<true>, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->madj);
+    else hipLaunchKernelGGL(k_mom_apply<false>, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, bufs(e, e->cur), e->madj);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->obs_valid = false;
+    return DPD_OK;
+}
+
+}  // namespace
+
+int dpd_zero_total_momentum(dpd_engine* e, int32_t box) { return adjust_momenta(e, box, false); }
+
+int dpd_renormalise_momenta(dpd_engine* e, int32_t box) { return adjust_momenta(e, box, true); }
 
 int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_out, double* bond_out)
 {

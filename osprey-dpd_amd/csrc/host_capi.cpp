@@ -463,7 +463,29 @@ int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* 
     dpd_run_stats st{};
     st.nbeads = n;
     size_t next_cmd = 0;
-    bool thermo_on = true, cons_on = true;
+    bool thermo_on = true, cons_on = true, forces_on = true;
+    int64_t renorm_period = 0;   // RenormaliseMomenta: 0 = off
+    // bond (pair) type of a command's name: the declared names joined, read from either end
+    // (CInputData::AddBondType / AddBondPairType, src/InputData.cpp:556-598, :626-674)
+    auto bond_type_of = [&](const std::string& name) -> int {
+        for (size_t b = 0; b < d.bonds.size(); ++b)
+            if (name == d.bonds[b].head + d.bonds[b].tail || name == d.bonds[b].tail + d.bonds[b].head) return (int)b;
+        return -1;
+    };
+    auto pair_type_of = [&](const std::string& name) -> int {
+        for (size_t b = 0; b < d.bondpairs.size(); ++b) {
+            const auto& q = d.bondpairs[b];
+            if (name == q.n1 + q.n2 + q.n3 || name == q.n3 + q.n2 + q.n1) return (int)b;
+        }
+        return -1;
+    };
+    // bond parameters as the commands leave them
+    std::vector<double> bk = s.bond_k, bl0 = s.bond_l0, pk = s.pair_k, pphi = s.pair_phi0;
+    auto upload_topology = [&]() {
+        return dpd_upload_topology(e, (int64_t)s.bond_head.size(), s.bond_head.data(), s.bond_tail.data(),
+                                   bk.data(), bl0.data(), (int64_t)s.pair_first.size(), s.pair_first.data(),
+                                   s.pair_second.data(), pk.data(), pphi.data());
+    };
     double dt = d.dt;
     std::vector<double> a = in->a, g = in->g;
     double seconds = 0.0;
@@ -475,22 +497,53 @@ int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* 
             if (c.time < t + 1) continue;
             if (c.name == "ToggleDPDBeadThermostat") { thermo_on = !thermo_on; rc = dpd_toggle(e, DPD_TOGGLE_THERMOSTAT, thermo_on); }
             else if (c.name == "ToggleDPDBeadConservativeForces") { cons_on = !cons_on; rc = dpd_toggle(e, DPD_TOGGLE_CONSERVATIVE, cons_on); }
+            else if (c.name == "ToggleDPDBeadForces") { forces_on = !forces_on; rc = dpd_toggle(e, DPD_TOGGLE_FORCES, forces_on); }
             else if (c.name == "SetTimeStepSize" && !c.args.empty()) { dt = std::atof(c.args[0].c_str()); rc = dpd_set_timestep(e, dt); }
-            else if ((c.name == "SetDPDBeadConsInt" || c.name == "SetDPDBeadDissInt") && c.args.size() >= 3) {
-                const int i = d.type_of(c.args[0]), j = d.type_of(c.args[1]);
-                if (i >= 0 && j >= 0) {
-                    std::vector<double>& m = c.name == "SetDPDBeadConsInt" ? a : g;
-                    m[i * d.ntypes() + j] = m[j * d.ntypes() + i] = std::atof(c.args[2].c_str());
-                    rc = dpd_set_interactions(e, a.data(), g.data());
-                }
+            else if ((c.name == "SetDPDBeadConsInt" || c.name == "SetDPDBeadDissInt" ||
+                      c.name == "SetDPDBeadConsIntByType" || c.name == "SetDPDBeadDissIntByType") && c.args.size() >= 3) {
+                // by bead names, or by type indices for the ByType forms (src/SimBox.cpp:3227-3446)
+                const bool by_type = c.name.size() > 6 && c.name.compare(c.name.size() - 6, 6, "ByType") == 0;
+                const int i = by_type ? std::atoi(c.args[0].c_str()) : d.type_of(c.args[0]);
+                const int j = by_type ? std::atoi(c.args[1].c_str()) : d.type_of(c.args[1]);
+                if (i < 0 || j < 0 || i >= d.ntypes() || j >= d.ntypes()) continue;   // the reference logs a failed command
+                std::vector<double>& m = c.name.compare(0, 17, "SetDPDBeadConsInt") == 0 ? a : g;
+                m[i * d.ntypes() + j] = m[j * d.ntypes() + i] = std::atof(c.args[2].c_str());
+                rc = dpd_set_interactions(e, a.data(), g.data());
+            } else if (c.name == "SetBondStrength" && c.args.size() >= 3) {
+                // CSimBox::SetBondStrength (src/SimBox.cpp:3505-3545): k and l0 of every bond of the type
+                const int bt = bond_type_of(c.args[0]);
+                if (bt < 0) continue;
+                const double k = std::atof(c.args[1].c_str()), l0 = std::atof(c.args[2].c_str());
+                for (size_t q = 0; q < s.bond_type.size(); ++q)
+                    if (s.bond_type[q] == bt) { bk[q] = k; bl0[q] = l0; }
+                rc = upload_topology();
+            } else if (c.name == "SetBondStiffness" && c.args.size() >= 3) {
+                // CSimBox::SetBondStiffness (src/SimBox.cpp:3447-3503); the angle is read in degrees
+                const int pt = pair_type_of(c.args[0]);
+                if (pt < 0) continue;
+                const double k = std::atof(c.args[1].c_str());
+                const double phi0 = std::atof(c.args[2].c_str()) * 3.141592653589793238 / 180.0;
+                for (size_t q = 0; q < s.pair_type.size(); ++q)
+                    if (s.pair_type[q] == pt) { pk[q] = k; pphi[q] = phi0; }
+                rc = upload_topology();
+            } else if (c.name == "ZeroTotalMom") {
+                rc = dpd_zero_total_momentum(e, -1);
+            } else if (c.name == "RenormaliseMomenta" && c.args.size() >= 2) {
+                // ccRenormaliseMomenta: Period, flag (src/ccRenormaliseMomenta.cpp:107)
+                renorm_period = std::atoi(c.args[1].c_str()) ? std::max<int64_t>(1, std::atoll(c.args[0].c_str())) : 0;
             } else {
                 continue;   // commands outside the step engine are not executed
             }
             if (rc) return fail(rc);
             st.commands_applied++;
         }
+        // CSimBox::Run renormalises before Evolve when m_SimTime % Period == 0
+        // (src/SimBox.cpp:1313-1317); m_SimTime of the coming step is t + 1
+        if (renorm_period > 0 && (t + 1) % renorm_period == 0)
+            if ((rc = dpd_renormalise_momenta(e, -1))) return fail(rc);
         int64_t stop = d.total_time;
         if (next_cmd < d.commands.size()) stop = std::min(stop, d.commands[next_cmd].time - 1);
+        if (renorm_period > 0) stop = std::min(stop, ((t + 1) / renorm_period + 1) * renorm_period - 1);
         stop = std::min(stop, (t / d.sample_period + 1) * d.sample_period);
         stop = std::min(stop, (t / d.restart_period + 1) * d.restart_period);
         if (stop <= t) stop = t + 1;

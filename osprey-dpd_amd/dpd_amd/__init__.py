@@ -22,7 +22,7 @@ PAIR_KERNEL_NAMES = {PAIR_BRICK: "k_pair_brick", PAIR_FULLSHELL: "k_pair_fullshe
                      PAIR_STREAM: "k_pair_stream", PAIR_STREAM1024: "k_pair_stream", PAIR_COMPACT: "k_pair_compact",
                      PAIR_CELLS: "k_pair_cells", PAIR_HITLIST: "k_pair_hitlist"}
 STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH = 1, 2, 4, 8
-TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE = 1, 2
+TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE, TOGGLE_FORCES = 1, 2, 3
 
 
 class EngineMissing(RuntimeError):
@@ -81,6 +81,8 @@ SIGNATURES = {
     "dpd_download_box": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     "dpd_set_box_interactions": (C.c_int, [_P, C.c_int32, _P, _P]),
     "dpd_slice_stress": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P]),
+    "dpd_zero_total_momentum": (C.c_int, [_P, C.c_int32]),
+    "dpd_renormalise_momenta": (C.c_int, [_P, C.c_int32]),
 }
 
 _lib = None
@@ -236,6 +238,12 @@ class Engine:
         ms = np.zeros(PAIR_KINDS)
         self._check(lib().dpd_autotune(self.h, int(nsteps), C.byref(chosen), _ptr(ms)))
         return chosen.value, {k + 1: float(ms[k]) for k in range(PAIR_KINDS)}
+
+    def zero_total_momentum(self, box=-1):
+        self._check(lib().dpd_zero_total_momentum(self.h, int(box)))
+
+    def renormalise_momenta(self, box=-1):
+        self._check(lib().dpd_renormalise_momenta(self.h, int(box)))
 
     def _ntypes(self, box):
         return self._a.shape[0]

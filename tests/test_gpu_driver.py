@@ -108,3 +108,83 @@ def test_command_line_front_end(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "run cli: 200 steps, 1536 beads" in r.stdout
     assert os.path.exists(os.path.join(tmp_path, "dmpcrs.cli.con.200.dat"))
+
+
+def test_forces_toggle_and_momentum_commands(tmp_path):
+    """ZeroTotalMom, RenormaliseMomenta (every 40 steps, before the step) and
+    ToggleDPDBeadForces: once the forces are off the velocities only change by the
+    renormalisation, so every sample from the first renormalisation after the toggle on
+    has kT = 1 exactly."""
+    cmds = ("Command ZeroTotalMom 50\n"
+            "Command RenormaliseMomenta 60 40 1\n"
+            "Command ToggleDPDBeadForces 300\n")
+    write(tmp_path, "m", commands=cmds)
+    st = H.run(tmp_path, "m")
+    assert st["commands_applied"] == 3
+    h = history(tmp_path, "m")
+    late = h[h[:, 0] >= 320]
+    print(f"  kT after the toggle: {late[:, 1]}", flush=True)
+    assert len(late) == 5 and np.all(np.abs(late[:, 1] - 1.0) < 2e-6)
+    # with forces on, the sample after a renormalised step is near but not at kT
+    assert np.any(np.abs(h[(h[:, 0] > 60) & (h[:, 0] < 300), 1] - 1.0) > 1e-4)
+
+
+CHAINS = """dpd
+
+Title	" chains "
+Date    02/12/26
+Comment	" bond commands "
+
+State	random
+
+Bead  W
+      0.5
+      25
+      4.5
+
+Bead  A
+      0.5
+      25  25
+      4.5 4.5
+
+Bond  A A  128  0.5
+BondPair  A A A  20.0  0.0
+
+Polymer	Water    0.8   " (W) "
+Polymer	Chain    0.2   " (A A A A A A A A) "
+
+Box         8 8 8       1  1  1
+Density		3
+Temp        1
+RNGSeed		-77
+Lambda		0.5
+Step		0.02
+Time		300
+SamplePeriod     20
+AnalysisPeriod	 300
+DensityPeriod    300
+DisplayPeriod    300
+RestartPeriod    150
+Grid		1  1  1
+Command SetBondStrength 151 AA 128 0.9
+Command SetBondStiffness 151 AAA 40 60
+"""
+
+
+def test_bond_commands(tmp_path):
+    """SetBondStrength (by the bond's name, head and tail names joined) and
+    SetBondStiffness (bond pair name, angle in degrees): the bonds relax to the new
+    unstretched length."""
+    with open(os.path.join(tmp_path, "dmpci.b"), "w") as fh:
+        fh.write(CHAINS)
+    st = H.run(tmp_path, "b")
+    assert st["commands_applied"] == 2
+    inp = H.Input(text=CHAINS)
+    (h, t, k, l0), _ = inp.topology()
+    n = inp.counts["nbeads"]
+    lengths = []
+    for step in (150, 300):
+        r = H.read_restart(os.path.join(tmp_path, f"dmpcrs.b.con.{step}.dat"), n)
+        lengths.append(np.linalg.norm(r["u"][h] - r["u"][t], axis=1).mean())
+    print(f"  mean bond length {lengths[0]:.3f} -> {lengths[1]:.3f}", flush=True)
+    assert lengths[0] < 0.75 and lengths[1] > 0.8
