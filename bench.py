@@ -38,6 +38,9 @@ BYTES_PER_BEAD_STEP = 76     # FP32 x, v, f_old read + type; x, v, f written
 BYTES_PER_BOND_STEP = 8
 BYTES_PER_ANGLE_STEP = 12
 HBM_PEAK_GBS = 8000.0
+# useful FP32 flops per bead-step, reported beside the byte roofline: ~52 per interacting
+# pair x 6.1 pairs + ~9 per rejected candidate x 34 candidates
+FLOPS_PER_BEAD_STEP = 52 * 6.1 + 9 * 34
 
 CONDENSATE_DMPCI = """dpd
 
@@ -241,6 +244,45 @@ def cpu_baseline_port(wl, nsteps=3):
             "sample": f"FP64 oracle step, {wl.n} beads, {nsteps} steps after 1 warm-up step"}
 
 
+def _loaded_hip_runtime():
+    """The HIP runtime library this process already mapped (the engine links it)."""
+    import ctypes as C
+    with open("/proc/self/maps") as fh:
+        for line in fh:
+            path = line.split()[-1]
+            if "libamdhip64.so" in path:
+                return C.CDLL(path)
+    return C.CDLL("libamdhip64.so")
+
+
+def measured_copy_gbs(nbytes=1 << 30, reps=10):
+    """HBM bandwidth of a plain device-to-device copy (read + write bytes per second),
+    through the engine's own HIP runtime (a second runtime in the process, such as
+    torch's, would not see the GPU)."""
+    import ctypes as C
+    hip = _loaded_hip_runtime()
+    a, b, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(nbytes)
+    d2d = 3   # hipMemcpyDeviceToDevice
+    ok = (hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0 and
+          hip.hipMemset(a, 1, n) == 0 and hip.hipEventCreate(C.byref(e0)) == 0 and
+          hip.hipEventCreate(C.byref(e1)) == 0 and hip.hipMemcpy(b, a, n, d2d) == 0)
+    ms = C.c_float(0.0)
+    if ok:
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, n, d2d, None)
+        hip.hipEventRecord(e1, None)
+        ok = hip.hipEventSynchronize(e1) == 0 and hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+    for p in (a, b):
+        if p.value:
+            hip.hipFree(p)
+    for ev in (e0, e1):
+        if ev.value:
+            hip.hipEventDestroy(ev)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9 if ok and ms.value > 0 else None
+
+
 def copy_probe(nbytes=256 << 20):
     """One plain device copy of known size: what FETCH_SIZE / WRITE_SIZE report for it
     scales the counters of the pair kernel to bytes."""
@@ -298,6 +340,7 @@ def main():
         if tj.get("kernel") == kname and tj.get("workload") == wl.name:
             traffic = tj.get("bytes_per_launch")
 
+    copy_gbs = measured_copy_gbs()
     batched = None if args.no_batched else bench_batched(pair_kernel=args.pair_kernel)
     configs = None if (args.no_batched or args.no_configs) else bench_configs()
 
@@ -315,6 +358,9 @@ def main():
                    "types": "W,S,P a_SS=10 a=25 gamma=4.5", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "measured_copy_gbs": copy_gbs,
+                     "flops_per_bead_step": FLOPS_PER_BEAD_STEP,
+                     "step_tflops": FLOPS_PER_BEAD_STEP * value / 1e12,
                      "kernel": kname, "pair_kernel": e.pair_kernel, "pair_ms_per_launch": t_pair * 1e3,
                      "step_gbs": alg_bytes / (wall / args.steps) / 1e9},
         "batched": batched,

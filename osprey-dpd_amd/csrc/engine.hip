@@ -251,7 +251,6 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
             v.y = v.y + B.hdt * (f.y + fo.y);
             v.z = v.z + B.hdt * (f.z + fo.z);
             S.vel[s] = v;
-            frc[s] = f;
             if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
                 atomicAdd(flags + 0, 1);
         }

@@ -549,7 +549,8 @@ int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* 
         if (stop <= t) stop = t + 1;
         const bool sample = stop % d.sample_period == 0;
         const auto t0 = std::chrono::steady_clock::now();
-        if ((rc = dpd_step(e, stop - t, sample ? DPD_STEP_OBSERVE : 0))) return fail(rc);
+        // captured step graphs are replayed for repeated call lengths
+        if ((rc = dpd_step(e, stop - t, DPD_STEP_GRAPH | (sample ? DPD_STEP_OBSERVE : 0)))) return fail(rc);
         seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         t = stop;
         if (sample) {
