@@ -330,6 +330,8 @@ def main():
     t_pair = ms3[1] / max(nlaunch, 1) * 1e-3
     achieved = alg_bytes / t_pair / 1e9
     kname = D.PAIR_KERNEL_NAMES[e.pair_kernel]
+    if e.pair_kernel == D.PAIR_HITLIST and wl.nbonds + wl.nangles > 0:
+        kname = "k_pair_hitlist_bonded"   # the bonded terms run in the same launch
     # HBM bytes per launch from the PMC runs summarised under profiles/, if they are of
     # this kernel on this workload
     traffic = None

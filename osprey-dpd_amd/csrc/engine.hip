@@ -153,11 +153,13 @@ __device__ inline void bonded_gather(int q, const Topo& T, double f[3])
 // goes to fbond / fang, from where k_advance / k_finish gather each bead's sum.  OBS adds
 // the bond virial (to the head bead's box, as CBond::AddForce adds its stress to the head)
 // and the bond and angle potential energies.  Bonds and angles never cross boxes.
-template <bool OBS, bool MULTI>
-__global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict__ slot_of, Topo T,
-                                                const BoxDev* boxes, BoxDev B0, double* __restrict__ acc)
+// q = this thread's bond or angle, q0 = the workgroup's first (its box receives the
+// workgroup's observables); workgroups of up to 64 * WAVES threads
+template <bool OBS, bool MULTI, int WAVES>
+__device__ __attribute__((always_inline)) inline void bonded_body(const Buffers& S, const int* __restrict__ slot_of,
+                                                                  const Topo& T, const BoxDev* boxes, const BoxDev& B0,
+                                                                  double* __restrict__ acc, int q, int q0)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     double ob[12];
     if (OBS)
         for (int i = 0; i < 12; ++i) ob[i] = 0.0;
@@ -201,14 +203,38 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
         // slots O_VIR .. O_PEA: virial (9), pair PE (untouched here), bond PE, angle PE
         int box0 = 0;
         if (MULTI) {
-            const int q0 = blockIdx.x * blockDim.x;
             const int s0 = q0 < T.nbonds ? slot_of[T.bond_bead[q0].x] : slot_of[T.ang_bead[q0 - T.nbonds].y];
             box0 = bead_box(__float_as_int(S.pos[s0].w));
         } else {
             box = box < 0 ? -1 : 0;
         }
-        accumulate_boxes<12>(ob, box, box0, kObsVir, acc);
+        accumulate_boxes<12, WAVES>(ob, box, box0, kObsVir, acc);
     }
+}
+
+template <bool OBS, bool MULTI>
+__global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict__ slot_of, Topo T,
+                                                const BoxDev* boxes, BoxDev B0, double* __restrict__ acc)
+{
+    const int q0 = blockIdx.x * blockDim.x;
+    bonded_body<OBS, MULTI, 4>(S, slot_of, T, boxes, B0, acc, q0 + threadIdx.x, q0);
+}
+
+// The hit-list pair kernel and the bonded terms of the same step in one launch: the pair
+// workgroups come first, the bond and angle workgroups after them, so the bonded work fills
+// the CUs the pair kernel's last workgroups leave idle (the two write disjoint buffers).
+template <bool OBS, bool MULTI>
+__global__ __launch_bounds__(kHitThreads) void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd,
+                                                                     Buffers S, const int* __restrict__ slot_of,
+                                                                     Topo T)
+{
+    const int npair = 8 * per_xcd;
+    if ((int)blockIdx.x >= npair) {
+        const int q0 = ((int)blockIdx.x - npair) * kHitThreads;
+        bonded_body<OBS, MULTI, kHitThreads / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
+        return;
+    }
+    hitlist_body<OBS, MULTI>(A, B0, nblocks, per_xcd);
 }
 
 // Add each bonded bead's bond and angle forces of this step to its pair force, in place
@@ -716,8 +742,11 @@ int sort_step(dpd_engine* e)
     return DPD_OK;
 }
 
-int launch_pair(dpd_engine* e, bool obs)
+// Pair forces of this step.  With the hit-list kernel the bond and angle terms run in the
+// same launch (k_pair_hitlist_bonded); `bonded` then tells launch_bonded to add only the sums.
+int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr)
 {
+    if (bonded) *bonded = false;
     PairArgs a;
     a.n = (int)e->n;
     a.nbox = (int)e->boxes.size();
@@ -731,6 +760,23 @@ int launch_pair(dpd_engine* e, bool obs)
     a.step = e->dstep;
     a.frc = e->frc;
     a.acc = e->acc;
+    const int nq = e->nbonds + e->nangles;
+    if (bonded && nq > 0 && e->pair_kernel == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask) {
+        const int gh = (a.n + kHitThreads - 1) / kHitThreads;
+        const int ph = (gh + 7) / 8;
+        const dim3 gr(8 * ph + grid(nq, kHitThreads)), bl(kHitThreads);
+        const BoxDev B0 = e->boxes[0].B;
+        const Buffers S = bufs(e, e->cur);
+        if (obs) {
+            if (multi(e)) hipLaunchKernelGGL((k_pair_hitlist_bonded<true, true>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
+            else hipLaunchKernelGGL((k_pair_hitlist_bonded<true, false>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
+        } else {
+            if (multi(e)) hipLaunchKernelGGL((k_pair_hitlist_bonded<false, true>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
+            else hipLaunchKernelGGL((k_pair_hitlist_bonded<false, false>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
+        }
+        *bonded = true;
+        return hipGetLastError() == hipSuccess ? DPD_OK : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
+    }
     return launch_pair_kernel(e->pair_kernel, a, e->boxes[0].B, obs, e->stream) == hipSuccess
                ? DPD_OK
                : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
@@ -749,14 +795,17 @@ void launch_finish(dpd_engine* e)
                            bufs(e, e->cur), e->frc, e->dboxes, B0, e->flags, e->acc);
 }
 
-// bonds and angles of this step (before k_advance / k_finish gather them)
+// bonds and angles of this step (unless the pair launch ran them) and each bonded bead's
+// sum added to its force
 template <bool OBS>
-void launch_bonded(dpd_engine* e)
+void launch_bonded(dpd_engine* e, bool terms_done = false)
 {
     const int nq = e->nbonds + e->nangles;
     if (nq == 0) return;
     const BoxDev B0 = e->boxes[0].B;
-    if (multi(e))
+    if (terms_done)
+        ;
+    else if (multi(e))
         hipLaunchKernelGGL((k_bonded<OBS, true>), dim3(grid(nq, 256)), dim3(256), 0, e->stream, bufs(e, e->cur),
                            e->slot_of, topo(e), e->dboxes, B0, e->acc);
     else
@@ -822,10 +871,11 @@ int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, bool timing)
         rc = (t == 0) ? sort_step<kBegin>(e) : sort_step<kEndBegin>(e);
         if (rc) return rc;
         if (timing) HIPCHK(hipEventRecord(e->pev[2 * t], e->stream));
-        if ((rc = launch_pair(e, obs && t == nsteps - 1))) return rc;
+        bool terms_done = false;
+        if ((rc = launch_pair(e, obs && t == nsteps - 1, &terms_done))) return rc;
         if (timing) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
-        if (obs && t == nsteps - 1) launch_bonded<true>(e);
-        else launch_bonded<false>(e);
+        if (obs && t == nsteps - 1) launch_bonded<true>(e, terms_done);
+        else launch_bonded<false>(e, terms_done);
     }
     if (obs) launch_finish<true, true>(e);
     else launch_finish<true, false>(e);
@@ -1290,8 +1340,9 @@ int dpd_compute_forces(dpd_engine* e, int64_t step, uint32_t flags, double* f_ou
     launch_zero(e, true);
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, step);
     // the uploaded state is sorted with vp = v: evaluate at it
-    if ((rc = launch_pair(e, true))) return rc;
-    launch_bonded<true>(e);
+    bool terms_done = false;
+    if ((rc = launch_pair(e, true, &terms_done))) return rc;
+    launch_bonded<true>(e, terms_done);
     launch_finish<false, true>(e);
     HIPCHK(hipGetLastError());
     if ((rc = read_flags(e))) return rc;

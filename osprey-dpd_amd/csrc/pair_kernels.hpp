@@ -135,8 +135,11 @@ constexpr int kHitFlush = 24;   // after each neighbour cell, the wavefront eval
 constexpr int kHitSlotBits = 26;
 constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
 
+// body of the hit-list kernel for workgroup blockIdx.x (< 8 * per_xcd); also called by the
+// kernel that runs the bonded terms in extra workgroups after the pair workgroups
 template <bool OBS, bool MULTI>
-__global__ __launch_bounds__(kHitThreads) void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
+__device__ __attribute__((always_inline)) inline void hitlist_body(const PairArgs& A, const BoxDev& B0, int nblocks,
+                                                                   int per_xcd)
 {
     __shared__ uint32_t hits[kHitCap][kHitThreads];
     __shared__ float stab[MULTI ? 1 : kTableStride];
@@ -259,6 +262,12 @@ __global__ __launch_bounds__(kHitThreads) void k_pair_hitlist(PairArgs A, BoxDev
         const int box0 = MULTI ? bead_box(__float_as_int(A.pos[blk * kHitThreads].w)) : 0;
         accumulate_boxes<10, kHitThreads / 64>(vir, box, box0, kObsVir, A.acc);
     }
+}
+
+template <bool OBS, bool MULTI>
+__global__ __launch_bounds__(kHitThreads) void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
+{
+    hitlist_body<OBS, MULTI>(A, B0, nblocks, per_xcd);
 }
 
 // The automatic choice by the number of beads the engine holds (all boxes).  A thread
