@@ -223,7 +223,8 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
 // The hit-list pair kernel and the bonded terms of the same step in one launch: the pair
 // workgroups come first, the bond and angle workgroups after them, so the bonded work fills
 // the CUs the pair kernel's last workgroups leave idle (the two write disjoint buffers).
-template <bool OBS, bool MULTI>
+// (one box)
+template <bool OBS>
 __global__ __launch_bounds__(kHitThreads) void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd,
                                                                      Buffers S, const int* __restrict__ slot_of,
                                                                      Topo T)
@@ -231,10 +232,10 @@ __global__ __launch_bounds__(kHitThreads) void k_pair_hitlist_bonded(PairArgs A,
     const int npair = 8 * per_xcd;
     if ((int)blockIdx.x >= npair) {
         const int q0 = ((int)blockIdx.x - npair) * kHitThreads;
-        bonded_body<OBS, MULTI, kHitThreads / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
+        bonded_body<OBS, false, kHitThreads / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
         return;
     }
-    hitlist_body<OBS, MULTI>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, false>(A, B0, nblocks, per_xcd);
 }
 
 // Add each bonded bead's bond and angle forces of this step to its pair force, in place
@@ -583,6 +584,8 @@ struct dpd_engine {
     double* gather = nullptr;   // 12n doubles for downloads
     double* slice = nullptr;    // slice stress tables (pair | bond)
     double4* madj = nullptr;    // per-box velocity shift and scale
+    int4* hitblk = nullptr;     // hit-list workgroups of a batch (box, first slot, end slot)
+    int nhitblk = 0;
     size_t slice_cap = 0;
     // topology
     int *bond_ptr = nullptr, *ang_ptr = nullptr;
@@ -760,20 +763,20 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr)
     a.step = e->dstep;
     a.frc = e->frc;
     a.acc = e->acc;
+    a.hitblk = e->hitblk;
+    a.nhitblk = e->nhitblk;
     const int nq = e->nbonds + e->nangles;
-    if (bonded && nq > 0 && e->pair_kernel == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask) {
+    // one box only: in a batch the bonded path's per-lane box raises the fused kernel to 87
+    // VGPRs and costs more than the launch it saves (3.59e9 against 3.72e9 bead-steps/s for
+    // 256 replicas)
+    if (bonded && nq > 0 && e->pair_kernel == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask && !multi(e)) {
         const int gh = (a.n + kHitThreads - 1) / kHitThreads;
         const int ph = (gh + 7) / 8;
         const dim3 gr(8 * ph + grid(nq, kHitThreads)), bl(kHitThreads);
         const BoxDev B0 = e->boxes[0].B;
         const Buffers S = bufs(e, e->cur);
-        if (obs) {
-            if (multi(e)) hipLaunchKernelGGL((k_pair_hitlist_bonded<true, true>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
-            else hipLaunchKernelGGL((k_pair_hitlist_bonded<true, false>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
-        } else {
-            if (multi(e)) hipLaunchKernelGGL((k_pair_hitlist_bonded<false, true>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
-            else hipLaunchKernelGGL((k_pair_hitlist_bonded<false, false>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
-        }
+        if (obs) hipLaunchKernelGGL((k_pair_hitlist_bonded<true>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
+        else hipLaunchKernelGGL((k_pair_hitlist_bonded<false>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
         *bonded = true;
         return hipGetLastError() == hipSuccess ? DPD_OK : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
     }
@@ -902,6 +905,20 @@ int commit_beads(dpd_engine* e, int64_t step0)
         return fail(e, DPD_ERR_PARAM, "batch too large");
     drop_graphs(e);
     int rc;
+    // hit-list workgroups of a batch: slots of one box each (a box's beads are the slots
+    // [bead_off, bead_off + n) since the sort keys are ordered by box)
+    if (e->boxes.size() > 1) {
+        std::vector<int4> hb;
+        for (size_t b = 0; b < e->boxes.size(); ++b) {
+            const int64_t s0 = e->boxes[b].bead_off, s1 = s0 + e->boxes[b].n;
+            for (int64_t s = s0; s < s1; s += kHitThreads)
+                hb.push_back(make_int4((int)b, (int)s, (int)std::min<int64_t>(s + kHitThreads, s1), 0));
+        }
+        if ((rc = dalloc(e, &e->hitblk, hb.size()))) return rc;
+        HIPCHK(hipMemcpyAsync(e->hitblk, hb.data(), sizeof(int4) * hb.size(), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        e->nhitblk = (int)hb.size();
+    }
     if (nbr != e->nbricks) {
         e->nbricks = nbr;
         e->nkeys = nbr * kBrickCells;
@@ -1181,7 +1198,7 @@ void dpd_destroy(dpd_engine* e)
                     e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
                     e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
-                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj};
+                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj, e->hitblk};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& ev : e->ev)

@@ -37,6 +37,8 @@ struct PairArgs {
     const int64_t* step;   // simulation time of this force evaluation (device)
     float4* frc;           // out: pair force per slot
     double* acc;           // out (OBS): per box [6..14] virial sum dx_b F_a, [15] pair PE
+    const int4* hitblk;    // hit-list workgroups of a batch: (box, first slot, end slot)
+    int nhitblk;
 };
 
 constexpr int kObsVir = 6;   // virial slots in the per-box accumulator
@@ -136,33 +138,39 @@ constexpr int kHitSlotBits = 26;
 constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
 
 // body of the hit-list kernel for workgroup blockIdx.x (< 8 * per_xcd); also called by the
-// kernel that runs the bonded terms in extra workgroups after the pair workgroups
+// kernel that runs the bonded terms in extra workgroups after the pair workgroups.  With
+// several boxes each workgroup takes slots of one box only (A.hitblk[blk] = box, first
+// slot, end slot), so the box, its constants and its table are uniform: scalar loads and
+// the table in LDS, as for one box.
 template <bool OBS, bool MULTI>
 __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArgs& A, const BoxDev& B0, int nblocks,
                                                                    int per_xcd)
 {
     __shared__ uint32_t hits[kHitCap][kHitThreads];
-    __shared__ float stab[MULTI ? 1 : kTableStride];
-    if (!MULTI) {
-        // one box: its interaction table in LDS
-        for (int q = threadIdx.x; q < kTableStride; q += kHitThreads) stab[q] = A.table[q];
-        __syncthreads();
-    }
+    __shared__ float stab[kTableStride];
     const int blk = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (blk >= nblocks) return;
+    if (blk >= nblocks) return;   // whole workgroups
+    int box = 0, i0 = blk * kHitThreads, i1 = A.n;
+    if (MULTI) {
+        const int4 hb = A.hitblk[blk];
+        box = hb.x; i0 = hb.y; i1 = hb.z;
+    }
+    // the box's interaction table in LDS
+    const float* gtab = A.table + (size_t)box * kTableStride;
+    for (int q = threadIdx.x; q < kTableStride; q += kHitThreads) stab[q] = gtab[q];
+    __syncthreads();
+    const BoxDev& B = MULTI ? A.boxes[box] : B0;
     const int t = threadIdx.x;
-    const int i = blk * kHitThreads + t;
+    const int i = i0 + t;
+    const bool valid = i < i1;
     double vir[10];
     if (OBS)
         for (int q = 0; q < 10; ++q) vir[q] = 0.0;
-    int box = -1;
-    if (i < A.n) {
+    if (valid) {
         const float4 pi = A.pos[i];
         const int wi = __float_as_int(pi.w);
-        box = bead_box(wi);
-        const BoxDev& B = MULTI ? A.boxes[box] : B0;
         const RngKey K = rng_key(B, *A.step);
-        const float* tab = MULTI ? A.table + box * kTableStride : stab;
+        const float* tab = stab;
         const float4 vi = A.vp[i];
         const int ti = bead_type(wi);
         const uint32_t idi = (uint32_t)__float_as_int(vi.w);
@@ -259,8 +267,7 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             for (int q = 0; q < 10; ++q) vir[q] = 0.5 * (double)sv[q];
     }
     if (OBS) {
-        const int box0 = MULTI ? bead_box(__float_as_int(A.pos[blk * kHitThreads].w)) : 0;
-        accumulate_boxes<10, kHitThreads / 64>(vir, box, box0, kObsVir, A.acc);
+        accumulate_boxes<10, kHitThreads / 64>(vir, valid ? box : -1, box, kObsVir, A.acc);
     }
 }
 
@@ -301,8 +308,8 @@ inline hipError_t launch_pair_kernel(int which, const PairArgs& a, const BoxDev&
     const int per_xcd = (g + 7) / 8;
     const dim3 grid(8 * per_xcd);
     const bool multi = a.nbox > 1;
-    if (which == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask) {
-        const int gh = (a.n + kHitThreads - 1) / kHitThreads;
+    if (which == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask && (!multi || a.hitblk)) {
+        const int gh = multi ? a.nhitblk : (a.n + kHitThreads - 1) / kHitThreads;
         const int ph = (gh + 7) / 8;
         const dim3 gr(8 * ph), bl(kHitThreads);
         if (obs) {
