@@ -128,8 +128,9 @@ __global__ __launch_bounds__(256) void k_pair_fullshell(PairArgs A, BoxDev B0, i
 // wavefront runs the whole pair evaluation (random stream, force) whenever any of its
 // lanes has a partner in range -- on nearly every candidate iteration, with ~8% of the
 // lanes busy -- while here the evaluation loop runs max(partners per lane) ~20 times per
-// wavefront.  Partners are evaluated in candidate order with the same arithmetic, so
-// the forces are bitwise those of k_pair_fullshell.  Slots must fit in 26 bits.
+// wavefront.  Partners are evaluated in candidate order with the same expressions as
+// k_pair_fullshell; the compiler contracts multiply-adds differently in the two code
+// shapes, so the forces agree to ~1 ulp per term, not bitwise.  Slots must fit in 26 bits.
 constexpr int kHitThreads = 512;
 constexpr int kHitCap = 24;     // list entries per thread (96 B of LDS each: 48 KB per workgroup)
 constexpr int kHitFlush = 24;   // after each neighbour cell, the wavefront evaluates its

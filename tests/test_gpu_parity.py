@@ -204,3 +204,33 @@ def test_momentum_conserved(kernel):
     img = (U - X) / L
     assert np.abs(img - np.round(img)).max() < 1e-6
     assert (X >= 0).all() and (X < L).all()
+
+
+def test_hitlist_overflow_matches_fullshell():
+    """A dense cluster (~90 partners per bead inside it, several times the 24-entry
+    per-thread partner list, so lists are evaluated and cleared mid-walk): the hit-list
+    kernel matches the oracle and the full-shell kernel, which evaluates the same partners in
+    the same order (to rounding: the two kernels' multiply-adds are contracted differently)."""
+    cells = (8, 8, 8)
+    rng = np.random.default_rng(21)
+    n0 = 3 * 8 ** 3
+    xs = [rng.uniform(0, 8, (n0, 3)), 3.0 + 2.0 * rng.uniform(0, 1, (300, 3))]
+    x = np.concatenate(xs)
+    n = len(x)
+    v = rng.normal(size=(n, 3))
+    t = rng.integers(0, 2, n).astype(np.int32)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    out = {}
+    for k in (D.PAIR_FULLSHELL, D.PAIR_HITLIST):
+        e = engine(cells, BINARY_A, BINARY_G, k)
+        e.upload(ids, t, x, v)
+        assert e.pair_kernel == k
+        out[k] = e.compute_forces(step=3)
+    fr, virr, _ = O.forces(O.Box(cells, BINARY_A, BINARY_G, seed=1234), ids, t, x, v, step=3)
+    norm, _ = force_errors(out[D.PAIR_HITLIST][0], fr)
+    say(f"  dense cluster: {n} beads, normwise vs oracle {norm:.2e}")
+    assert norm < FORCE_TOL
+    fh, fs = out[D.PAIR_HITLIST][0], out[D.PAIR_FULLSHELL][0]
+    say(f"  hit list vs full shell: normwise {rel_norm(fh, fs):.2e}")
+    assert rel_norm(fh, fs) < 1e-6
+    assert rel_norm(out[D.PAIR_HITLIST][1], out[D.PAIR_FULLSHELL][1]) < 1e-6
