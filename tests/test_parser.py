@@ -169,3 +169,15 @@ def test_bond_type_keying_of_the_slice_stress():
         O.slice_add(want, inp.cells[2], decl[names[t[h[q]]] + names[t[tl[q]]]], x[h[q], 2], x[tl[q], 2], F, d)
     assert np.abs(want).max() > 1.0
     assert np.allclose(got, want, rtol=1e-12, atol=1e-9)
+
+
+def test_run_with_missing_restart_fails_with_the_file_named(tmp_path):
+    """dpd_run reads the restart snapshot before it touches a device: a missing file is an
+    error naming the file (the reference's CRestartBuilder returns false, src/RestartBuilder.cpp:84-92)."""
+    text = open(os.path.join(GOLDEN, "lipid8", "case.dmpci")).read()
+    text = text.replace("restart RunId golden StateId 0", "restart RunId nowhere StateId 5")
+    with open(os.path.join(tmp_path, "dmpci.m"), "w") as fh:
+        fh.write(text)
+    with pytest.raises(H.InputError) as err:
+        H.run(tmp_path, "m")
+    assert "dmpcrs.nowhere.con.5.dat" in str(err.value)
