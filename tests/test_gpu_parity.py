@@ -234,3 +234,21 @@ def test_hitlist_overflow_matches_fullshell():
     say(f"  hit list vs full shell: normwise {rel_norm(fh, fs):.2e}")
     assert rel_norm(fh, fs) < 1e-6
     assert rel_norm(out[D.PAIR_HITLIST][1], out[D.PAIR_FULLSHELL][1]) < 1e-6
+
+
+@pytest.mark.parametrize("kernel", KERNELS + [pytest.param(D.PAIR_BRICK512, id="brick512")])
+def test_forces_with_wider_cells(kernel):
+    """CNT cells wider than the cutoff (Box ... 1.25 1 1.5 in a dmpci file): the
+    neighbour shifts and cell keys use each axis' width."""
+    cells, widths = (8, 9, 7), (1.25, 1.0, 1.5)
+    L = np.array(cells) * np.array(widths)
+    n = int(3 * np.prod(L))
+    ids, t, x, v = random_state(cells, n, 2, seed=44, widths=widths)
+    e = engine(cells, BINARY_A, BINARY_G, kernel, widths=widths)
+    e.upload(ids, t, x, v)
+    f, vir, _ = e.compute_forces(step=6)
+    fr, virr, _ = O.forces(O.Box(cells, BINARY_A, BINARY_G, seed=1234, widths=widths), ids, t, x, v, step=6)
+    norm, per_bead = force_errors(f, fr)
+    say(f"  widths {widths}: normwise {norm:.2e} per-bead {per_bead:.2e}")
+    assert norm < FORCE_TOL
+    assert rel_norm(vir, virr) < FORCE_TOL
