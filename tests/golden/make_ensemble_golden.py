@@ -4,7 +4,7 @@ for the statistical-parity tests of the GPU engine (whose random force is a coun
 pair stream, so trajectories agree only in distribution).
 
 Run in the development container only:
-    make -f oracle/ref.mk -j8 && python tests/golden/make_ensemble_golden.py
+    make -f oracle/ref.mk -j8 && python tests/golden/make_ensemble_golden.py [case ...]
 
 Cases:
   water20   pure DPD water, 20^3 cells, rho 3 (24,000 beads), a = 25, gamma = 4.5, kT = 1,
@@ -14,6 +14,10 @@ Cases:
             cells, a_HT 35, a_TW 75, a_HW 35, others 25, started as a lamella
             (State lamella), 4000 steps: kT, P, density profiles along the normal and
             the mesh area per lipid from snapshots every 250 steps.
+  binary20  water/oil 50/50 at random (the binary demixing configuration on a 20^3 box,
+            24,000 beads), a_WW = a_OO = 25, a_WO = 80, 4000 steps from a random start:
+            kT and P, and the W-W and W-O g(r) of snapshots every 250 steps after step 2000
+            while the mixture demixes.
 Each case runs NRUNS times from independent starting states (RNGSeed differs per run).
 The bilayer runs start from this build's lamella states (dpd_amd.host.Input.build_state
 with the run's seed), handed to the reference as restart snapshots (State restart), so
@@ -76,6 +80,44 @@ SamplePeriod     10
 AnalysisPeriod	 1000
 DensityPeriod    1000
 DisplayPeriod    5000
+RestartPeriod    250
+Grid		1  1  1
+"""
+
+BINARY_SEEDS = [3131 + 7919 * r for r in range(NRUNS)]
+
+BINARY = """dpd
+
+Title	" binary 20 "
+Date    02/12/26
+Comment	" ensemble statistics "
+
+State	random
+
+Bead  W
+      0.5
+      25
+      4.5
+
+Bead  O
+      0.5
+      80  25
+      4.5 4.5
+
+Polymer	Water    0.5   " (W) "
+Polymer	Oil      0.5   " (O) "
+
+Box         20 20 20       1  1  1
+Density		3
+Temp        1
+RNGSeed		-{seed}
+Lambda		0.5
+Step		0.02
+Time		4000
+SamplePeriod     10
+AnalysisPeriod	 1000
+DensityPeriod    1000
+DisplayPeriod    4000
 RestartPeriod    250
 Grid		1  1  1
 """
@@ -180,6 +222,30 @@ def water_run(work, r):
     return {"kT": stats(h[:, 1]), "P": stats(h[:, 2]), "r": r_, "g": gs}
 
 
+def binary_run(work, r):
+    rid = f"o{r}"
+    run(work, rid, BINARY.format(seed=BINARY_SEEDS[r]))
+    h = history(os.path.join(work, "dmpchs." + rid))
+    h = h[h[:, 0] > 1000]
+    L = np.array([20.0, 20.0, 20.0])
+    gww, gwo = [], []
+    for t in range(2250, 4001, 250):
+        types, x = snapshot(os.path.join(work, f"dmpcrs.{rid}.con.{t}.dat"))
+        r_, g = A.rdf(x, L, types=types, pair=(0, 0))
+        gww.append(g)
+        gwo.append(A.rdf(x, L, types=types, pair=(0, 1))[1])
+    return {"kT": stats(h[:, 1]), "P": stats(h[:, 2]), "r": r_, "gww": gww, "gwo": gwo}
+
+
+def binary(runs):
+    gww, gwwse = pooled([g for r in runs for g in r["gww"]])
+    gwo, gwose = pooled([g for r in runs for g in r["gwo"]])
+    return {"runs": [{"kT": r["kT"], "P": r["P"]} for r in runs], "seeds": BINARY_SEEDS,
+            "rdf_r": runs[0]["r"].tolist(), "g_ww": gww, "g_ww_se": gwwse, "g_wo": gwo, "g_wo_se": gwose,
+            "nsnap": sum(len(r["gww"]) for r in runs), "steps": 4000, "discard": 1000,
+            "snap_from": 2250, "snap_every": 250}, BINARY.format(seed=BINARY_SEEDS[0])
+
+
 def restart_text(text, run_id):
     """The control file with its State block replaced by a restart from run_id, step 0."""
     out, skip = [], False
@@ -256,15 +322,21 @@ def bilayer(runs, refb):
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
+    wanted = set(sys.argv[1:]) or {"water20", "bilayer16", "binary20"}
     os.makedirs(OUT, exist_ok=True)
     work = tempfile.mkdtemp(prefix="dpd_ens_")
     try:
         with ThreadPoolExecutor(max_workers=8) as pool:
-            wf = [pool.submit(water_run, work, r) for r in range(NRUNS)]
-            bf = [pool.submit(bilayer_run, work, r) for r in range(NRUNS)]
-            rf = [pool.submit(bilayer_run, work, r, False) for r in range(NRUNS)]
-            cases = (("water20", water([f.result() for f in wf])),
-                     ("bilayer16", bilayer([f.result() for f in bf], [f.result() for f in rf])))
+            jobs = {}
+            if "water20" in wanted:
+                jobs["water20"] = ([pool.submit(water_run, work, r) for r in range(NRUNS)],)
+            if "bilayer16" in wanted:
+                jobs["bilayer16"] = ([pool.submit(bilayer_run, work, r) for r in range(NRUNS)],
+                                     [pool.submit(bilayer_run, work, r, False) for r in range(NRUNS)])
+            if "binary20" in wanted:
+                jobs["binary20"] = ([pool.submit(binary_run, work, r) for r in range(NRUNS)],)
+            make = {"water20": water, "bilayer16": bilayer, "binary20": binary}
+            cases = [(name, make[name](*[[f.result() for f in fs] for fs in futs])) for name, futs in jobs.items()]
         for name, (res, text) in cases:
             with open(os.path.join(OUT, name + ".json"), "w") as fh:
                 json.dump(res, fh, indent=1)

@@ -92,3 +92,24 @@ def test_bilayer_ensemble():
     nlip = ref["nlipids"]
     apl = [np.mean([A.area_per_lipid(x, types, L, 0, 1, nlip) for x in s]) for s in snaps]
     _close("area per lipid", apl, [r["apl"] for r in ref["runs"]], 0.03)
+
+
+def test_binary_ensemble():
+    """Water/oil 50/50 at a_WO = 80 from random starts: kT, P and the W-W / W-O g(r) while
+    the mixture demixes (snapshots from step 2250 on)."""
+    ref = json.load(open(os.path.join(ENS, "binary20.json")))
+    runs, snaps, types, L = _run("binary20", ref["discard"], ref["snap_from"], ref["snap_every"],
+                                 [5151 + 104729 * r for r in range(REPLICAS)])
+    _close("kT", [A.block_average(r[:, 0])[0] for r in runs], [r["kT"][0] for r in ref["runs"]], 0.005)
+    _close("P", [A.block_average(r[:, 1])[0] for r in runs], [r["P"][0] for r in ref["runs"]], 0.01)
+    r = np.array(ref["rdf_r"])
+    sel = r > 0.3
+    for pair, key in (((0, 0), "g_ww"), ((0, 1), "g_wo")):
+        gs = np.array([A.rdf(x, L, types=types, pair=pair)[1] for s in snaps for x in s])
+        g, gse = gs.mean(0), gs.std(0, ddof=1) / np.sqrt(len(gs))
+        dg = np.abs(g - np.array(ref[key]))[sel]
+        # floor: at short range g_WO is zero on both sides, with zero spread
+        z = dg / (np.hypot(gse, np.array(ref[key + "_se"]))[sel] + 0.005)
+        print(f"  {key}: max |dg| {dg.max():.4f}, mean |dg|/sigma {z.mean():.2f}, max {z.max():.2f}", flush=True)
+        assert dg.max() < 0.05
+        assert np.mean(z) < 2.0
