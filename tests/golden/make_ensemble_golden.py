@@ -18,6 +18,11 @@ Cases:
             24,000 beads), a_WW = a_OO = 25, a_WO = 80, 4000 steps from a random start:
             kT and P, and the W-W and W-O g(r) of snapshots every 250 steps after step 2000
             while the mixture demixes.
+  condensate20  the condensate composition on a 20^3 box (24,000 beads): 16-mers
+            (S P P P)x4 at 20% of the beads in water, a_SS = 10, other a = 25, Hookean bonds
+            k 128 l0 0.5, 4000 steps from this build's random states (State restart, as for
+            the bilayer): kT, P, the sticker-sticker g(r) and the mean bond length from
+            snapshots every 250 steps after step 2000.
 Each case runs NRUNS times from independent starting states (RNGSeed differs per run).
 The bilayer runs start from this build's lamella states (dpd_amd.host.Input.build_state
 with the run's seed), handed to the reference as restart snapshots (State restart), so
@@ -106,6 +111,52 @@ Bead  O
 
 Polymer	Water    0.5   " (W) "
 Polymer	Oil      0.5   " (O) "
+
+Box         20 20 20       1  1  1
+Density		3
+Temp        1
+RNGSeed		-{seed}
+Lambda		0.5
+Step		0.02
+Time		4000
+SamplePeriod     10
+AnalysisPeriod	 1000
+DensityPeriod    1000
+DisplayPeriod    4000
+RestartPeriod    250
+Grid		1  1  1
+"""
+
+CONDENSATE_SEEDS = [4545 + 7919 * r for r in range(NRUNS)]
+
+CONDENSATE = """dpd
+
+Title	" condensate 20 "
+Date    02/12/26
+Comment	" ensemble statistics "
+
+State	random
+
+Bead  W
+      0.5
+      25
+      4.5
+
+Bead  S
+      0.5
+      25  10
+      4.5 4.5
+
+Bead  P
+      0.5
+      25  25  25
+      4.5 4.5 4.5
+
+Bond    S P  128  0.5
+Bond    P P  128  0.5
+
+Polymer	Water    0.984615   " (W) "
+Polymer	Chain    0.015385   " (S P P P S P P P S P P P S P P P) "
 
 Box         20 20 20       1  1  1
 Density		3
@@ -246,6 +297,49 @@ def binary(runs):
             "snap_from": 2250, "snap_every": 250}, BINARY.format(seed=BINARY_SEEDS[0])
 
 
+def condensate_run(work, r):
+    rid = f"c{r}"
+    text = CONDENSATE.format(seed=CONDENSATE_SEEDS[r])
+    own_start(work, f"cown{r}", text, CONDENSATE_SEEDS[r])
+    run(work, rid, restart_text(text, f"cown{r}"))
+    h = history(os.path.join(work, "dmpchs." + rid))
+    h = h[h[:, 0] > 1000]
+    L = np.array([20.0, 20.0, 20.0])
+    (bh, bt, _, _), _ = H.Input(text=text).topology()
+    gss, blen = [], []
+    for t in range(2250, 4001, 250):
+        types, x = snapshot(os.path.join(work, f"dmpcrs.{rid}.con.{t}.dat"))
+        r_, g = A.rdf(x, L, types=types, pair=(1, 1))
+        gss.append(g)
+        u = unwrapped(os.path.join(work, f"dmpcrs.{rid}.con.{t}.dat"))
+        blen.append(float(np.linalg.norm(u[bh] - u[bt], axis=1).mean()))
+    return {"kT": stats(h[:, 1]), "P": stats(h[:, 2]), "r": r_, "gss": gss, "blen": blen}
+
+
+def unwrapped(path):
+    """Unwrapped positions in bead-id order (ids 1..N are the creation order the topology
+    indexes)."""
+    rows = []
+    with open(path) as fh:
+        fh.readline()
+        for line in fh:
+            t = line.split()
+            if not t or t[0] == "inclusive":
+                break
+            rows.append([float(t[2])] + [float(v) for v in t[8:11]])
+    d = np.array(rows)
+    return d[np.argsort(d[:, 0]), 1:]
+
+
+def condensate(runs):
+    gss, gsse = pooled([g for r in runs for g in r["gss"]])
+    return {"runs": [{"kT": r["kT"], "P": r["P"], "bond_length": float(np.mean(r["blen"]))} for r in runs],
+            "seeds": CONDENSATE_SEEDS, "start": "dpd_amd random builder, seed per run",
+            "rdf_r": runs[0]["r"].tolist(), "g_ss": gss, "g_ss_se": gsse,
+            "nsnap": sum(len(r["gss"]) for r in runs), "steps": 4000, "discard": 1000,
+            "snap_from": 2250, "snap_every": 250}, CONDENSATE.format(seed=CONDENSATE_SEEDS[0])
+
+
 def restart_text(text, run_id):
     """The control file with its State block replaced by a restart from run_id, step 0."""
     out, skip = [], False
@@ -322,7 +416,7 @@ def bilayer(runs, refb):
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
-    wanted = set(sys.argv[1:]) or {"water20", "bilayer16", "binary20"}
+    wanted = set(sys.argv[1:]) or {"water20", "bilayer16", "binary20", "condensate20"}
     os.makedirs(OUT, exist_ok=True)
     work = tempfile.mkdtemp(prefix="dpd_ens_")
     try:
@@ -335,7 +429,9 @@ def main():
                                      [pool.submit(bilayer_run, work, r, False) for r in range(NRUNS)])
             if "binary20" in wanted:
                 jobs["binary20"] = ([pool.submit(binary_run, work, r) for r in range(NRUNS)],)
-            make = {"water20": water, "bilayer16": bilayer, "binary20": binary}
+            if "condensate20" in wanted:
+                jobs["condensate20"] = ([pool.submit(condensate_run, work, r) for r in range(NRUNS)],)
+            make = {"water20": water, "bilayer16": bilayer, "binary20": binary, "condensate20": condensate}
             cases = [(name, make[name](*[[f.result() for f in fs] for fs in futs])) for name, futs in jobs.items()]
         for name, (res, text) in cases:
             with open(os.path.join(OUT, name + ".json"), "w") as fh:

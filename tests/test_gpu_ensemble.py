@@ -22,7 +22,7 @@ ENS = os.path.join(GOLDEN, "ensemble")
 REPLICAS = 8
 
 
-def _run(name, discard, snap_from, snap_every, seeds):
+def _run(name, discard, snap_from, snap_every, seeds, unwrapped=False):
     inp = H.Input(os.path.join(ENS, name + ".dmpci"))
     e, meta = H.replica_batch([inp] * REPLICAS, seeds, pair_kernel=0)
     steps = inp.counts["total_time"]
@@ -36,7 +36,8 @@ def _run(name, discard, snap_from, snap_every, seeds):
                 hist[b].append((o["kT"], o["pressure"]))
         if s >= snap_from and s % snap_every == 0:
             for b in range(REPLICAS):
-                snaps[b].append(e.download_box(b)[0])
+                X, U = e.download_box(b)[:2]
+                snaps[b].append((X, U) if unwrapped else X)
     assert e.observables(0)["jumps"] == 0
     runs = [np.array(h) for h in hist]
     return runs, snaps, meta[0][0], meta[0][2]
@@ -113,3 +114,26 @@ def test_binary_ensemble():
         print(f"  {key}: max |dg| {dg.max():.4f}, mean |dg|/sigma {z.mean():.2f}, max {z.max():.2f}", flush=True)
         assert dg.max() < 0.05
         assert np.mean(z) < 2.0
+
+
+def test_condensate_ensemble():
+    """The condensate composition (16-mers (S P P P)x4 at 20%, a_SS = 10, Hookean bonds) on a
+    20^3 box, both sides from this build's random states: kT, P, mean bond length and the
+    sticker-sticker g(r)."""
+    ref = json.load(open(os.path.join(ENS, "condensate20.json")))
+    seeds = list(ref["seeds"]) + [6161 + 104729 * r for r in range(REPLICAS - len(ref["seeds"]))]
+    runs, snaps, types, L = _run("condensate20", ref["discard"], ref["snap_from"], ref["snap_every"], seeds,
+                                 unwrapped=True)
+    _close("kT", [A.block_average(r[:, 0])[0] for r in runs], [r["kT"][0] for r in ref["runs"]], 0.005)
+    _close("P", [A.block_average(r[:, 1])[0] for r in runs], [r["P"][0] for r in ref["runs"]], 0.01)
+    (bh, bt, _, _), _ = H.Input(os.path.join(ENS, "condensate20.dmpci")).topology()
+    blen = [np.mean([np.linalg.norm(u[bh] - u[bt], axis=1).mean() for _, u in s]) for s in snaps]
+    _close("bond length", blen, [r["bond_length"] for r in ref["runs"]], 0.005)
+    r = np.array(ref["rdf_r"])
+    sel = r > 0.3
+    gs = np.array([A.rdf(x, L, types=types, pair=(1, 1))[1] for s in snaps for x, _ in s])
+    g, gse = gs.mean(0), gs.std(0, ddof=1) / np.sqrt(len(gs))
+    dg = np.abs(g - np.array(ref["g_ss"]))[sel]
+    z = dg / (np.hypot(gse, np.array(ref["g_ss_se"]))[sel] + 0.005)
+    print(f"  g_ss: max |dg| {dg.max():.4f}, mean |dg|/sigma {z.mean():.2f}, max {z.max():.2f}", flush=True)
+    assert np.mean(z) < 2.0
