@@ -7,7 +7,7 @@
  * products run on MI355X matrix cores in BF16 with FP32 accumulation
  * (v_mfma_f32_16x16x16_bf16); master weights, biases and the Adam state are FP32.
  * Inputs and targets are standardised with the statistics given to
- * dpd_mlp_set_scaling.  The simulation stays the judge of every prediction.
+ * dpd_mlp_set_scaling.  The simulation stays the check on every prediction.
  */
 #ifndef DPD_MLP_H
 #define DPD_MLP_H

@@ -6,7 +6,7 @@ a_HW, a_TW, a_HT, a_TT (a_HH = a_WW = 25), the lipid bead fraction and the Hooke
 stiffness.  Hundreds of replicas of uneven size advance together in one batched engine;
 the time-averaged pressure and the mesh area per lipid of each replica are the targets.
 The surrogate (dpd_amd.mlp.Surrogate, BF16 matrix cores) is trained on those runs and
-screens candidate parameter sets; the simulation then judges every accepted candidate.
+screens candidate parameter sets; the simulation then checks every accepted candidate.
 """
 import time
 

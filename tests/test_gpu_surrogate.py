@@ -1,7 +1,7 @@
 """The learned part end to end: batched replica runs on the GPU produce (pressure, area
 per lipid) for random parameter sets; the BF16 surrogate trained on 80% of them must
 predict the held-out pressure within 2%; screening a large sweep with it costs a tiny
-fraction of simulating the sweep, and the simulation judges the accepted candidates."""
+fraction of simulating the sweep, and the simulation checks the accepted candidates."""
 import time
 
 import numpy as np
@@ -55,7 +55,7 @@ def test_surrogate_predicts_held_out_pressure(dataset):
     assert errP.mean() < 0.2 * base.mean()
 
 
-def test_screening_gain_and_simulation_judges(dataset):
+def test_screening_gain_and_simulation_checks(dataset):
     params, sizes, res = dataset
     Y = np.stack([res["P"], res["apl"]], axis=1)
     ok = np.isfinite(Y).all(axis=1)
@@ -67,13 +67,13 @@ def test_screening_gain_and_simulation_judges(dataset):
     keep, pred = S.screen(net, cand, target, tol=0.02)
     t_pred = time.perf_counter() - t0
     assert len(keep) > 0
-    judge = keep[:32]
-    L = rng.integers(8, 12, size=len(judge))
-    sim = S.run_replicas(cand[judge], L, rng.integers(1, 2 ** 31, size=len(judge)), steps=3000, discard=1500)
-    err = np.abs(pred[judge, 0] - sim["P"]) / sim["P"]
-    t_sim_per = sim["wall"] / len(judge)
+    checked = keep[:32]
+    L = rng.integers(8, 12, size=len(checked))
+    sim = S.run_replicas(cand[checked], L, rng.integers(1, 2 ** 31, size=len(checked)), steps=3000, discard=1500)
+    err = np.abs(pred[checked, 0] - sim["P"]) / sim["P"]
+    t_sim_per = sim["wall"] / len(checked)
     t_pred_per = t_pred / len(cand)
-    print(f"screened {len(cand)} candidates in {t_pred * 1e3:.1f} ms, accepted {len(keep)}; judged {len(judge)}: "
+    print(f"screened {len(cand)} candidates in {t_pred * 1e3:.1f} ms, accepted {len(keep)}; checked {len(checked)}: "
           f"mean error {err.mean():.4f}, within 2%: {np.mean(err < 0.02):.2f}; cost per candidate "
           f"{t_pred_per * 1e6:.2f} us vs {t_sim_per * 1e3:.1f} ms simulated (x{t_sim_per / t_pred_per:.0f})",
           flush=True)
