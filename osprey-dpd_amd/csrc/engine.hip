@@ -866,17 +866,24 @@ int finish_obs(dpd_engine* e)
 }
 
 // the steps of one dpd_step call, enqueued on the stream (capturable)
-int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, bool timing)
+// steps whose pair launch is bracketed by events: every step, or every
+// kTimingStride-th with DPD_STEP_TIMING_SAMPLED
+constexpr int64_t kTimingStride = 4;
+inline bool timed_step(int64_t t, int timing) { return timing == 1 || (timing == 2 && t % kTimingStride == 0); }
+
+// timing: 0 none, 1 every pair launch, 2 every kTimingStride-th
+int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, int timing)
 {
     int rc;
     launch_zero(e, false);
     for (int64_t t = 0; t < nsteps; ++t) {
         rc = (t == 0) ? sort_step<kBegin>(e) : sort_step<kEndBegin>(e);
         if (rc) return rc;
-        if (timing) HIPCHK(hipEventRecord(e->pev[2 * t], e->stream));
+        const bool timed = timed_step(t, timing);
+        if (timed) HIPCHK(hipEventRecord(e->pev[2 * t], e->stream));
         bool terms_done = false;
         if ((rc = launch_pair(e, obs && t == nsteps - 1, &terms_done))) return rc;
-        if (timing) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
+        if (timed) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
         if (obs && t == nsteps - 1) launch_bonded<true>(e, terms_done);
         else launch_bonded<false>(e, terms_done);
     }
@@ -1273,6 +1280,7 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
     HIPCHK(hipSetDevice(e->device));
     const bool obs = flags & DPD_STEP_OBSERVE;
     const bool timing = flags & DPD_STEP_TIMING;
+    const int timing_mode = timing ? ((flags & DPD_STEP_TIMING_SAMPLED) ? 2 : 1) : 0;
     const bool use_graph = (flags & DPD_STEP_GRAPH) && !timing;
     int rc;
     e->jumps = 0;
@@ -1299,7 +1307,7 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
                 const int cur0 = e->cur;
                 hipGraph_t g;
                 HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-                rc = enqueue_steps(e, chunk, last_obs, false);
+                rc = enqueue_steps(e, chunk, last_obs, 0);
                 hipError_t st = hipStreamEndCapture(e->stream, &g);
                 if (rc) return rc;
                 if (st != hipSuccess) return fail(e, DPD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(st));
@@ -1315,11 +1323,11 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
             e->step += chunk;
         }
         if (done < nsteps) {
-            if ((rc = enqueue_steps(e, nsteps - done, obs, false))) return rc;
+            if ((rc = enqueue_steps(e, nsteps - done, obs, 0))) return rc;
             e->step += nsteps - done;
         }
     } else {
-        if ((rc = enqueue_steps(e, nsteps, obs, timing))) return rc;
+        if ((rc = enqueue_steps(e, nsteps, obs, timing_mode))) return rc;
         e->step += nsteps;
     }
     HIPCHK(hipEventRecord(e->ev[1], e->stream));
@@ -1331,12 +1339,15 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
     e->n_pair = 0;
     if (timing) {
         double tp = 0.0;
+        int64_t np = 0;
         for (int64_t t = 0; t < nsteps; ++t) {
+            if (!timed_step(t, timing_mode)) continue;
             HIPCHK(hipEventElapsedTime(&ms, e->pev[2 * t], e->pev[2 * t + 1]));
             tp += ms;
+            ++np;
         }
         e->t_pair = tp;
-        e->n_pair = nsteps;
+        e->n_pair = np;
     }
     if (obs) return finish_obs(e);
     e->obs_valid = false;

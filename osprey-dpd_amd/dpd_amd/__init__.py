@@ -21,7 +21,7 @@ PAIR_KINDS = 8
 PAIR_KERNEL_NAMES = {PAIR_BRICK: "k_pair_brick", PAIR_FULLSHELL: "k_pair_fullshell", PAIR_BRICK512: "k_pair_brick",
                      PAIR_STREAM: "k_pair_stream", PAIR_STREAM1024: "k_pair_stream", PAIR_COMPACT: "k_pair_compact",
                      PAIR_CELLS: "k_pair_cells", PAIR_HITLIST: "k_pair_hitlist"}
-STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH = 1, 2, 4, 8
+STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH, STEP_TIMING_SAMPLED = 1, 2, 4, 8, 16
 TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE, TOGGLE_FORCES = 1, 2, 3
 
 
@@ -179,8 +179,9 @@ class Engine:
                                               _ptr(modulus), _ptr(phi0)))
 
     def step(self, nsteps=1, observe=False, timing=False, graph=False):
+        """timing: False, True (every pair launch timed) or "sampled" (every 4th)."""
         flags = (STEP_OBSERVE if observe else 0) | (STEP_TIMING if timing else 0) | \
-            (STEP_GRAPH if graph else 0)
+            (STEP_TIMING_SAMPLED if timing == "sampled" else 0) | (STEP_GRAPH if graph else 0)
         self._check(lib().dpd_step(self.h, int(nsteps), flags))
 
     def compute_forces(self, step, thermostat=True):

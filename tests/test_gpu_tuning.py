@@ -59,3 +59,19 @@ def test_autotune_restores_state_and_keeps_parity():
     norm, _ = force_errors(F, s.state()[3])
     say(f"  force normwise after autotune {norm:.2e}")
     assert norm < 1e-5
+
+
+def test_pair_timing_every_launch_or_sampled():
+    cells = (12, 12, 12)
+    ids, t, x, v = random_state(cells, 5184, 2, seed=10)
+    e = D.Engine(cells, A2, G2, seed=4, pair_kernel=D.PAIR_HITLIST)
+    e.upload(ids, t, x, v)
+    e.step(5)
+    e.step(20, timing=True)
+    n, ms = e.last_timing()
+    assert n == 20 and 0 < ms[1] < ms[0] and abs(ms[2] - (ms[0] - ms[1])) < 1e-9
+    e.step(22, timing="sampled")
+    n2, ms2 = e.last_timing()
+    assert n2 == 6                                  # steps 0, 4, ..., 20 of the call
+    say(f"  pair per launch: all {ms[1] / n * 1e3:.1f} us, sampled {ms2[1] / n2 * 1e3:.1f} us")
+    assert 0.5 < (ms2[1] / n2) / (ms[1] / n) < 2.0
