@@ -156,11 +156,17 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         const int4 hb = A.hitblk[blk];
         box = hb.x; i0 = hb.y; i1 = hb.z;
     }
-    // the box's interaction table in LDS
+    const BoxDev& B = MULTI ? A.boxes[box] : B0;
+    // the box's interaction table in LDS, and the shift of each neighbour-cell code
+    __shared__ float4 shtab[64];
     const float* gtab = A.table + (size_t)box * kTableStride;
     for (int q = threadIdx.x; q < kTableStride; q += kHitThreads) stab[q] = gtab[q];
+    if (threadIdx.x < 64) {
+        const int q = threadIdx.x;
+        shtab[q] = make_float4((float)(1 - (q & 3)) * B.w[0], (float)(1 - ((q >> 2) & 3)) * B.w[1],
+                               (float)(1 - (q >> 4)) * B.w[2], 0.f);
+    }
     __syncthreads();
-    const BoxDev& B = MULTI ? A.boxes[box] : B0;
     const int t = threadIdx.x;
     const int i = i0 + t;
     const bool valid = i < i1;
@@ -182,10 +188,8 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         int nh = 0;
         // one listed partner: slot | neighbour-cell code, its position and velocity
         auto eval = [&](uint32_t e, const float4& pj, const float4& vj) {
-            const int code = (int)(e >> kHitSlotBits);
-            const float sx = (float)(1 - (code & 3)) * B.w[0];
-            const float sy = (float)(1 - ((code >> 2) & 3)) * B.w[1];
-            const float sz = (float)(1 - (code >> 4)) * B.w[2];
+            const float4 sh = shtab[e >> kHitSlotBits];
+            const float sx = sh.x, sy = sh.y, sz = sh.z;
             const float ddx = (pi.x - pj.x) + sx;
             const float ddy = (pi.y - pj.y) + sy;
             const float ddz = (pi.z - pj.z) + sz;
@@ -229,7 +233,9 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             const float ddy = (pi.y - pj.y) + sy;
             const float ddz = (pi.z - pj.z) + sz;
             const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (dr2 < 1.0f && dr2 > 1e-18f) {
+            // 1e-18 < dr2 < 1 as one unsigned compare of the bits (dr2 >= 0; NaN fails)
+            constexpr uint32_t lo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
+            if (__float_as_uint(dr2) - lo < 0x3F800000u - lo) {
                 if (nh == kHitCap) drain();   // overflow inside a cell: this lane alone
                 hits[nh][t] = (uint32_t)j | code;
                 ++nh;
