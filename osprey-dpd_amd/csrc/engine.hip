@@ -223,11 +223,11 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
 // The hit-list pair kernel and the bonded terms of the same step in one launch: the pair
 // workgroups come first, the bond and angle workgroups after them, so the bonded work fills
 // the CUs the pair kernel's last workgroups leave idle (the two write disjoint buffers).
-// (one box)
+// (one box; held to 80 VGPRs like the hit-list kernel, six wavefronts per SIMD)
 template <bool OBS>
-__global__ __launch_bounds__(kHitThreads) void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd,
-                                                                     Buffers S, const int* __restrict__ slot_of,
-                                                                     Topo T)
+__global__ __launch_bounds__(kHitThreads) __attribute__((amdgpu_waves_per_eu(OBS ? 4 : 6)))
+void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buffers S,
+                           const int* __restrict__ slot_of, Topo T)
 {
     const int npair = 8 * per_xcd;
     if ((int)blockIdx.x >= npair) {

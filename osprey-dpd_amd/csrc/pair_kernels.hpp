@@ -189,10 +189,9 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         // one listed partner: slot | neighbour-cell code, its position and velocity
         auto eval = [&](uint32_t e, const float4& pj, const float4& vj) {
             const float4 sh = shtab[e >> kHitSlotBits];
-            const float sx = sh.x, sy = sh.y, sz = sh.z;
-            const float ddx = (pi.x - pj.x) + sx;
-            const float ddy = (pi.y - pj.y) + sy;
-            const float ddz = (pi.z - pj.z) + sz;
+            const float ddx = (pi.x + sh.x) - pj.x;
+            const float ddy = (pi.y + sh.y) - pj.y;
+            const float ddz = (pi.z + sh.z) - pj.z;
             const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
             const int tt = ti * kMaxTypes + bead_type(__float_as_int(pj.w));
             const float dxdv = ddx * (vi.x - vj.x) + ddy * (vi.y - vj.y) + ddz * (vi.z - vj.z);
@@ -228,10 +227,10 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             nh = 0;
         };
         // candidate test; the bead itself has dr2 = 0 and is never listed
-        auto test = [&](int j, const float4& pj, float sx, float sy, float sz, uint32_t code) {
-            const float ddx = (pi.x - pj.x) + sx;
-            const float ddy = (pi.y - pj.y) + sy;
-            const float ddz = (pi.z - pj.z) + sz;
+        auto test = [&](int j, const float4& pj, float qx, float qy, float qz, uint32_t code) {
+            const float ddx = qx - pj.x;
+            const float ddy = qy - pj.y;
+            const float ddz = qz - pj.z;
             const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
             // 1e-18 < dr2 < 1 as one unsigned compare of the bits (dr2 >= 0; NaN fails)
             constexpr uint32_t lo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
@@ -252,17 +251,26 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                     cx += (cx < 0) ? B.n[0] : ((cx >= B.n[0]) ? -B.n[0] : 0);
                     const int k = B.koff + cell_key(cx, cy, cz, B);
                     const int j0 = A.cstart[k], j1 = A.cstart[k + 1];
-                    const float sx = -(float)dx * B.w[0], sy = -(float)dy * B.w[1], sz = -(float)dz * B.w[2];
-                    const uint32_t code = (uint32_t)((dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4)) << kHitSlotBits;
+                    // the bead's position seen from this neighbour cell, same shift table
+                    // as the evaluation so both compute identical separations
+                    const uint32_t q = (uint32_t)((dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4));
+                    const uint32_t code = q << kHitSlotBits;
+                    const float4 sh = shtab[q];
+                    const float qx = pi.x + sh.x, qy = pi.y + sh.y, qz = pi.z + sh.z;
                     // four candidates per pass, all loads in flight; the extra loads of a
-                    // short pass are clamped into the cell
+                    // short pass are clamped into the cell.  Slots are below 2^26, so the
+                    // byte offsets fit 32 bits and the loads address from the array base.
+                    const char* pbase = reinterpret_cast<const char*>(A.pos);
                     for (int j = j0; j < j1; j += 4) {
                         const int jb = min(j + 1, j1 - 1), jc = min(j + 2, j1 - 1), jd = min(j + 3, j1 - 1);
-                        const float4 pa = A.pos[j], pb = A.pos[jb], pc = A.pos[jc], pd = A.pos[jd];
-                        test(j, pa, sx, sy, sz, code);
-                        if (j + 1 < j1) test(j + 1, pb, sx, sy, sz, code);
-                        if (j + 2 < j1) test(j + 2, pc, sx, sy, sz, code);
-                        if (j + 3 < j1) test(j + 3, pd, sx, sy, sz, code);
+                        const float4 pa = *reinterpret_cast<const float4*>(pbase + ((uint32_t)j << 4));
+                        const float4 pb = *reinterpret_cast<const float4*>(pbase + ((uint32_t)jb << 4));
+                        const float4 pc = *reinterpret_cast<const float4*>(pbase + ((uint32_t)jc << 4));
+                        const float4 pd = *reinterpret_cast<const float4*>(pbase + ((uint32_t)jd << 4));
+                        test(j, pa, qx, qy, qz, code);
+                        if (j + 1 < j1) test(j + 1, pb, qx, qy, qz, code);
+                        if (j + 2 < j1) test(j + 2, pc, qx, qy, qz, code);
+                        if (j + 3 < j1) test(j + 3, pd, qx, qy, qz, code);
                     }
                     if (kHitFlush < kHitCap && __ballot(nh >= kHitFlush) != 0) drain();
                 }
@@ -278,8 +286,12 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
     }
 }
 
+// Six wavefronts per SIMD (three 512-thread workgroups per CU, which the 52 KB of LDS
+// also allows) need at most 80 VGPRs: the force-only kernel is held to that; the kernel
+// with observables runs once per batch of steps and is left unconstrained.
 template <bool OBS, bool MULTI>
-__global__ __launch_bounds__(kHitThreads) void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
+__global__ __launch_bounds__(kHitThreads) __attribute__((amdgpu_waves_per_eu(OBS ? 4 : 6)))
+void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
 {
     hitlist_body<OBS, MULTI>(A, B0, nblocks, per_xcd);
 }
