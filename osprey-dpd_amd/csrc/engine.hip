@@ -61,6 +61,7 @@ struct Buffers {
     float4* fold;  // previous force
     int4* aux;     // image xyz, bead index
     int* key;      // global cell key
+    uint32_t* code;   // scan code (scan_code) of the offset
 };
 
 template <bool MULTI>
@@ -403,6 +404,7 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     D.fold[dst] = f;
     D.aux[dst] = make_int4(m.img[0], m.img[1], m.img[2], ax.w);
     D.key[dst] = k;
+    D.code[dst] = scan_code(m.o, B);
     if (MODE == kRebin) {
         vp[dst] = v;
     } else {
@@ -574,6 +576,7 @@ struct dpd_engine {
     float4 *pos[2] = {}, *vel[2] = {}, *fold[2] = {};
     int4* aux[2] = {};
     int* key[2] = {};
+    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 3 slots past the end
     float4 *frc = nullptr, *vp = nullptr;
     int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
     int *cnt = nullptr, *bcnt = nullptr, *bstart = nullptr, *cstart = nullptr;
@@ -707,7 +710,10 @@ int upload_boxes(dpd_engine* e)
     return DPD_OK;
 }
 
-Buffers bufs(dpd_engine* e, int i) { return Buffers{e->pos[i], e->vel[i], e->fold[i], e->aux[i], e->key[i]}; }
+Buffers bufs(dpd_engine* e, int i)
+{
+    return Buffers{e->pos[i], e->vel[i], e->fold[i], e->aux[i], e->key[i], e->code[i]};
+}
 
 Topo topo(dpd_engine* e)
 {
@@ -758,6 +764,7 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr)
     a.pos = e->pos[e->cur];
     a.vp = e->vp;
     a.key = e->key[e->cur];
+    a.code = e->code[e->cur];
     a.cstart = e->cstart;
     a.table = e->table;
     a.step = e->dstep;
@@ -939,8 +946,9 @@ int commit_beads(dpd_engine* e, int64_t step0)
         for (int b = 0; b < 2; ++b)
             if ((rc = dalloc(e, &e->pos[b], n)) || (rc = dalloc(e, &e->vel[b], n)) ||
                 (rc = dalloc(e, &e->fold[b], n)) || (rc = dalloc(e, &e->aux[b], n)) ||
-                (rc = dalloc(e, &e->key[b], n)))
+                (rc = dalloc(e, &e->key[b], n)) || (rc = dalloc(e, &e->code[b], n + kCodePad)))
                 return rc;
+        for (int b = 0; b < 2; ++b) HIPCHK(hipMemsetAsync(e->code[b] + n, 0, sizeof(uint32_t) * kCodePad, e->stream));
         if ((rc = dalloc(e, &e->frc, n)) || (rc = dalloc(e, &e->vp, n)) || (rc = dalloc(e, &e->slot_of, n)) ||
             (rc = dalloc(e, &e->nkey, n)) || (rc = dalloc(e, &e->rank, n)) || (rc = dalloc(e, &e->gather, 12 * n)))
             return rc;
@@ -1202,7 +1210,7 @@ void dpd_destroy(dpd_engine* e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
     void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->aux[0], e->aux[1],
-                    e->key[0], e->key[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
+                    e->key[0], e->key[1], e->code[0], e->code[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
                     e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
                     e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj, e->hitblk};
@@ -1528,10 +1536,11 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
     float4 *sp = nullptr, *sv = nullptr, *sfo = nullptr, *sf = nullptr, *svp = nullptr;
     int4* sa = nullptr;
     int *sk = nullptr, *sso = nullptr, *scs = nullptr;
+    uint32_t* scd = nullptr;
     int rc;
     if ((rc = dalloc(e, &sp, n)) || (rc = dalloc(e, &sv, n)) || (rc = dalloc(e, &sfo, n)) || (rc = dalloc(e, &sf, n)) ||
         (rc = dalloc(e, &svp, n)) || (rc = dalloc(e, &sa, n)) || (rc = dalloc(e, &sk, n)) ||
-        (rc = dalloc(e, &sso, n)) || (rc = dalloc(e, &scs, e->nkeys + 1)))
+        (rc = dalloc(e, &sso, n)) || (rc = dalloc(e, &scs, e->nkeys + 1)) || (rc = dalloc(e, &scd, n)))
         return rc;
     const int c0 = e->cur;
     const int64_t step0 = e->step;
@@ -1546,6 +1555,7 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
             HIPCHK(hipMemcpyAsync(svp, e->vp, f4, k, e->stream));
             HIPCHK(hipMemcpyAsync(sa, e->aux[c0], sizeof(int4) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(sk, e->key[c0], sizeof(int) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(scd, e->code[c0], sizeof(uint32_t) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(sso, e->slot_of, sizeof(int) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(scs, e->cstart, sizeof(int) * (e->nkeys + 1), k, e->stream));
         } else {
@@ -1556,6 +1566,7 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
             HIPCHK(hipMemcpyAsync(e->vp, svp, f4, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->aux[c0], sa, sizeof(int4) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->key[c0], sk, sizeof(int) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->code[c0], scd, sizeof(uint32_t) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->slot_of, sso, sizeof(int) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->cstart, scs, sizeof(int) * (e->nkeys + 1), k, e->stream));
             hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, step0);
@@ -1578,7 +1589,7 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
         if (ms < best) { best = ms; bestk = v + 1; }
         if ((rc = copy(false))) break;
     }
-    void* tmp[] = {sp, sv, sfo, sf, svp, sa, sk, sso, scs};
+    void* tmp[] = {sp, sv, sfo, sf, svp, sa, sk, sso, scs, scd};
     for (void* q : tmp) (void)hipFree(q);
     if (rc) return rc;
     e->obs_valid = false;

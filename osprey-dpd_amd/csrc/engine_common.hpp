@@ -138,6 +138,21 @@ __device__ inline float pair_scalar(float dr2, float dxdv, float a, float sg, fl
     return (con + diss + rnd) * inv;
 }
 
+// Scan code of a bead: its offsets in the cell to 8 bits each, q = floor(256 o / w) in
+// byte 0 (x), 1 (y), 2 (z).  The hit-list kernel tests candidates on these 4 bytes instead
+// of the 16-byte position; the centre of code q, (q + 1/2) w / 256, is within w / 512 of
+// the offset on each axis.
+__device__ inline uint32_t scan_code(const float o[3], const BoxDev& B)
+{
+    uint32_t c = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int q = (int)(o[a] * (256.0f / B.w[a]));
+        c |= (uint32_t)min(max(q, 0), 255) << (8 * a);
+    }
+    return c;
+}
+
 // Move a bead by displacement d from offset o in cell c (image img): new offset in
 // [0, w), cell and image; `jump` when it crossed more than one cell.
 struct Moved {

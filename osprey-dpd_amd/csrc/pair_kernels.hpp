@@ -32,6 +32,7 @@ struct PairArgs {
     const float4* pos;     // offset xyz, type | box << 8 (sorted)
     const float4* vp;      // predicted velocity xyz, id bits (sorted)
     const int* key;        // global cell key per slot
+    const uint32_t* code;  // scan code per slot (scan_code), kCodePad slots of padding after n
     const int* cstart;     // [nkeys+1] first slot of each global cell key
     const float* table;    // per box: a | gamma | sqrt(gamma), kTypeStride each
     const int64_t* step;   // simulation time of this force evaluation (device)
@@ -137,6 +138,13 @@ constexpr int kHitFlush = 24;   // after each neighbour cell, the wavefront eval
                                 // lists together once any of them holds this many
 constexpr int kHitSlotBits = 26;
 constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
+// slots of padding after the last scan code: the scan reads candidates four at a time
+constexpr int kCodePad = 4;
+// 1e-18 < dr2 < 1 as one unsigned compare of the bits: dr2 >= 0, NaN fails
+constexpr uint32_t kPairLo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
+__device__ inline bool pair_in_range(float dr2) { return __float_as_uint(dr2) - kPairLo < 0x3F800000u - kPairLo; }
+// four scan codes from one 4-byte-aligned 16-byte load
+typedef uint32_t code4_t __attribute__((ext_vector_type(4), aligned(4)));
 
 // body of the hit-list kernel for workgroup blockIdx.x (< 8 * per_xcd); also called by the
 // kernel that runs the bonded terms in extra workgroups after the pair workgroups.  With
@@ -196,8 +204,12 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             const int tt = ti * kMaxTypes + bead_type(__float_as_int(pj.w));
             const float dxdv = ddx * (vi.x - vj.x) + ddy * (vi.y - vj.y) + ddz * (vi.z - vj.z);
             float pe;
-            const float s = pair_scalar(dr2, dxdv, tab[tt], tab[2 * kTypeStride + tt], tab[kTypeStride + tt], idi,
-                                        (uint32_t)__float_as_int(vj.w), B.invrootdt, K, OBS ? &pe : nullptr);
+            float s = pair_scalar(dr2, dxdv, tab[tt], tab[2 * kTypeStride + tt], tab[kTypeStride + tt], idi,
+                                  (uint32_t)__float_as_int(vj.w), B.invrootdt, K, OBS ? &pe : nullptr);
+            // the scan lists a superset: the bead itself and partners just past the cut-off
+            const bool in_range = pair_in_range(dr2);
+            s = in_range ? s : 0.f;
+            if (OBS) pe = in_range ? pe : 0.f;
             const float Fx = s * ddx, Fy = s * ddy, Fz = s * ddz;
             fx += Fx; fy += Fy; fz += Fz;
             if (OBS) {
@@ -226,20 +238,25 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             }
             nh = 0;
         };
-        // candidate test; the bead itself has dr2 = 0 and is never listed
-        auto test = [&](int j, const float4& pj, float qx, float qy, float qz, uint32_t code) {
-            const float ddx = qx - pj.x;
-            const float ddy = qy - pj.y;
-            const float ddz = qz - pj.z;
+        // candidate test on the scan codes.  A decoded offset is within w/512 of the true
+        // one on each axis, so the distance is within m of the true distance and every pair
+        // inside the cut-off passes |dr| < 1 + m; the evaluation drops the rest (the bead
+        // itself among them).  Test in code units: offset = (q + 1/2) u, u = w / 256.
+        const float ux = B.w[0] * (1.0f / 256.0f), uy = B.w[1] * (1.0f / 256.0f), uz = B.w[2] * (1.0f / 256.0f);
+        const float m = sqrtf(B.w[0] * B.w[0] + B.w[1] * B.w[1] + B.w[2] * B.w[2]) * (1.01f / 512.0f) + 1e-5f;
+        const uint32_t thr = __float_as_uint((1.0f + m) * (1.0f + m));
+        auto test = [&](uint32_t jc, uint32_t cj, float qx, float qy, float qz, bool in_cell) {
+            const float ddx = qx - (float)(cj & 0xFFu) * ux;
+            const float ddy = qy - (float)((cj >> 8) & 0xFFu) * uy;
+            const float ddz = qz - (float)((cj >> 16) & 0xFFu) * uz;
             const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            // 1e-18 < dr2 < 1 as one unsigned compare of the bits (dr2 >= 0; NaN fails)
-            constexpr uint32_t lo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
-            if (__float_as_uint(dr2) - lo < 0x3F800000u - lo) {
+            if ((__float_as_uint(dr2) < thr) & in_cell) {
                 if (nh == kHitCap) drain();   // overflow inside a cell: this lane alone
-                hits[nh][t] = (uint32_t)j | code;
+                hits[nh][t] = jc;
                 ++nh;
             }
         };
+        const char* cbase = reinterpret_cast<const char*>(A.code);
         for (int dz = -1; dz <= 1; ++dz) {
             int cz = c[2] + dz;
             cz += (cz < 0) ? B.n[2] : ((cz >= B.n[2]) ? -B.n[2] : 0);
@@ -251,26 +268,25 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                     cx += (cx < 0) ? B.n[0] : ((cx >= B.n[0]) ? -B.n[0] : 0);
                     const int k = B.koff + cell_key(cx, cy, cz, B);
                     const int j0 = A.cstart[k], j1 = A.cstart[k + 1];
-                    // the bead's position seen from this neighbour cell, same shift table
-                    // as the evaluation so both compute identical separations
+                    // the bead's position seen from this neighbour cell (same shift table as
+                    // the evaluation), less half a code unit
                     const uint32_t q = (uint32_t)((dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4));
                     const uint32_t code = q << kHitSlotBits;
                     const float4 sh = shtab[q];
-                    const float qx = pi.x + sh.x, qy = pi.y + sh.y, qz = pi.z + sh.z;
-                    // four candidates per pass, all loads in flight; the extra loads of a
-                    // short pass are clamped into the cell.  Slots are below 2^26, so the
+                    const float qx = (pi.x + sh.x) - 0.5f * ux, qy = (pi.y + sh.y) - 0.5f * uy,
+                                qz = (pi.z + sh.z) - 0.5f * uz;
+                    // four consecutive candidates per pass from one load.  A short pass reads
+                    // up to three codes past the cell (the next cells' beads, or the padding
+                    // after the last slot) and lists none of them.  Slots are below 2^26, so
                     // byte offsets fit 32 bits and the loads address from the array base.
-                    const char* pbase = reinterpret_cast<const char*>(A.pos);
                     for (int j = j0; j < j1; j += 4) {
-                        const int jb = min(j + 1, j1 - 1), jc = min(j + 2, j1 - 1), jd = min(j + 3, j1 - 1);
-                        const float4 pa = *reinterpret_cast<const float4*>(pbase + ((uint32_t)j << 4));
-                        const float4 pb = *reinterpret_cast<const float4*>(pbase + ((uint32_t)jb << 4));
-                        const float4 pc = *reinterpret_cast<const float4*>(pbase + ((uint32_t)jc << 4));
-                        const float4 pd = *reinterpret_cast<const float4*>(pbase + ((uint32_t)jd << 4));
-                        test(j, pa, qx, qy, qz, code);
-                        if (j + 1 < j1) test(j + 1, pb, qx, qy, qz, code);
-                        if (j + 2 < j1) test(j + 2, pc, qx, qy, qz, code);
-                        if (j + 3 < j1) test(j + 3, pd, qx, qy, qz, code);
+                        const code4_t c4 = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j << 2));
+                        const uint32_t jc = (uint32_t)j | code;
+                        const int rem = j1 - j;
+                        test(jc, c4.x, qx, qy, qz, true);
+                        test(jc + 1, c4.y, qx, qy, qz, rem > 1);
+                        test(jc + 2, c4.z, qx, qy, qz, rem > 2);
+                        test(jc + 3, c4.w, qx, qy, qz, rem > 3);
                     }
                     if (kHitFlush < kHitCap && __ballot(nh >= kHitFlush) != 0) drain();
                 }
