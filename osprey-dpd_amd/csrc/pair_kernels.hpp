@@ -143,6 +143,8 @@ constexpr int kCodePad = 4;
 // 1e-18 < dr2 < 1 as one unsigned compare of the bits: dr2 >= 0, NaN fails
 constexpr uint32_t kPairLo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
 __device__ inline bool pair_in_range(float dr2) { return __float_as_uint(dr2) - kPairLo < 0x3F800000u - kPairLo; }
+// a 32-bit pointer into LDS (a generic pointer is 64 bits)
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 // four scan codes from one 4-byte-aligned 16-byte load
 typedef uint32_t code4_t __attribute__((ext_vector_type(4), aligned(4)));
 
@@ -193,7 +195,10 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         key_cell(A.key[i] - B.koff, B, c);
         float fx = 0.f, fy = 0.f, fz = 0.f;
         float sv[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        int nh = 0;
+        // this thread's list: column t of hits, entry e at hits[e][t]; wp is the next free one
+        lds_u32* const lbase = (lds_u32*)&hits[0][t];
+        const uint32_t laddr = (uint32_t)(size_t)lbase;   // its LDS byte address
+        uint32_t wp = laddr;
         // one listed partner: slot | neighbour-cell code, its position and velocity
         auto eval = [&](uint32_t e, const float4& pj, const float4& vj) {
             const float4 sh = shtab[e >> kHitSlotBits];
@@ -222,9 +227,10 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         // evaluate and clear this thread's list, two entries per pass (both loads in
         // flight, two independent random-stream chains), in list order
         auto drain = [&]() {
+            const int nh = (int)(wp - laddr) / (4 * kHitThreads);
             int h = 0;
             for (; h + 1 < nh; h += 2) {
-                const uint32_t ea = hits[h][t], eb = hits[h + 1][t];
+                const uint32_t ea = lbase[h * kHitThreads], eb = lbase[(h + 1) * kHitThreads];
                 const int ja = (int)(ea & kHitSlotMask), jb = (int)(eb & kHitSlotMask);
                 const float4 pa = A.pos[ja], va = A.vp[ja];
                 const float4 pb = A.pos[jb], vb = A.vp[jb];
@@ -232,11 +238,11 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                 eval(eb, pb, vb);
             }
             if (h < nh) {
-                const uint32_t ea = hits[h][t];
+                const uint32_t ea = lbase[h * kHitThreads];
                 const int ja = (int)(ea & kHitSlotMask);
                 eval(ea, A.pos[ja], A.vp[ja]);
             }
-            nh = 0;
+            wp = laddr;
         };
         // candidate test on the scan codes.  A decoded offset is within w/512 of the true
         // one on each axis, so the distance is within m of the true distance and every pair
@@ -251,9 +257,8 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             const float ddz = qz - (float)((cj >> 16) & 0xFFu) * uz;
             const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if ((__float_as_uint(dr2) < thr) & in_cell) {
-                if (nh == kHitCap) drain();   // overflow inside a cell: this lane alone
-                hits[nh][t] = jc;
-                ++nh;
+                *(lds_u32*)(size_t)wp = jc;
+                wp += 4 * kHitThreads;
             }
         };
         const char* cbase = reinterpret_cast<const char*>(A.code);
@@ -283,12 +288,15 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                         const code4_t c4 = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j << 2));
                         const uint32_t jc = (uint32_t)j | code;
                         const int rem = j1 - j;
+                        // room for the four: a list past kHitCap - 4 entries is evaluated
+                        // first (overflow inside a cell: this lane alone)
+                        if (wp > laddr + 4 * (kHitCap - 4) * kHitThreads) drain();
                         test(jc, c4.x, qx, qy, qz, true);
                         test(jc + 1, c4.y, qx, qy, qz, rem > 1);
                         test(jc + 2, c4.z, qx, qy, qz, rem > 2);
                         test(jc + 3, c4.w, qx, qy, qz, rem > 3);
                     }
-                    if (kHitFlush < kHitCap && __ballot(nh >= kHitFlush) != 0) drain();
+                    if (kHitFlush < kHitCap && __ballot(wp >= laddr + 4 * kHitFlush * kHitThreads) != 0) drain();
                 }
             }
         }
