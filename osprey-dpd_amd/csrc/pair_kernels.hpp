@@ -138,8 +138,9 @@ constexpr int kHitFlush = 24;   // after each neighbour cell, the wavefront eval
                                 // lists together once any of them holds this many
 constexpr int kHitSlotBits = 26;
 constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
-// slots of padding after the last scan code: the scan reads candidates four at a time
-constexpr int kCodePad = 4;
+// slots of padding after the last scan code: the scan reads candidates four at a time,
+// one pass ahead
+constexpr int kCodePad = 8;
 // 1e-18 < dr2 < 1 as one unsigned compare of the bits: dr2 >= 0, NaN fails
 constexpr uint32_t kPairLo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
 __device__ inline bool pair_in_range(float dr2) { return __float_as_uint(dr2) - kPairLo < 0x3F800000u - kPairLo; }
@@ -280,12 +281,14 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                     const float4 sh = shtab[q];
                     const float qx = (pi.x + sh.x) - 0.5f * ux, qy = (pi.y + sh.y) - 0.5f * uy,
                                 qz = (pi.z + sh.z) - 0.5f * uz;
-                    // four consecutive candidates per pass from one load.  A short pass reads
-                    // up to three codes past the cell (the next cells' beads, or the padding
-                    // after the last slot) and lists none of them.  Slots are below 2^26, so
-                    // byte offsets fit 32 bits and the loads address from the array base.
+                    // four consecutive candidates per pass from one load, the next pass's
+                    // four loaded before these are tested.  The loads run up to seven codes
+                    // past the cell (the next cells' beads, or the padding after the last
+                    // slot); none of those is listed.  Slots are below 2^26, so byte offsets
+                    // fit 32 bits and the loads address from the array base.
+                    code4_t c4 = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j0 << 2));
                     for (int j = j0; j < j1; j += 4) {
-                        const code4_t c4 = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j << 2));
+                        const code4_t cn = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j << 2) + 16);
                         const uint32_t jc = (uint32_t)j | code;
                         const int rem = j1 - j;
                         // room for the four: a list past kHitCap - 4 entries is evaluated
@@ -295,6 +298,7 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                         test(jc + 1, c4.y, qx, qy, qz, rem > 1);
                         test(jc + 2, c4.z, qx, qy, qz, rem > 2);
                         test(jc + 3, c4.w, qx, qy, qz, rem > 3);
+                        c4 = cn;
                     }
                     if (kHitFlush < kHitCap && __ballot(wp >= laddr + 4 * kHitFlush * kHitThreads) != 0) drain();
                 }
