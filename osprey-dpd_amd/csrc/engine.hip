@@ -576,7 +576,7 @@ struct dpd_engine {
     float4 *pos[2] = {}, *vel[2] = {}, *fold[2] = {};
     int4* aux[2] = {};
     int* key[2] = {};
-    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 7 slots past the end
+    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 3 slots past the end
     float4 *frc = nullptr, *vp = nullptr;
     int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
     int *cnt = nullptr, *bcnt = nullptr, *bstart = nullptr, *cstart = nullptr;

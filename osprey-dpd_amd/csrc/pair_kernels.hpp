@@ -138,16 +138,16 @@ constexpr int kHitFlush = 24;   // after each neighbour cell, the wavefront eval
                                 // lists together once any of them holds this many
 constexpr int kHitSlotBits = 26;
 constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
-// slots of padding after the last scan code: the scan reads candidates four at a time,
-// one pass ahead
+// slots of padding after the last scan code: the scan reads candidates two at a time,
+// one pass ahead (three past the end at most)
 constexpr int kCodePad = 8;
 // 1e-18 < dr2 < 1 as one unsigned compare of the bits: dr2 >= 0, NaN fails
 constexpr uint32_t kPairLo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
 __device__ inline bool pair_in_range(float dr2) { return __float_as_uint(dr2) - kPairLo < 0x3F800000u - kPairLo; }
 // a 32-bit pointer into LDS (a generic pointer is 64 bits)
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
-// four scan codes from one 4-byte-aligned 16-byte load
-typedef uint32_t code4_t __attribute__((ext_vector_type(4), aligned(4)));
+// two scan codes from one 4-byte-aligned 8-byte load
+typedef uint32_t code2_t __attribute__((ext_vector_type(2), aligned(4)));
 
 // body of the hit-list kernel for workgroup blockIdx.x (< 8 * per_xcd); also called by the
 // kernel that runs the bonded terms in extra workgroups after the pair workgroups.  With
@@ -281,24 +281,25 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                     const float4 sh = shtab[q];
                     const float qx = (pi.x + sh.x) - 0.5f * ux, qy = (pi.y + sh.y) - 0.5f * uy,
                                 qz = (pi.z + sh.z) - 0.5f * uz;
-                    // four consecutive candidates per pass from one load, the next pass's
-                    // four loaded before these are tested.  The loads run up to seven codes
-                    // past the cell (the next cells' beads, or the padding after the last
-                    // slot); none of those is listed.  Slots are below 2^26, so byte offsets
-                    // fit 32 bits and the loads address from the array base.
-                    code4_t c4 = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j0 << 2));
-                    for (int j = j0; j < j1; j += 4) {
-                        const code4_t cn = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j << 2) + 16);
+                    // two consecutive candidates per pass from one 8-byte load, the next
+                    // pass's two loaded before these are tested.  (Four per pass waste more
+                    // slots: a wavefront runs each cell to its lanes' longest count rounded
+                    // up to the pass width, 223 candidate slots per lane against 194 for two
+                    // and 82 candidates; one per pass loses the overlap of loads.)  The loads
+                    // run up to three codes past the cell (the next cells' beads, or the
+                    // padding after the last slot); none of those is listed.  Slots are below
+                    // 2^26, so byte offsets fit 32 bits and the loads address from the array
+                    // base.
+                    code2_t c2 = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j0 << 2));
+                    for (int j = j0; j < j1; j += 2) {
+                        const code2_t cn = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j << 2) + 8);
                         const uint32_t jc = (uint32_t)j | code;
-                        const int rem = j1 - j;
-                        // room for the four: a list past kHitCap - 4 entries is evaluated
+                        // room for the two: a list past kHitCap - 2 entries is evaluated
                         // first (overflow inside a cell: this lane alone)
-                        if (wp > laddr + 4 * (kHitCap - 4) * kHitThreads) drain();
-                        test(jc, c4.x, qx, qy, qz, true);
-                        test(jc + 1, c4.y, qx, qy, qz, rem > 1);
-                        test(jc + 2, c4.z, qx, qy, qz, rem > 2);
-                        test(jc + 3, c4.w, qx, qy, qz, rem > 3);
-                        c4 = cn;
+                        if (wp > laddr + 4 * (kHitCap - 2) * kHitThreads) drain();
+                        test(jc, c2.x, qx, qy, qz, true);
+                        test(jc + 1, c2.y, qx, qy, qz, j + 1 < j1);
+                        c2 = cn;
                     }
                     if (kHitFlush < kHitCap && __ballot(wp >= laddr + 4 * kHitFlush * kHitThreads) != 0) drain();
                 }
