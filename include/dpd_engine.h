@@ -60,7 +60,7 @@ typedef struct dpd_params {
 #define DPD_MAX_TYPES 16
 
 #define DPD_PAIR_AUTO      0    /* by bead count, re-chosen at each commit: the LDS  */
-                                /* stream kernel (1024) below 70k beads, the hit-   */
+                                /* stream kernel (1024) below 56k beads, the hit-   */
                                 /* list kernel above (measured crossover on MI355X)  */
 #define DPD_PAIR_BRICK     1    /* LDS cell-brick kernel, 256 threads per brick      */
 #define DPD_PAIR_FULLSHELL 2    /* thread-per-bead 27-cell kernel                    */

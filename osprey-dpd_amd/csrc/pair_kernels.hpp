@@ -327,11 +327,12 @@ void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
 
 // The automatic choice by the number of beads the engine holds (all boxes).  A thread
 // per bead leaves most of the 256 CUs idle for small boxes (24k beads are 375 wavefronts),
-// where the LDS stream kernel with 1024-thread workgroups is faster (1.4x at 24k, 1.3x at
-// 41k beads); the two break even at ~66k beads, and above that the hit-list kernel is
-// fastest (192k-bead box 1.28x, 786k-bead box 1.48x, 4M-bead batch of 256 replicas 1.85x
-// the next best kernel, measured on MI355X).
-constexpr int64_t kHitListMinBeads = 70000;
+// where the LDS stream kernel with 1024-thread workgroups is faster (1.25x at 24k, 1.25x
+// at 41k beads); the two break even at ~57k beads, and above that the hit-list kernel is
+// fastest (1.11x at 66k, 1.13x at 98k beads; 192k-bead box 1.28x, 786k-bead box 1.48x,
+// 4M-bead batch of 256 replicas 1.85x the next best kernel before its scan took codes and
+// two candidates per pass; measured on MI355X with graph replay).
+constexpr int64_t kHitListMinBeads = 56000;
 
 inline int default_pair_kernel(int64_t nbeads)
 {
