@@ -236,6 +236,51 @@ def test_hitlist_overflow_matches_fullshell():
     assert rel_norm(out[D.PAIR_HITLIST][1], out[D.PAIR_FULLSHELL][1]) < 1e-6
 
 
+@pytest.mark.parametrize("widths", [(1.0, 1.0, 1.0), (1.25, 1.0, 1.5)])
+def test_hitlist_pairs_at_the_cutoff(widths):
+    """Isolated pairs at distances just inside and just outside the cut-off, and coincident
+    beads: the hit-list kernel tests candidates on 8-bit cell offsets with a widened bound and
+    drops the extra partners at evaluation, so every pair inside must be kept (with the force
+    of the oracle and of the full-shell kernel) and every pair outside must contribute
+    nothing.  Pair centres sit two cells apart, on cell faces as well as inside cells."""
+    cells = (12, 12, 12)
+    w = np.array(widths)
+    L = np.array(cells) * w
+    rng = np.random.default_rng(5)
+    dists = [1.0 - 1e-6, 1.0 - 1e-4, 1.0 - 3e-3, 0.999999, 1.0 + 1e-6, 1.0 + 1e-4, 1.0 + 3e-3, 0.5, 0.0]
+    xs, k = [], 0
+    for cz in range(1, 12, 3):
+        for cy in range(1, 12, 3):
+            for cx in range(1, 12, 3):
+                d = dists[k % len(dists)]
+                k += 1
+                c = (np.array([cx, cy, cz]) + (0.0 if k % 3 == 0 else rng.uniform(0.1, 0.9, 3))) * w
+                u = rng.normal(size=3)
+                u /= np.linalg.norm(u)
+                xs += [c - 0.5 * d * u, c + 0.5 * d * u]
+    x = np.mod(np.array(xs), L)
+    n = len(x)
+    v = rng.normal(size=(n, 3))
+    t = rng.integers(0, 2, n).astype(np.int32)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    out = {}
+    for kern in (D.PAIR_FULLSHELL, D.PAIR_HITLIST):
+        e = engine(cells, BINARY_A, BINARY_G, kern, widths=widths)
+        e.upload(ids, t, x, v)
+        out[kern] = e.compute_forces(step=2)
+    fr, _, _ = O.forces(O.Box(cells, BINARY_A, BINARY_G, seed=1234, widths=widths), ids, t, x, v, step=2)
+    fh = out[D.PAIR_HITLIST][0]
+    dr = np.linalg.norm(x[1::2] - x[0::2], axis=1)
+    # (pairs within 1e-5 of the cut-off may fall either side in FP32: only the norms below)
+    outside = (dr > 1.0 + 1e-5) | (dr < 1e-9)
+    inside = (dr < 1.0 - 1e-5) & (dr > 1e-9)
+    say(f"  widths {widths}: {n // 2} pairs, {int(inside.sum())} inside, {int(outside.sum())} outside or coincident")
+    assert np.all(fh[0::2][outside] == 0) and np.all(fh[1::2][outside] == 0)
+    assert np.all(np.linalg.norm(fh[0::2][inside], axis=1) > 0)
+    assert rel_norm(fh, fr) < FORCE_TOL
+    assert rel_norm(fh, out[D.PAIR_FULLSHELL][0]) < 1e-6
+
+
 @pytest.mark.parametrize("kernel", KERNELS + [pytest.param(D.PAIR_BRICK512, id="brick512")])
 def test_forces_with_wider_cells(kernel):
     """CNT cells wider than the cutoff (Box ... 1.25 1 1.5 in a dmpci file): the
