@@ -56,7 +56,7 @@ struct Topo {
 };
 
 struct Buffers {
-    float4* pos;   // offset xyz, type | box << 8
+    float4* pos;   // offset xyz, type | box << 8 | has bonds << 30
     float4* vel;   // velocity xyz, id bits
     float4* fold;  // previous force
     int4* aux;     // image xyz, bead index
@@ -239,6 +239,22 @@ void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buff
     hitlist_body<OBS, false>(A, B0, nblocks, per_xcd);
 }
 
+// The bonded flag in pos.w: cleared on every slot, then set on the bonded beads' slots.
+__global__ __launch_bounds__(256) void k_clear_bonded_flag(int n, float4* __restrict__ pos)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) pos[s].w = __int_as_float(__float_as_int(pos[s].w) & ~kBondedBit);
+}
+
+__global__ __launch_bounds__(256) void k_set_bonded_flag(int nbonded, const int* __restrict__ bonded,
+                                                         const int* __restrict__ slot_of, float4* __restrict__ pos)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbonded) return;
+    const int s = slot_of[bonded[t]];
+    pos[s].w = __int_as_float(__float_as_int(pos[s].w) | kBondedBit);
+}
+
 // Add each bonded bead's bond and angle forces of this step to its pair force, in place
 // (one thread per bead that has bonds; its sum in FP64, rounded once).
 __global__ __launch_bounds__(256) void k_bonded_sum(Topo T, const int* __restrict__ slot_of,
@@ -256,8 +272,11 @@ __global__ __launch_bounds__(256) void k_bonded_sum(Topo T, const int* __restric
 }
 
 // Close the previous step (kEndBegin) and start the next one: new cell key and rank.
+// kEndBegin also adds the previous step's bond and angle forces of bonded beads (flagged in
+// pos.w) to their pair force, in place: within a sequence of steps the bonded sum of every
+// step but the last is left to the next step's k_advance (same arithmetic as k_bonded_sum).
 template <int MODE, bool MULTI>
-__global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc,
+__global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc, Topo T,
                                                  const BoxDev* boxes,
                                                  BoxDev B0, int* __restrict__ cnt, int* __restrict__ bcnt,
                                                  int* __restrict__ nkey, int* __restrict__ rank,
@@ -273,6 +292,12 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
         float4 v = S.vel[s];
         float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
         const int4 ax = S.aux[s];
+        if (MODE == kEndBegin && (__float_as_int(p.w) & kBondedBit)) {
+            double fb[3];
+            bonded_gather(ax.w, T, fb);
+            f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
+            frc[s] = f;
+        }
         if (MODE == kEndBegin) {
             const float4 fo = S.fold[s];
             v.x = v.x + B.hdt * (f.x + fo.x);
@@ -733,7 +758,7 @@ void sort_step_t(dpd_engine* e)
     Buffers S = bufs(e, e->cur), D = bufs(e, e->cur ^ 1);
     const BoxDev B0 = e->boxes[0].B;
     hipLaunchKernelGGL((k_advance<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
-                       e->dboxes, B0, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->dstep);
+                       topo(e), e->dboxes, B0, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->dstep);
     hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, e->stream, e->nbricks, e->bcnt, e->bstart);
     hipLaunchKernelGGL(k_cell_start, dim3(grid(e->nbricks, 4)), dim3(256), 0, e->stream, e->nbricks, n, e->bstart,
                        e->cnt, e->cstart);
@@ -807,8 +832,9 @@ void launch_finish(dpd_engine* e)
 
 // bonds and angles of this step (unless the pair launch ran them) and each bonded bead's
 // sum added to its force
+// `sum` false: the bonded sum is left to the next step's k_advance (kEndBegin)
 template <bool OBS>
-void launch_bonded(dpd_engine* e, bool terms_done = false)
+void launch_bonded(dpd_engine* e, bool terms_done = false, bool sum = true)
 {
     const int nq = e->nbonds + e->nangles;
     if (nq == 0) return;
@@ -821,8 +847,9 @@ void launch_bonded(dpd_engine* e, bool terms_done = false)
     else
         hipLaunchKernelGGL((k_bonded<OBS, false>), dim3(grid(nq, 256)), dim3(256), 0, e->stream, bufs(e, e->cur),
                            e->slot_of, topo(e), e->dboxes, B0, e->acc);
-    hipLaunchKernelGGL(k_bonded_sum, dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, topo(e), e->slot_of,
-                       e->frc);
+    if (sum)
+        hipLaunchKernelGGL(k_bonded_sum, dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, topo(e),
+                           e->slot_of, e->frc);
 }
 
 // clear the observable accumulators (and, when `flags`, the error flags)
@@ -891,8 +918,9 @@ int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, int timing)
         bool terms_done = false;
         if ((rc = launch_pair(e, obs && t == nsteps - 1, &terms_done))) return rc;
         if (timed) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
+        // the bonded sum of every step but the last is added by the next step's k_advance
         if (obs && t == nsteps - 1) launch_bonded<true>(e, terms_done);
-        else launch_bonded<false>(e, terms_done);
+        else launch_bonded<false>(e, terms_done, t == nsteps - 1);
     }
     if (obs) launch_finish<true, true>(e);
     else launch_finish<true, false>(e);
@@ -1089,6 +1117,12 @@ int commit_topology(dpd_engine* e)
     e->nbonded = (int)bonded.size();
     e->has_bonds = nbonds > 0;
     e->has_angles = npairs > 0;
+    hipLaunchKernelGGL(k_clear_bonded_flag, dim3(grid(n, 256)), dim3(256), 0, e->stream, (int)n, e->pos[e->cur]);
+    if (e->nbonded > 0)
+        hipLaunchKernelGGL(k_set_bonded_flag, dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, e->nbonded,
+                           e->bonded, e->slot_of, e->pos[e->cur]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
     return DPD_OK;
 }
 

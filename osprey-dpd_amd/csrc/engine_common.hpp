@@ -81,7 +81,9 @@ __host__ __device__ inline int floordiv(int a, int n)
 
 // bead word packing: pos.w = type | box << 8
 __host__ __device__ inline int bead_type(int w) { return w & 0xFF; }
-__host__ __device__ inline int bead_box(int w) { return w >> 8; }
+__host__ __device__ inline int bead_box(int w) { return (w >> 8) & 0x3FFFFF; }
+// bit 30 of a position's w: the bead has bonds or angles (set from the topology)
+constexpr int kBondedBit = 1 << 30;
 
 // Philox4x32-10 (Salmon et al., SC'11).  Pair uniform: counter (min id, max id, step),
 // key = seed; U = (word0 >> 8) * 2^-24, exactly representable in FP32.
