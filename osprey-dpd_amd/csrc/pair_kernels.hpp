@@ -14,8 +14,9 @@
 //                       and evaluated 64 at a time (pair_compact.hip).
 //   DPD_PAIR_CELLS      a wavefront per home cell: the 27-cell candidate list is loaded once
 //                       per cell and tested against each of its beads (pair_cells.hip).
-//   DPD_PAIR_HITLIST    the full-shell walk in two passes: partners in range are listed in
-//                       LDS, then evaluated in one dense loop (below).
+//   DPD_PAIR_HITLIST    the full-shell walk in two passes: candidates are tested on 4-byte
+//                       scan codes (cell offsets to 8 bits), partners that may be in range
+//                       are listed in LDS, then evaluated exactly in one dense loop (below).
 #pragma once
 #include <hip/hip_runtime.h>
 
