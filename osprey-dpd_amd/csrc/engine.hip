@@ -59,7 +59,8 @@ struct Buffers {
     float4* pos;   // offset xyz, type | box << 8 | has bonds << 30
     float4* vel;   // velocity xyz, id bits
     float4* fold;  // previous force
-    int4* aux;     // image xyz, bead index
+    int* bead;     // bead index (upload order) of each slot
+    int4* img;     // periodic image counts xyz by bead index (not sorted; one buffer)
     int* key;      // global cell key
     uint32_t* code;   // scan code (scan_code) of the offset
 };
@@ -70,12 +71,12 @@ __device__ inline const BoxDev& box_of(const BoxDev* boxes, const BoxDev& B0, in
     return MULTI ? boxes[bead_box(w)] : B0;
 }
 
-__device__ inline void unwrapped_pos(int slot, const Buffers& S, const BoxDev& B, double u[3])
+__device__ inline void unwrapped_pos(int slot, int bead, const Buffers& S, const BoxDev& B, double u[3])
 {
     int c[3];
     key_cell(S.key[slot] - B.koff, B, c);
     const float4 p = S.pos[slot];
-    const int4 a = S.aux[slot];
+    const int4 a = S.img[bead];
     u[0] = (double)(c[0] + a.x * B.n[0]) * B.wd[0] + (double)p.x;
     u[1] = (double)(c[1] + a.y * B.n[1]) * B.wd[1] + (double)p.y;
     u[2] = (double)(c[2] + a.z * B.n[2]) * B.wd[2] + (double)p.z;
@@ -172,8 +173,8 @@ __device__ __attribute__((always_inline)) inline void bonded_body(const Buffers&
         box = bead_box(w);
         const BoxDev& B = box_of<MULTI>(boxes, B0, w);
         double uh[3], ut[3];
-        unwrapped_pos(sh, S, B, uh);
-        unwrapped_pos(st, S, B, ut);
+        unwrapped_pos(sh, hb.x, S, B, uh);
+        unwrapped_pos(st, hb.y, S, B, ut);
         const double2 par = T.bond_par[q];
         const double d[3] = {uh[0] - ut[0], uh[1] - ut[1], uh[2] - ut[2]};
         const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -193,9 +194,9 @@ __device__ __attribute__((always_inline)) inline void bonded_body(const Buffers&
         box = bead_box(w);
         const BoxDev& B = box_of<MULTI>(boxes, B0, w);
         double ua[3], ub[3], uc[3], f[3][3], pe = 0.0;
-        unwrapped_pos(sa, S, B, ua);
-        unwrapped_pos(sb, S, B, ub);
-        unwrapped_pos(sc, S, B, uc);
+        unwrapped_pos(sa, abc.x, S, B, ua);
+        unwrapped_pos(sb, abc.y, S, B, ub);
+        unwrapped_pos(sc, abc.z, S, B, uc);
         angle_forces(ua, ub, uc, T.ang_par[g], f, OBS ? &pe : nullptr);
         for (int r = 0; r < 3; ++r) T.fang[3 * g + r] = make_double4(f[r][0], f[r][1], f[r][2], 0.0);
         if (OBS) ob[11] = pe;
@@ -291,10 +292,9 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
         const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
         float4 v = S.vel[s];
         float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
-        const int4 ax = S.aux[s];
         if (MODE == kEndBegin && (__float_as_int(p.w) & kBondedBit)) {
             double fb[3];
-            bonded_gather(ax.w, T, fb);
+            bonded_gather(S.bead[s], T, fb);
             f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
             frc[s] = f;
         }
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
         int c[3];
         key_cell(S.key[s] - B.koff, B, c);
         const float o[3] = {p.x, p.y, p.z};
-        const int img[3] = {ax.x, ax.y, ax.z};
+        const int img[3] = {0, 0, 0};   // the image change is applied by k_permute
         float d[3] = {0.f, 0.f, 0.f};
         if (MODE != kRebin) {
             d[0] = B.dt * v.x + B.hdt2 * f.x;
@@ -409,11 +409,11 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
     const float4 v = S.vel[s];
     const float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
-    const int4 ax = S.aux[s];
+    const int q = S.bead[s];
     int c[3];
     key_cell(S.key[s] - B.koff, B, c);
     const float o[3] = {p.x, p.y, p.z};
-    const int img[3] = {ax.x, ax.y, ax.z};
+    const int img[3] = {0, 0, 0};   // m.img is the change of image this step
     float d[3] = {0.f, 0.f, 0.f};
     if (MODE != kRebin) {
         d[0] = B.dt * v.x + B.hdt2 * f.x;
@@ -427,7 +427,14 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     D.pos[dst] = make_float4(m.o[0], m.o[1], m.o[2], p.w);
     D.vel[dst] = v;
     D.fold[dst] = f;
-    D.aux[dst] = make_int4(m.img[0], m.img[1], m.img[2], ax.w);
+    D.bead[dst] = q;
+    // the image counts live by bead index: only a bead that crosses the box changes them,
+    // and only its own thread writes them
+    if (m.img[0] | m.img[1] | m.img[2]) {
+        int4 im = S.img[q];
+        im.x += m.img[0]; im.y += m.img[1]; im.z += m.img[2];
+        S.img[q] = im;
+    }
     D.key[dst] = k;
     D.code[dst] = scan_code(m.o, B);
     if (MODE == kRebin) {
@@ -435,7 +442,7 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     } else {
         vp[dst] = make_float4(v.x + B.lamdt * f.x, v.y + B.lamdt * f.y, v.z + B.lamdt * f.z, v.w);
     }
-    slot_of[ax.w] = dst;
+    slot_of[q] = dst;
 }
 
 // REBIN leaves the uploaded force in fold; the canonical state needs it in frc too
@@ -535,10 +542,10 @@ __global__ __launch_bounds__(256) void k_gather(int n, Buffers S, const float4* 
     const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
     int c[3];
     key_cell(S.key[s] - B.koff, B, c);
-    const int4 a = S.aux[s];
+    const int q = S.bead[s];
+    const int4 a = S.img[q];
     const float4 vv = S.vel[s];
     const float4 ff = frc[s];
-    const int q = a.w;
     const double o[3] = {p.x, p.y, p.z};
     const int im[3] = {a.x, a.y, a.z};
     for (int d = 0; d < 3; ++d) {
@@ -599,7 +606,8 @@ struct dpd_engine {
     // device buffers
     BoxDev* dboxes = nullptr;
     float4 *pos[2] = {}, *vel[2] = {}, *fold[2] = {};
-    int4* aux[2] = {};
+    int* bead[2] = {};
+    int4* img = nullptr;   // periodic image counts by bead index
     int* key[2] = {};
     uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 3 slots past the end
     float4 *frc = nullptr, *vp = nullptr;
@@ -737,7 +745,7 @@ int upload_boxes(dpd_engine* e)
 
 Buffers bufs(dpd_engine* e, int i)
 {
-    return Buffers{e->pos[i], e->vel[i], e->fold[i], e->aux[i], e->key[i], e->code[i]};
+    return Buffers{e->pos[i], e->vel[i], e->fold[i], e->bead[i], e->img, e->key[i], e->code[i]};
 }
 
 Topo topo(dpd_engine* e)
@@ -973,9 +981,10 @@ int commit_beads(dpd_engine* e, int64_t step0)
     if (n != e->n) {
         for (int b = 0; b < 2; ++b)
             if ((rc = dalloc(e, &e->pos[b], n)) || (rc = dalloc(e, &e->vel[b], n)) ||
-                (rc = dalloc(e, &e->fold[b], n)) || (rc = dalloc(e, &e->aux[b], n)) ||
+                (rc = dalloc(e, &e->fold[b], n)) || (rc = dalloc(e, &e->bead[b], n)) ||
                 (rc = dalloc(e, &e->key[b], n)) || (rc = dalloc(e, &e->code[b], n + kCodePad)))
                 return rc;
+        if ((rc = dalloc(e, &e->img, n))) return rc;
         for (int b = 0; b < 2; ++b) HIPCHK(hipMemsetAsync(e->code[b] + n, 0, sizeof(uint32_t) * kCodePad, e->stream));
         if ((rc = dalloc(e, &e->frc, n)) || (rc = dalloc(e, &e->vp, n)) || (rc = dalloc(e, &e->slot_of, n)) ||
             (rc = dalloc(e, &e->nkey, n)) || (rc = dalloc(e, &e->rank, n)) || (rc = dalloc(e, &e->gather, 12 * n)))
@@ -984,7 +993,8 @@ int commit_beads(dpd_engine* e, int64_t step0)
     }
     if ((rc = upload_boxes(e))) return rc;
     std::vector<float4> hp(n), hv(n), hf(n);
-    std::vector<int4> ha(n);
+    std::vector<int> hb(n);
+    std::vector<int4> hi(n);
     std::vector<int> hk(n);
     for (size_t b = 0; b < e->boxes.size(); ++b) {
         const BoxHost& h = e->boxes[b];
@@ -1013,7 +1023,8 @@ int commit_beads(dpd_engine* e, int64_t step0)
             hv[g] = make_float4((float)h.v[3 * i], (float)h.v[3 * i + 1], (float)h.v[3 * i + 2], ib);
             hf[g] = h.has_f ? make_float4((float)h.f[3 * i], (float)h.f[3 * i + 1], (float)h.f[3 * i + 2], 0.f)
                             : make_float4(0.f, 0.f, 0.f, 0.f);
-            ha[g] = make_int4(im[0], im[1], im[2], (int)g);
+            hb[g] = (int)g;
+            hi[g] = make_int4(im[0], im[1], im[2], 0);
             hk[g] = B.koff + cell_key(c[0], c[1], c[2], B);
         }
     }
@@ -1021,7 +1032,8 @@ int commit_beads(dpd_engine* e, int64_t step0)
     HIPCHK(hipMemcpyAsync(e->pos[src], hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->vel[src], hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->fold[src], hf.data(), sizeof(float4) * n, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemcpyAsync(e->aux[src], ha.data(), sizeof(int4) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->bead[src], hb.data(), sizeof(int) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->img, hi.data(), sizeof(int4) * n, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->key[src], hk.data(), sizeof(int) * n, hipMemcpyHostToDevice, e->stream));
     launch_zero(e, true);
     if ((rc = sort_step<kRebin>(e))) return rc;
@@ -1243,7 +1255,7 @@ void dpd_destroy(dpd_engine* e)
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
-    void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->aux[0], e->aux[1],
+    void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->bead[0], e->bead[1], e->img,
                     e->key[0], e->key[1], e->code[0], e->code[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
                     e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
@@ -1569,12 +1581,13 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
     // snapshot of the state the trials start from
     float4 *sp = nullptr, *sv = nullptr, *sfo = nullptr, *sf = nullptr, *svp = nullptr;
     int4* sa = nullptr;
-    int *sk = nullptr, *sso = nullptr, *scs = nullptr;
+    int *sk = nullptr, *sso = nullptr, *scs = nullptr, *sbd = nullptr;
     uint32_t* scd = nullptr;
     int rc;
     if ((rc = dalloc(e, &sp, n)) || (rc = dalloc(e, &sv, n)) || (rc = dalloc(e, &sfo, n)) || (rc = dalloc(e, &sf, n)) ||
         (rc = dalloc(e, &svp, n)) || (rc = dalloc(e, &sa, n)) || (rc = dalloc(e, &sk, n)) ||
-        (rc = dalloc(e, &sso, n)) || (rc = dalloc(e, &scs, e->nkeys + 1)) || (rc = dalloc(e, &scd, n)))
+        (rc = dalloc(e, &sso, n)) || (rc = dalloc(e, &scs, e->nkeys + 1)) || (rc = dalloc(e, &scd, n)) ||
+        (rc = dalloc(e, &sbd, n)))
         return rc;
     const int c0 = e->cur;
     const int64_t step0 = e->step;
@@ -1587,7 +1600,8 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
             HIPCHK(hipMemcpyAsync(sfo, e->fold[c0], f4, k, e->stream));
             HIPCHK(hipMemcpyAsync(sf, e->frc, f4, k, e->stream));
             HIPCHK(hipMemcpyAsync(svp, e->vp, f4, k, e->stream));
-            HIPCHK(hipMemcpyAsync(sa, e->aux[c0], sizeof(int4) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(sa, e->img, sizeof(int4) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(sbd, e->bead[c0], sizeof(int) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(sk, e->key[c0], sizeof(int) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(scd, e->code[c0], sizeof(uint32_t) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(sso, e->slot_of, sizeof(int) * n, k, e->stream));
@@ -1598,7 +1612,8 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
             HIPCHK(hipMemcpyAsync(e->fold[c0], sfo, f4, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->frc, sf, f4, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->vp, svp, f4, k, e->stream));
-            HIPCHK(hipMemcpyAsync(e->aux[c0], sa, sizeof(int4) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->img, sa, sizeof(int4) * n, k, e->stream));
+            HIPCHK(hipMemcpyAsync(e->bead[c0], sbd, sizeof(int) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->key[c0], sk, sizeof(int) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->code[c0], scd, sizeof(uint32_t) * n, k, e->stream));
             HIPCHK(hipMemcpyAsync(e->slot_of, sso, sizeof(int) * n, k, e->stream));
@@ -1623,7 +1638,7 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
         if (ms < best) { best = ms; bestk = v + 1; }
         if ((rc = copy(false))) break;
     }
-    void* tmp[] = {sp, sv, sfo, sf, svp, sa, sk, sso, scs, scd};
+    void* tmp[] = {sp, sv, sfo, sf, svp, sa, sk, sso, scs, scd, sbd};
     for (void* q : tmp) (void)hipFree(q);
     if (rc) return rc;
     e->obs_valid = false;
@@ -1702,7 +1717,7 @@ int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_o
     a.B = h.B;
     a.pos = e->pos[e->cur];
     a.vp = e->vp;
-    a.aux = e->aux[e->cur];
+    a.img = e->img;
     a.key = e->key[e->cur];
     a.cstart = e->cstart;
     a.table = e->table + (size_t)box * kTableStride;

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_slice_bonds(SliceArgs A, double* __rest
         int c[3];
         key_cell(A.key[slot] - B.koff, B, c);
         const float4 p = A.pos[slot];
-        const int4 a = A.aux[slot];
+        const int4 a = A.img[e ? hb.y : hb.x];
         double* u = e ? ut : uh;
         u[0] = (double)(c[0] + a.x * B.n[0]) * B.wd[0] + (double)p.x;
         u[1] = (double)(c[1] + a.y * B.n[1]) * B.wd[1] + (double)p.y;

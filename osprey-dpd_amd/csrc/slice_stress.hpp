@@ -10,7 +10,7 @@ struct SliceArgs {
     BoxDev B;                 // the box whose profile is taken
     const float4* pos;        // offset xyz, type | box << 8 (sorted)
     const float4* vp;         // predicted velocity xyz, id bits (sorted)
-    const int4* aux;          // image xyz, bead index (sorted)
+    const int4* img;          // periodic image counts by bead index
     const int* key;           // global cell key per slot
     const int* cstart;        // first slot of each global cell key
     const float* table;       // this box's a | gamma | sqrt(gamma)
