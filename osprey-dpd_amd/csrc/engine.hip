@@ -240,11 +240,15 @@ void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buff
     hitlist_body<OBS, false>(A, B0, nblocks, per_xcd);
 }
 
-// The bonded flag in pos.w: cleared on every slot, then set on the bonded beads' slots.
-__global__ __launch_bounds__(256) void k_clear_bonded_flag(int n, float4* __restrict__ pos)
+// The bonded flag in pos.w: cleared on every slot (and the slot of every bead rewritten,
+// since steps store it for flagged beads only), then set on the bonded beads' slots.
+__global__ __launch_bounds__(256) void k_clear_bonded_flag(int n, float4* __restrict__ pos,
+                                                           const int* __restrict__ bead, int* __restrict__ slot_of)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) pos[s].w = __int_as_float(__float_as_int(pos[s].w) & ~kBondedBit);
+    if (s >= n) return;
+    pos[s].w = __int_as_float(__float_as_int(pos[s].w) & ~kBondedBit);
+    slot_of[bead[s]] = s;
 }
 
 __global__ __launch_bounds__(256) void k_set_bonded_flag(int nbonded, const int* __restrict__ bonded,
@@ -442,7 +446,10 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     } else {
         vp[dst] = make_float4(v.x + B.lamdt * f.x, v.y + B.lamdt * f.y, v.z + B.lamdt * f.z, v.w);
     }
-    slot_of[q] = dst;
+    // the slot of each bead is read for bonded beads only (bond and angle terms); a scattered
+    // 4-byte store per bead costs a partial-line write each, so the others are skipped once
+    // the topology has flagged the bonded ones (an upload re-binning writes every bead)
+    if (MODE == kRebin || (__float_as_int(p.w) & kBondedBit)) slot_of[q] = dst;
 }
 
 // REBIN leaves the uploaded force in fold; the canonical state needs it in frc too
@@ -1129,7 +1136,8 @@ int commit_topology(dpd_engine* e)
     e->nbonded = (int)bonded.size();
     e->has_bonds = nbonds > 0;
     e->has_angles = npairs > 0;
-    hipLaunchKernelGGL(k_clear_bonded_flag, dim3(grid(n, 256)), dim3(256), 0, e->stream, (int)n, e->pos[e->cur]);
+    hipLaunchKernelGGL(k_clear_bonded_flag, dim3(grid(n, 256)), dim3(256), 0, e->stream, (int)n, e->pos[e->cur],
+                       e->bead[e->cur], e->slot_of);
     if (e->nbonded > 0)
         hipLaunchKernelGGL(k_set_bonded_flag, dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, e->nbonded,
                            e->bonded, e->slot_of, e->pos[e->cur]);
