@@ -326,8 +326,8 @@ def main():
     e = build_engine(wl, args.pair_kernel)
     e.step(args.warmup)
     t0 = time.perf_counter()
-    # HIP events bracket every 4th pair launch of the timed steps (timing every launch adds
-    # ~3.5% of marker overhead to the step); synchronises on return
+    # HIP events bracket every 16th pair launch of the timed steps (13 of 200; each timed
+    # launch adds ~30 us of event markers to its step); synchronises on return
     e.step(args.steps, observe=True, timing="sampled")
     wall = time.perf_counter() - t0
     nlaunch, ms3 = e.last_timing()

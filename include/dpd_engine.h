@@ -96,7 +96,7 @@ typedef struct dpd_obs {
 #define DPD_NO_THERMOSTAT  2u   /* dpd_compute_forces: drop dissipative + random     */
 #define DPD_STEP_TIMING    4u   /* time every pair-kernel launch with HIP events     */
 #define DPD_STEP_GRAPH     8u   /* replay captured hipGraphs of up to 64 steps       */
-#define DPD_STEP_TIMING_SAMPLED 16u /* with DPD_STEP_TIMING: time every 4th pair launch */
+#define DPD_STEP_TIMING_SAMPLED 16u /* with DPD_STEP_TIMING: time every 16th pair launch */
                                 /* only (a quarter of the event-marker overhead)     */
 
 /* toggles of dpd_toggle (command handlers of src/CNTCell.cpp:223-346) */
@@ -174,7 +174,7 @@ int dpd_set_box_interactions(dpd_engine* e, int32_t box, const double* a, const 
 
 /* Timing of the last dpd_step call (ms, from HIP events on the engine's stream):
  * [0] whole call, [1] the timed pair-kernel launches (with DPD_STEP_TIMING; all of them,
- * or every 4th with DPD_STEP_TIMING_SAMPLED), [2] [0] - [1].  Returns the number of
+ * or every 16th with DPD_STEP_TIMING_SAMPLED), [2] [0] - [1].  Returns the number of
  * pair-kernel launches timed. */
 int64_t dpd_last_timing(dpd_engine* e, double* ms3);
 int64_t dpd_get_step(const dpd_engine* e);

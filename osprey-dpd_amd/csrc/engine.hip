@@ -917,7 +917,7 @@ int finish_obs(dpd_engine* e)
 // the steps of one dpd_step call, enqueued on the stream (capturable)
 // steps whose pair launch is bracketed by events: every step, or every
 // kTimingStride-th with DPD_STEP_TIMING_SAMPLED
-constexpr int64_t kTimingStride = 4;
+constexpr int64_t kTimingStride = 16;
 inline bool timed_step(int64_t t, int timing) { return timing == 1 || (timing == 2 && t % kTimingStride == 0); }
 
 // timing: 0 none, 1 every pair launch, 2 every kTimingStride-th

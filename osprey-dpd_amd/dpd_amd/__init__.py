@@ -179,7 +179,7 @@ class Engine:
                                               _ptr(modulus), _ptr(phi0)))
 
     def step(self, nsteps=1, observe=False, timing=False, graph=False):
-        """timing: False, True (every pair launch timed) or "sampled" (every 4th)."""
+        """timing: False, True (every pair launch timed) or "sampled" (every 16th)."""
         flags = (STEP_OBSERVE if observe else 0) | (STEP_TIMING if timing else 0) | \
             (STEP_TIMING_SAMPLED if timing == "sampled" else 0) | (STEP_GRAPH if graph else 0)
         self._check(lib().dpd_step(self.h, int(nsteps), flags))

@@ -70,8 +70,8 @@ def test_pair_timing_every_launch_or_sampled():
     e.step(20, timing=True)
     n, ms = e.last_timing()
     assert n == 20 and 0 < ms[1] < ms[0] and abs(ms[2] - (ms[0] - ms[1])) < 1e-9
-    e.step(22, timing="sampled")
+    e.step(40, timing="sampled")
     n2, ms2 = e.last_timing()
-    assert n2 == 6                                  # steps 0, 4, ..., 20 of the call
+    assert n2 == 3                                  # steps 0, 16, 32 of the call
     say(f"  pair per launch: all {ms[1] / n * 1e3:.1f} us, sampled {ms2[1] / n2 * 1e3:.1f} us")
     assert 0.5 < (ms2[1] / n2) / (ms[1] / n) < 2.0
