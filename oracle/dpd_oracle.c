@@ -228,6 +228,26 @@ void orc_set_a(orc_sys* s, const double* a)
     memcpy(s->a, a, sizeof(double) * s->p.ntypes * s->p.ntypes);
 }
 
+/* CCNTCell::ChangeTimeStepConstants (src/CNTCell.cpp:357-377): every constant that
+ * depends on the step, the half-step ones included */
+void orc_set_timestep(orc_sys* s, double dt)
+{
+    if (!(dt > 0.0)) return;
+    s->p.dt = dt;
+    s->halfdt = 0.5 * dt;
+    s->halfdt2 = 0.5 * dt * dt;
+    s->invrootdt = sqrt(24.0 * s->p.kT / dt);
+    s->lamdt = s->p.lambda * dt;
+}
+
+/* CCNTCell::ToggleDPDBeadForces (src/CNTCell.cpp:223-227): the half-step constants only
+ * (dt = 0 stops forces moving beads or changing velocities) */
+void orc_set_halfstep(orc_sys* s, double dt)
+{
+    s->halfdt = 0.5 * dt;
+    s->halfdt2 = 0.5 * dt * dt;
+}
+
 /* CCNTCell::UpdatePos (src/CNTCell.cpp:1704-2262), DPD branch */
 static void update_pos(orc_sys* s)
 {
@@ -577,6 +597,62 @@ void orc_forces(const orc_params* p, int32_t n, const int32_t* id, const int32_t
             for (int qq = 0; qq < 9; ++qq) virial[qq] += s->stress[9 * b + qq];
     }
     if (pe) { pe[0] = pes[0]; pe[1] = pes[1]; pe[2] = pes[2]; }
+    orc_destroy(s);
+}
+
+/* The potential energy the reference writes with ToggleEnergyOutput
+ * (CMonitor::ZeroTotalEnergy + CCNTCell::UpdateTotalEnergy, src/Monitor.cpp:2755-2778,
+ * src/CNTCell.cpp:2909-3214): pairs of one cell once each (reverse loop to the bead
+ * itself), but a pair across cells only from the half-shell side and only when the
+ * neighbour's id is larger (src/CNTCell.cpp:3081), so about half of the cross-cell pairs
+ * are left out; bonds 1/2 k (L - l0)^2 and bond pairs (src/BondPair.cpp:320-378) in
+ * full.  out[0] pair, out[1] bond, out[2] bond-pair energy. */
+void orc_energy_output(const orc_params* p, int32_t n, const int32_t* id, const int32_t* type,
+                       const double* x, const double* unpbc, const orc_topology* topo, double* out)
+{
+    orc_params q = *p;
+    q.rng_mode = ORC_RNG_PHILOX;
+    orc_sys* s = orc_create(&q, n, id, type, x, unpbc, x, NULL, topo);
+    const int nt = s->p.ntypes;
+    double pair = 0.0;
+    for (int c = 0; c < s->ncell; ++c) {
+        for (int b1 = s->head[c]; b1 >= 0; b1 = s->next[b1]) {
+            const double* x1 = s->x + 3 * b1;
+            for (int b2 = s->tail[c]; b2 != b1; b2 = s->prev[b2]) {
+                double dr2 = 0.0;
+                for (int d = 0; d < 3; ++d) { double dd = x1[d] - s->x[3 * b2 + d]; dr2 += dd * dd; }
+                if (dr2 < 1.0) {
+                    double dr = sqrt(dr2);
+                    if (dr > 0.000000001) pair += 0.5 * s->a[s->type[b1] * nt + s->type[b2]] * (1.0 - dr) * (1.0 - dr);
+                }
+            }
+            for (int h = 0; h < 13; ++h) {
+                int nc = s->nn27[27 * c + 13 + HALF[h][0] + 3 * HALF[h][1] + 9 * HALF[h][2]];
+                int pbc = s->ext[c] && s->ext[nc];
+                for (int b2 = s->head[nc]; b2 >= 0; b2 = s->next[b2]) {
+                    if (!(s->id[b2] > s->id[b1])) continue;
+                    double dx[3];
+                    for (int d = 0; d < 3; ++d) dx[d] = x1[d] - s->x[3 * b2 + d];
+                    if (pbc)
+                        for (int d = 0; d < 3; ++d) {
+                            if (dx[d] > s->halfL[d]) dx[d] -= s->L[d];
+                            else if (dx[d] < -s->halfL[d]) dx[d] += s->L[d];
+                        }
+                    double dr2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+                    if (dr2 < 1.0) {
+                        double dr = sqrt(dr2);
+                        if (dr > 0.000000001) pair += 0.5 * s->a[s->type[b1] * nt + s->type[b2]] * (1.0 - dr) * (1.0 - dr);
+                    }
+                }
+            }
+        }
+    }
+    double pb = 0.0, pa = 0.0;
+    bond_forces(s, &pb);
+    angle_forces(s, &pa);
+    out[0] = pair;
+    out[1] = pb;
+    out[2] = pa;
     orc_destroy(s);
 }
 

@@ -87,6 +87,10 @@ void     orc_get(const orc_sys* s, double* x, double* unpbc, double* v, double* 
 void     orc_observables(const orc_sys* s, orc_obs* out);
 void     orc_set_gamma(orc_sys* s, const double* g);   /* thermostat toggles */
 void     orc_set_a(orc_sys* s, const double* a);
+/* CCNTCell::ChangeTimeStepConstants (SetTimeStepSize): dt and every constant using it */
+void     orc_set_timestep(orc_sys* s, double dt);
+/* CCNTCell::ToggleDPDBeadForces: the half-step constants from dt (0 = forces off) */
+void     orc_set_halfstep(orc_sys* s, double dt);
 
 /* One force evaluation at given positions and (predicted) velocities, Philox stream
  * keyed by `step`.  thermostat: 1 = dissipative + random, 0 = neither, 2 = dissipative
@@ -96,6 +100,11 @@ void orc_forces(const orc_params* p, int32_t n, const int32_t* id, const int32_t
                 const double* x, const double* unpbc, const double* v,
                 const orc_topology* topo, int64_t step, int32_t thermostat,
                 double* f, double* virial, double* pe);
+/* The potential energy of the reference's ToggleEnergyOutput rows at positions x: out[0]
+ * pair (pairs across cells only when the neighbour's id is larger, as the reference sums
+ * them), out[1] bonds, out[2] bond pairs. */
+void orc_energy_output(const orc_params* p, int32_t n, const int32_t* id, const int32_t* type,
+                       const double* x, const double* unpbc, const orc_topology* topo, double* out);
 /* Same as the pair part of orc_forces but by brute force over all pairs with the
  * minimum image; checks the cell list and the half shell. */
 void orc_pair_forces_brute(const orc_params* p, int32_t n, const int32_t* id,

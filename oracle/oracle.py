@@ -57,6 +57,8 @@ def lib():
         L.orc_observables.argtypes = [C.c_void_p, C.POINTER(Obs)]
         L.orc_set_gamma.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_set_a.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_timestep.argtypes = [C.c_void_p, C.c_double]
+        L.orc_set_halfstep.argtypes = [C.c_void_p, C.c_double]
         L.orc_forces.argtypes = [C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.POINTER(Topology), C.c_int64, C.c_int32,
                                  C.c_void_p, C.c_void_p, C.c_void_p]
@@ -68,6 +70,8 @@ def lib():
         L.orc_slice_stress.argtypes = [C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.POINTER(Topology), C.c_int64, C.c_int32,
                                        C.c_int32, C.c_void_p, C.c_void_p]
+        L.orc_energy_output.argtypes = [C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.POINTER(Topology), C.c_void_p]
         L.orc_lcg_uniform.restype = C.c_double
         L.orc_lcg_uniform.argtypes = [C.POINTER(C.c_uint64)]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -149,6 +153,7 @@ class System:
                       for a, dt in ((ids, np.int32), (types, np.int32), (x, np.float64),
                                     (unpbc, np.float64), (v, np.float64), (f, np.float64))]
         self.box = box
+        self.dt = box.dt
         self._p = box.params()
         self._t = (topo or Topo()).struct()
         self._topo = topo or Topo()
@@ -167,6 +172,44 @@ class System:
     def set_gamma(self, g):
         g = np.ascontiguousarray(g, dtype=np.float64)
         lib().orc_set_gamma(self.h, _p(g))
+
+    def set_a(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        lib().orc_set_a(self.h, _p(a))
+        self._a_now = a.copy()
+
+    def set_timestep(self, dt):
+        """SetTimeStepSize: CCNTCell::ChangeTimeStepConstants."""
+        lib().orc_set_timestep(self.h, float(dt))
+        self.dt = float(dt)
+
+    def set_halfstep(self, dt):
+        """ToggleDPDBeadForces: the half-step constants (dt = 0 switches the forces off)."""
+        lib().orc_set_halfstep(self.h, float(dt))
+
+    def _box_now(self):
+        b = self.box
+        if getattr(self, "_a_now", None) is not None:
+            b = Box(b.cells, self._a_now, b.g, dt=b.dt, lam=b.lam, kT=b.kT, seed=b.seed, widths=b.widths,
+                    rng_mode=b.rng_mode)
+        return b
+
+    def energies(self):
+        """(KE, PE, parts) as the reference's energy output writes them after a step
+        (CMonitor::ZeroTotalEnergy / CCNTCell::UpdateTotalEnergy, src/Monitor.cpp:2755-2778):
+        KE = 1/2 sum v^2; PE = pair + bond + bond-pair energy at the current positions, the
+        pair part summed the reference's way (energy_output)."""
+        x, u, v, _ = self.state()
+        pe = energy_output(self._box_now(), self._keep[0], self._keep[1], x, unpbc=u, topo=self._topo)
+        return 0.5 * float((v * v).sum()), float(pe.sum()), pe
+
+    def potential_energies(self, step=0):
+        """The true pair, bond and bond-pair energies at the current positions (every pair
+        once)."""
+        x, u, v, _ = self.state()
+        _, _, pe = forces(self._box_now(), self._keep[0], self._keep[1], x, v, step, thermostat=False, unpbc=u,
+                          topo=self._topo)
+        return pe
 
     def state(self):
         x = np.empty((self.n, 3)); u = np.empty((self.n, 3))
@@ -231,6 +274,22 @@ def slice_add(tab, Lz, idx, z1, z2, F, dx, wgt=1.0):
     lib().orc_slice_add(_p(tab), tab.shape[0], float(Lz), int(idx), tab.shape[1], float(z1), float(z2), _p(F),
                         _p(dx), float(wgt))
     return tab
+
+
+def energy_output(box, ids, types, x, unpbc=None, topo=None):
+    """[pair, bond, bond-pair] potential energy as the reference's ToggleEnergyOutput rows
+    sum it (pairs across cells only when the neighbour's id is larger)."""
+    L = lib()
+    n = len(ids)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    u = np.ascontiguousarray(unpbc, dtype=np.float64) if unpbc is not None else None
+    p = box.params()
+    ts = (topo or Topo()).struct()
+    out = np.zeros(3)
+    L.orc_energy_output(C.byref(p), n, _p(ids), _p(types), _p(x), _p(u), C.byref(ts), _p(out))
+    return out
 
 
 def pair_forces_brute(box, ids, types, x, v, step, thermostat=True):

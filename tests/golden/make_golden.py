@@ -23,6 +23,16 @@ fixtures keep the reference's own 6-significant-digit values
 (src/RestartState.cpp:134-156).  Files are gzip'd text: one bead per line,
 ``polyId polyType beadId beadType x y z ux uy uz vx vy vz fx fy fz``.
 
+``th10_energy.txt`` (every case) repeats the ten thermostatted steps with
+``Command ToggleEnergyOutput 1 0`` (src/mcToggleEnergyOutput.cpp, src/Monitor.cpp:3171):
+its ``dmpchs`` rows carry the kinetic and the potential energy (pair, bond and bond-pair
+terms, src/Monitor.cpp:2755-2778, src/BondPair.cpp:320-378) after every step.
+
+``commands/`` starts from the lipidphi8 snapshot and runs 14 steps under the command
+schedule COMMANDS (time step changes, thermostat, conservative-force and force toggles,
+interaction changes: src/CNTCell.cpp:223-377, :434-456): ``cmd14.dat.gz`` is the state
+after the last step and ``cmd_history.txt`` the ``dmpchs`` rows of every step.
+
 ``rng.txt`` holds the first 1000 draws of the reference LCG for seed |RNGSeed|, logged
 by ``Command GenerateRNGUniform`` (src/mcGenerateRNGUniformImpl.cpp:44-62).
 """
@@ -244,6 +254,62 @@ def make_case(name, work):
         json.dump(count_summary(os.path.join(work, f"dmpcis.{name}u")), fh)
 
 
+# (time, command text) of the commands fixture; a command of time t runs before step t
+# (CSimBox::Run, src/SimBox.cpp:1272-1320)
+COMMANDS = [
+    (3, "SetTimeStepSize 3 0.01"),
+    (4, "ToggleDPDBeadThermostat 4"),
+    (6, "ToggleDPDBeadConservativeForces 6"),
+    (7, "ToggleDPDBeadThermostat 7"),
+    (8, "ToggleDPDBeadConservativeForces 8"),
+    (9, "SetDPDBeadConsInt 9 T W 60"),
+    (10, "SetDPDBeadDissInt 10 H W 3.0"),
+    (11, "ToggleDPDBeadForces 11"),
+    (12, "ToggleDPDBeadForces 12"),
+    (13, "SetTimeStepSize 13 0.02"),
+]
+COMMAND_STEPS = 14
+
+
+def warm_restart(name, work):
+    """The relaxed snapshot every fixture of `name` starts from (the run is deterministic:
+    it reproduces the committed in.dat.gz)."""
+    w = name + "w"
+    run(work, w, dmpci(name, "random", WARM))
+    return w, f"restart RunId {w} StateId {WARM}"
+
+
+def make_energy(name, work):
+    w, restart = warm_restart(name, work)
+    run(work, name + "e", dmpci(name, restart, 10, "Command ToggleEnergyOutput 1 0\n"))
+    hist = history(os.path.join(work, f"dmpchs.{name}e"))
+    with open(os.path.join(HERE, name, "th10_energy.txt"), "w") as fh:
+        fh.write("# time kT P KE PE E ... (reference dmpchs rows, thermostat on, ToggleEnergyOutput)\n")
+        for row in hist:
+            fh.write(" ".join(row) + "\n")
+
+
+def make_commands(work):
+    name = "lipidphi8"
+    out = os.path.join(HERE, "commands")
+    os.makedirs(out, exist_ok=True)
+    w, restart = warm_restart(name, work)
+    inp = bead_lines(os.path.join(work, f"dmpcrs.{w}.con.{WARM}.dat"))
+    with gzip.open(os.path.join(HERE, name, "in.dat.gz"), "rt") as fh:
+        assert fh.read().split("\n")[:-1] == inp, "warm-up no longer reproduces the lipidphi8 snapshot"
+    cmds = "".join(f"Command {c}\n" for _, c in COMMANDS)
+    run(work, "cmd", dmpci(name, restart, COMMAND_STEPS, cmds))
+    write_gz(os.path.join(out, f"cmd{COMMAND_STEPS}.dat.gz"),
+             bead_lines(os.path.join(work, f"dmpcrs.cmd.con.{COMMAND_STEPS}.dat")))
+    hist = history(os.path.join(work, "dmpchs.cmd"))
+    with open(os.path.join(out, "cmd_history.txt"), "w") as fh:
+        fh.write("# time kT P ... (reference dmpchs rows under the command schedule)\n")
+        for row in hist:
+            fh.write(" ".join(row) + "\n")
+    with open(os.path.join(out, "commands.json"), "w") as fh:
+        json.dump({"case": name, "steps": COMMAND_STEPS, "commands": COMMANDS}, fh, indent=1)
+
+
 def make_rng(work):
     # a restart-based run draws nothing before step 1, so the logged draws are the
     # first draws of the stream seeded with |RNGSeed|
@@ -263,11 +329,23 @@ def main():
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
     work = tempfile.mkdtemp(prefix="dpd_golden_")
     try:
+        args = sys.argv[1:]
+        if args[:1] == ["extras"]:
+            # only the energy and command fixtures (the snapshots stay as committed)
+            for name in CASES:
+                make_energy(name, work)
+                print("wrote energy rows of", name)
+            make_commands(work)
+            print("wrote commands")
+            return
         make_rng(work)
-        names = sys.argv[1:] or list(CASES)
+        names = args or list(CASES)
         for name in names:
             make_case(name, work)
+            make_energy(name, work)
             print("wrote", name)
+        make_commands(work)
+        print("wrote commands")
     finally:
         shutil.rmtree(work)
 
