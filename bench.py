@@ -4,16 +4,17 @@
 Default workload: the polymer condensate box (64^3 cells at rho = 3, 786,432 beads;
 16-mer (S P P P)x4 chains at 20% of the beads, a_SS = 10, other a = 25, gamma = 4.5,
 Hookean bonds k = 128, l0 = 0.5, dt = 0.02): the single-box throughput run.  The state is
-synthetic (uniform water, random-walk chains, Maxwell velocities) and is relaxed by the
-warm-up steps.  One "step" is one full CSimBox::Evolve equivalent (sort, pair forces,
+synthetic (uniform water, random-walk chains, Maxwell velocities) and is relaxed before
+the warm-up steps (steps with periodic velocity rescaling to kT, then free steps).  One "step" is one full CSimBox::Evolve equivalent (sort, pair forces,
 bonds, integration) of the whole box; inputs stay in HBM.
 
 Prints one JSON line.  `value` is bead-steps/s on the GPU; `roofline` is the pair kernel
-(the dominant kernel) against HBM bandwidth; `batched` is 256 replicas advanced together
+(the dominant kernel, timed as back-to-back launches between two HIP events) against HBM
+bandwidth; `batched` is 256 replicas advanced together
 against the same boxes one at a time; `other_configs` the water 20^3, binary 40^3 and
 bilayer 48x48x32 boxes; `cpu_baseline` times the compiled reference (oracle/_ref/dpd) on
-one host core (and, under `all_cores`, 16 copies at once) on the condensate configuration,
-by differencing two run lengths so that set-up cancels.
+one host core (and, under `all_cores`, one copy per core of the job's CPU share at once) on
+the condensate configuration, by differencing two run lengths so that set-up cancels.
 """
 import argparse
 import json
@@ -156,7 +157,7 @@ def bench_configs(steps=100, warmup=20):
     return out
 
 
-def cpu_baseline_reference(t1=2, t2=6):
+def cpu_baseline_reference(t1=2, t2=12):
     """Time the reference binary on the condensate configuration: run lengths t1 and t2
     concurrently (one core each); per-step time = (wall2 - wall1)/(t2 - t1)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "dpd")
@@ -193,7 +194,7 @@ def cpu_baseline_reference(t1=2, t2=6):
         shutil.rmtree(work, ignore_errors=True)
 
 
-def cpu_all_cores_reference(t1=2, t2=6, copies=None):
+def cpu_all_cores_reference(t1=2, t2=8, copies=None):
     """Whole-host throughput of the reference binary: `copies` independent condensate runs
     at once (one per core), first all of length t1, then all of length t2; per-step time
     under the same contention = (mean wall t2 - mean wall t1)/(t2 - t1)."""
@@ -201,7 +202,7 @@ def cpu_all_cores_reference(t1=2, t2=6, copies=None):
     if not os.path.exists(exe):
         return None
     if copies is None:
-        copies = min(16, len(os.sched_getaffinity(0)))
+        copies = cpu_share()
     work = tempfile.mkdtemp(prefix="dpd_cpuall_")
     try:
         mean_wall = {}
@@ -302,12 +303,51 @@ def copy_probe(nbytes=256 << 20):
     print(json.dumps({"copy_probe_bytes": nbytes}))
 
 
+def relax(e, rounds=8, steps=50, free=100):
+    """Untimed relaxation of the synthetic start (uniform positions, random-walk chains):
+    `rounds` blocks of `steps` steps, each followed by a velocity rescale to kT
+    (RenormaliseMomenta), then `free` steps without rescaling, so that the timed steps run
+    at the thermostat's kT rather than through the start-up transient."""
+    for _ in range(rounds):
+        e.step(steps, graph=True)
+        e.renormalise_momenta()
+    e.step(free, graph=True)
+
+
+def host_cpu():
+    """CPU model and core counts of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "share": cpu_share()}
+
+
+def cpu_share():
+    """Cores this process may keep busy: the affinity mask, capped by OMP_NUM_THREADS when the
+    host sets it to the job's CPU share."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        omp = 0
+    return min(n, omp) if omp > 0 else n
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--pair-kernel", type=int, default=D.PAIR_AUTO)
+    ap.add_argument("--pair-launches", type=int, default=20,
+                    help="back-to-back pair launches timed for the roofline figure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the water, binary and bilayer boxes")
@@ -324,32 +364,40 @@ def main():
 
     wl = W.condensate()
     e = build_engine(wl, args.pair_kernel)
-    e.step(args.warmup)
+    relax(e)
+    # warm-up steps, then one untimed pass of the timed call so that its graphs are captured
+    _warm_graphs(e, args.warmup, args.steps)
+    kT_start = e.observables()["kT"]
     t0 = time.perf_counter()
-    # HIP events bracket every 16th pair launch of the timed steps (13 of 200; each timed
-    # launch adds ~30 us of event markers to its step); synchronises on return
-    e.step(args.steps, observe=True, timing="sampled")
+    e.step(args.steps, observe=True, graph=True)   # synchronises on return
     wall = time.perf_counter() - t0
-    nlaunch, ms3 = e.last_timing()
     obs = e.observables()
+    # the dominant kernel's launch time on the state the timed steps ended at: back-to-back
+    # launches between two events on the engine's stream (no markers between launches)
+    t_pair = e.time_pair(args.pair_launches) * 1e-3
 
     bead_steps = wl.n * args.steps
     value = bead_steps / wall
+    ms_per_step = wall / args.steps * 1e3
     alg_bytes = BYTES_PER_BEAD_STEP * wl.n + BYTES_PER_BOND_STEP * wl.nbonds + BYTES_PER_ANGLE_STEP * wl.nangles
-    t_pair = ms3[1] / max(nlaunch, 1) * 1e-3
     achieved = alg_bytes / t_pair / 1e9
     kname = D.PAIR_KERNEL_NAMES[e.pair_kernel]
     if e.pair_kernel == D.PAIR_HITLIST and wl.nbonds + wl.nangles > 0:
         kname = "k_pair_hitlist_bonded"   # the bonded terms run in the same launch
-    # HBM bytes per launch from the PMC runs summarised under profiles/, if they are of
-    # this kernel on this workload
-    traffic = None
+    consistent = t_pair * 1e3 <= ms_per_step
+    if not consistent:
+        print(f"bench: pair launch {t_pair * 1e3:.4f} ms exceeds the step {ms_per_step:.4f} ms",
+              file=sys.stderr, flush=True)
+    # HBM bytes per launch and the issue limiter from the counter runs summarised under
+    # profiles/, if they are of this kernel on this workload
+    traffic, limiter = None, None
     tpath = os.path.join(ROOT, "profiles", "pair_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tj = json.load(fh)
         if tj.get("kernel") == kname and tj.get("workload") == wl.name:
             traffic = tj.get("bytes_per_launch")
+            limiter = tj.get("limiter")
 
     copy_gbs = measured_copy_gbs()
     batched = None if args.no_batched else bench_batched(pair_kernel=args.pair_kernel)
@@ -357,22 +405,33 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline_reference() or cpu_baseline_port(wl)
-        if cpu and cpu["kind"] == "reference":
+        cpu = cpu_baseline_reference()
+        if cpu is None:
+            print("bench: reference binary oracle/_ref/dpd unavailable; timing the oracle port instead",
+                  file=sys.stderr, flush=True)
+            cpu = cpu_baseline_port(wl)
+        cpu["host"] = host_cpu()
+        if cpu["kind"] == "reference":
             cpu["all_cores"] = cpu_all_cores_reference()
 
     out = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": wl.name, "beads": wl.n, "bonds": wl.nbonds, "rho": 3, "dt": wl.dt,
-                   "types": "W,S,P a_SS=10 a=25 gamma=4.5", "parallelism": "single GPU"},
+                   "types": "W,S,P a_SS=10 a=25 gamma=4.5", "parallelism": "single GPU",
+                   "start": "synthetic, relaxed untimed (400 steps with kT rescaling + 100 free)",
+                   "kT_start": kT_start},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "limiter": limiter,
                      "measured_copy_gbs": copy_gbs,
                      "flops_per_bead_step": FLOPS_PER_BEAD_STEP,
                      "step_tflops": FLOPS_PER_BEAD_STEP * value / 1e12,
                      "kernel": kname, "pair_kernel": e.pair_kernel, "pair_ms_per_launch": t_pair * 1e3,
+                     "pair_launches_timed": args.pair_launches,
+                     "pair_within_step": consistent,
+                     "alg_bytes_per_launch": alg_bytes,
                      "step_gbs": alg_bytes / (wall / args.steps) / 1e9},
         "batched": batched,
         "other_configs": configs,

@@ -137,7 +137,9 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags);
 
 /* One force evaluation at the uploaded positions, treating the uploaded velocities
  * as the predicted velocities, with the pair stream of simulation time `step`.
- * f_out [3n] in upload order; virial[9] and pe[3] may be NULL. */
+ * f_out [3n] in upload order; virial[9] and pe[3] may be NULL.  A diagnostic: the state
+ * the next dpd_step starts from is left unchanged.  After dpd_step it evaluates the last
+ * step's positions and predicted velocities (the inputs of that step's force). */
 int dpd_compute_forces(dpd_engine* e, int64_t step, uint32_t flags, double* f_out,
                        double* virial, double* pe);
 
@@ -179,6 +181,14 @@ int dpd_set_box_interactions(dpd_engine* e, int32_t box, const double* a, const 
 int64_t dpd_last_timing(dpd_engine* e, double* ms3);
 int64_t dpd_get_step(const dpd_engine* e);
 
+/* Time the pair-force launch of the current state without event markers between
+ * launches: one untimed launch, then `nlaunch` back-to-back launches of the launch the
+ * step uses (the pair kernel, with the bonded terms when they share its launch) between
+ * two HIP events on the engine's stream; *ms_per_launch = elapsed / nlaunch.  The state
+ * is left as it was (the force the next step starts from is restored).  No counterpart
+ * in the reference, which has no timers. */
+int dpd_time_pair(dpd_engine* e, int32_t nlaunch, double* ms_per_launch);
+
 /* Pair-kernel choice.  dpd_autotune times `nsteps` steps of every pair kernel on the
  * current state (restoring it after each trial, so the simulation does not advance),
  * keeps the fastest and returns it; ms_per_step[DPD_PAIR_KINDS] (may be NULL) receives
@@ -214,6 +224,18 @@ int dpd_renormalise_momenta(dpd_engine* e, int32_t box);
  * pair and bond virials.  The tables are instantaneous, as the reference's are. */
 int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_out,
                      double* bond_out);
+
+/* Bond-pair stress profile across `nslices` equal z-slices of box `box` (replaces the
+ * per-step tables of CMonitor::AddBondPairStress, src/Monitor.cpp:2520-2739, which
+ * CPolymer::AddBondPairForces fills for every bond pair, src/Polymer.cpp:703).  Each bond
+ * pair's three bead pairs, first tail -> first head -> second head -> first tail, add
+ * 0.33333333 (F[j+1] - F[j]) (x) (u[j+1] - u[j]) (F the bond pair's force on each bead, u
+ * unwrapped positions) with the slice split of dpd_slice_stress.
+ * out[nslices][nt][nt][nt][9], keyed by the bead types (second head, first head, first
+ * tail); the reference keys by bond-pair type, which the declaration of the three bead
+ * types fixes.  Summed over slices and types: the bond pairs' virial sum F (x) u times
+ * 0.99999999.  The positions are those of the last completed step. */
+int dpd_slice_stress_bondpairs(dpd_engine* e, int32_t box, int32_t nslices, double* out);
 
 #ifdef __cplusplus
 }

@@ -785,3 +785,40 @@ void orc_slice_stress(const orc_params* p, int32_t n, const int32_t* id, const i
     }
     orc_destroy(s);
 }
+
+/* CMonitor::AddBondPairStress (src/Monitor.cpp:2520-2739): every bond pair adds, for each
+ * of its three bead pairs (first tail -> first head -> second head -> first tail),
+ * F (x) dx with dx = u[j+1] - u[j] (unwrapped) and F = 0.33333333 (F[j+1] - F[j]) (the
+ * bond pair's forces on the two beads) to the z-slices the pair spans, with the split of
+ * orc_slice_add.  Keyed by the bead types (second head, first head, first tail) in place of
+ * the reference's bond-pair type index, which a declaration of the three bead types fixes. */
+void orc_slice_stress_bondpairs(const orc_params* p, int32_t n, const int32_t* id, const int32_t* type,
+                                const double* x, const double* unpbc, const orc_topology* topo, int32_t nslice,
+                                double* tab)
+{
+    double* v0 = (double*)calloc((size_t)3 * (n > 0 ? n : 1), sizeof(double));
+    orc_sys* s = orc_create(p, n, id, type, x, unpbc, v0, NULL, topo);
+    free(v0);
+    const int nt = s->p.ntypes, nidx = nt * nt * nt;
+    const double Lz = s->L[2];
+    memset(tab, 0, sizeof(double) * (size_t)nslice * nidx * 9);
+    memset(s->f, 0, sizeof(double) * 3 * n);
+    bond_forces(s, NULL);   /* the bond vectors and lengths the bond pairs use */
+    for (int i = 0; i < s->np; ++i) {
+        const int b1 = s->pf[i], b2 = s->ps[i];
+        double ft[3], fh[3], fm[3];
+        angle_one(s->pk[i], s->pphi0[i], s->bdx + 3 * b1, s->blen[b1], s->bdx + 3 * b2, s->blen[b2], ft, fh, fm);
+        const int bead[4] = {s->bt[b1], s->bh[b1], s->bh[b2], s->bt[b1]};
+        const double* F[4] = {ft, fm, fh, ft};
+        const int idx = (s->type[bead[2]] * nt + s->type[bead[1]]) * nt + s->type[bead[0]];
+        for (int j = 0; j < 3; ++j) {
+            double dx[3], fj[3];
+            for (int d = 0; d < 3; ++d) {
+                dx[d] = s->u[3 * bead[j + 1] + d] - s->u[3 * bead[j] + d];
+                fj[d] = 0.33333333 * (F[j + 1][d] - F[j][d]);
+            }
+            orc_slice_add(tab, nslice, Lz, idx, nidx, s->x[3 * bead[j + 1] + 2], s->x[3 * bead[j] + 2], fj, dx, 1.0);
+        }
+    }
+    orc_destroy(s);
+}

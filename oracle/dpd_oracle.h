@@ -124,6 +124,14 @@ void orc_slice_stress(const orc_params* p, int32_t n, const int32_t* id, const i
                       const double* x, const double* unpbc, const double* v, const orc_topology* topo,
                       int64_t step, int32_t thermostat, int32_t nslice, double* pair_tab, double* bond_tab);
 
+/* The bond-pair slice stress table of the state (CMonitor::AddBondPairStress,
+ * src/Monitor.cpp:2520-2739): tab[nslice][ntypes^3][9] keyed by the bead types (second
+ * head, first head, first tail) of each bond pair; summed over slices it is the bond
+ * pairs' virial sum_i F_i (x) u_i times 0.99999999. */
+void orc_slice_stress_bondpairs(const orc_params* p, int32_t n, const int32_t* id, const int32_t* type,
+                                const double* x, const double* unpbc, const orc_topology* topo, int32_t nslice,
+                                double* tab);
+
 /* reference LCG (src/CNTCell.cpp:2454-2465): advances *state, returns U in [0,1) */
 double   orc_lcg_uniform(uint64_t* state);
 /* Philox4x32-10 block: ctr[4], key[2] -> out[4] */

@@ -70,6 +70,9 @@ def lib():
         L.orc_slice_stress.argtypes = [C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.POINTER(Topology), C.c_int64, C.c_int32,
                                        C.c_int32, C.c_void_p, C.c_void_p]
+        L.orc_slice_stress_bondpairs.argtypes = [C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p, C.POINTER(Topology), C.c_int32,
+                                                 C.c_void_p]
         L.orc_energy_output.argtypes = [C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.POINTER(Topology), C.c_void_p]
         L.orc_lcg_uniform.restype = C.c_double
@@ -265,6 +268,23 @@ def slice_stress(box, ids, types, x, v, step, nslice, thermostat=True, unpbc=Non
     L.orc_slice_stress(C.byref(p), n, _p(ids), _p(types), _p(x), _p(u), _p(v), C.byref(ts), int(step),
                        int(thermostat), int(nslice), _p(pair), _p(bond))
     return pair, bond
+
+
+def slice_stress_bondpairs(box, ids, types, x, nslice, unpbc=None, topo=None):
+    """The bond-pair slice stress table (CMonitor::AddBondPairStress): [nslice,
+    ntypes**3, 9] keyed by the bead types (second head, first head, first tail)."""
+    L = lib()
+    n = len(ids)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    u = np.ascontiguousarray(unpbc, dtype=np.float64) if unpbc is not None else None
+    p = box.params()
+    ts = (topo or Topo()).struct()
+    nt = box.ntypes
+    tab = np.zeros((nslice, nt ** 3, 9))
+    L.orc_slice_stress_bondpairs(C.byref(p), n, _p(ids), _p(types), _p(x), _p(u), C.byref(ts), int(nslice), _p(tab))
+    return tab
 
 
 def slice_add(tab, Lz, idx, z1, z2, F, dx, wgt=1.0):

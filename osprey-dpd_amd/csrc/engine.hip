@@ -7,7 +7,8 @@
 //               (src/CNTCell.cpp:2280-2310, :1704-1757); computes each bead's new cell
 //               key and its rank in that cell
 //   k_brick_scan / k_cell_start   counting-sort prefix sums (bricks, then cells)
-//   k_permute   scatter into the new cell-sorted SoA buffers
+//   k_place / k_permute   counting-sort placement into the new cell-sorted SoA buffers,
+//               beads of a cell in ascending bead index (deterministic layout)
 //   pair kernel DPD pair forces (src/CNTCell.cpp:1144-1555)
 //   k_bonded / k_bonded_sum   bonds and angles (src/Bond.cpp:111-162,
 //               src/BondPair.cpp:138-286), each once, summed into the bonded beads' forces
@@ -80,50 +81,6 @@ __device__ inline void unwrapped_pos(int slot, int bead, const Buffers& S, const
     u[0] = (double)(c[0] + a.x * B.n[0]) * B.wd[0] + (double)p.x;
     u[1] = (double)(c[1] + a.y * B.n[1]) * B.wd[1] + (double)p.y;
     u[2] = (double)(c[2] + a.z * B.n[2]) * B.wd[2] + (double)p.z;
-}
-
-// CBondPair::AddForce (src/BondPair.cpp:138-286) for one angle in FP64.  first = bond
-// (b -> c), second = bond (a -> b); f[0] is the force on c (tail of first), f[1] on the
-// shared bead b, f[2] on a (head of second); pe (may be null) gets
-// CBondPair::AddPotentialEnergy (src/BondPair.cpp:320-378).
-__device__ inline void angle_forces(const double ua[3], const double ub[3], const double uc[3], const double4 par,
-                                    double f[3][3], double* pe)
-{
-    double first[3], second[3];
-    for (int d = 0; d < 3; ++d) { first[d] = ub[d] - uc[d]; second[d] = ua[d] - ub[d]; }
-    const double l1 = sqrt(first[0] * first[0] + first[1] * first[1] + first[2] * first[2]);
-    const double l2 = sqrt(second[0] * second[0] + second[1] * second[1] + second[2] * second[2]);
-    for (int r = 0; r < 3; ++r) f[r][0] = f[r][1] = f[r][2] = 0.0;
-    const double magProduct = l1 * l2;
-    if (!(magProduct > 0.0001)) { if (pe) *pe = 0.0; return; }
-    const double k = par.x, phi0 = par.y, cosPhi0 = par.z, sinPhi0 = par.w;
-    const double b1Dotb2 = first[0] * second[0] + first[1] * second[1] + first[2] * second[2];
-    const double b1b2Overb1Sq = b1Dotb2 / (l1 * l1);
-    const double b1b2Overb2Sq = b1Dotb2 / (l2 * l2);
-    const double cosPhiSq = b1b2Overb1Sq * b1b2Overb2Sq;
-    double forceMag;
-    if (fabs(b1Dotb2) > 0.000001) {
-        const double prefactor = sqrt(1.0 / cosPhiSq - 1.0);
-        forceMag = (phi0 > 0.0) ? k * (cosPhi0 - sinPhi0 / prefactor) / magProduct : k / magProduct;
-    } else {
-        forceMag = k / magProduct;
-    }
-    for (int d = 0; d < 3; ++d) {
-        const double ft = forceMag * (b1b2Overb1Sq * first[d] - second[d]);
-        const double fh = forceMag * (first[d] - b1b2Overb2Sq * second[d]);
-        f[0][d] = ft;
-        f[2][d] = fh;
-        f[1][d] = -ft - fh;
-    }
-    if (pe) {
-        const double cps = (b1Dotb2 * b1Dotb2) / (magProduct * magProduct);
-        const double sinPhi = sqrt(fmax(0.0, 1.0 - cps));
-        double e;
-        if (b1Dotb2 > 0.000001) e = phi0 > 0.0 ? k * (1.0 - sqrt(cps) * cosPhi0 - sinPhi * sinPhi0) : k * (1.0 - sqrt(cps));
-        else if (b1Dotb2 < -0.000001) e = phi0 > 0.0 ? k * (1.0 + sqrt(cps) * cosPhi0 - sinPhi * sinPhi0) : k * (1.0 + sqrt(cps));
-        else e = k * (1.0 - sinPhi0);
-        *pe = e;
-    }
 }
 
 // Sum of the bonded forces on bead index q, gathered from the per-bond and per-angle
@@ -400,15 +357,44 @@ __global__ __launch_bounds__(256) void k_cell_start(int nbricks, int n, const in
     if (brick == nbricks - 1 && lane == 63) cstart[nbricks * 64] = n;
 }
 
-// scatter into the new sorted buffers (recomputes the same advance as k_advance)
-template <int MODE, bool MULTI>
-__global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, float4* __restrict__ frc,
-                                                 float4* __restrict__ vp, const BoxDev* boxes, BoxDev B0,
-                                                 const int* __restrict__ nkey, const int* __restrict__ rank,
-                                                 const int* __restrict__ cstart, int* __restrict__ slot_of)
+// Counting-sort placement, first half: every source slot s writes (bead index, s) to the
+// position its rank gives it inside its new cell.  Ranks come from atomics, so the order
+// inside a cell is arbitrary here; k_permute restores a deterministic one.
+__global__ __launch_bounds__(256) void k_place(int n, const int* __restrict__ bead, const int* __restrict__ nkey,
+                                               const int* __restrict__ rank, const int* __restrict__ cstart,
+                                               int2* __restrict__ tmp)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    tmp[cstart[nkey[s]] + rank[s]] = make_int2(bead[s], s);
+}
+
+// Counting-sort placement, second half: slot d of the new order takes the bead whose index
+// is the (d - start)-th smallest of its cell's beads, so the order inside every cell is
+// ascending bead index, a function of the state alone (reruns, and a state downloaded and
+// uploaded again, lay the beads out identically).  O(m^2) reads of the cell's m entries per
+// slot, from L1.  Then the move itself (recomputing the same advance as k_advance).
+template <int MODE, bool MULTI>
+__global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, float4* __restrict__ frc,
+                                                 float4* __restrict__ vp, const BoxDev* boxes, BoxDev B0,
+                                                 const int* __restrict__ nkey, const int2* __restrict__ tmp,
+                                                 const int* __restrict__ cstart, int* __restrict__ slot_of)
+{
+    const int dst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dst >= n) return;
+    const int2 t0 = tmp[dst];
+    const int k = nkey[t0.y];
+    const int c0 = cstart[k], c1 = cstart[k + 1];
+    int s = t0.y;
+    if (c1 - c0 > 1) {
+        const int r = dst - c0;
+        for (int e = c0; e < c1; ++e) {
+            const int2 te = tmp[e];
+            int below = 0;
+            for (int x = c0; x < c1; ++x) below += tmp[x].x < te.x;
+            if (below == r) { s = te.y; break; }
+        }
+    }
     const float4 p = S.pos[s];
     const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
     const float4 v = S.vel[s];
@@ -426,8 +412,6 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     }
     Moved m;
     advance(o, c, img, d, B, m);
-    const int k = nkey[s];
-    const int dst = cstart[k] + rank[s];
     D.pos[dst] = make_float4(m.o[0], m.o[1], m.o[2], p.w);
     D.vel[dst] = v;
     D.fold[dst] = f;
@@ -619,6 +603,7 @@ struct dpd_engine {
     uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 3 slots past the end
     float4 *frc = nullptr, *vp = nullptr;
     int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
+    int2* place = nullptr;    // counting-sort placement: (bead index, source slot) per new slot
     int *cnt = nullptr, *bcnt = nullptr, *bstart = nullptr, *cstart = nullptr;
     int* flags = nullptr;
     int64_t* dstep = nullptr;
@@ -777,8 +762,10 @@ void sort_step_t(dpd_engine* e)
     hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, e->stream, e->nbricks, e->bcnt, e->bstart);
     hipLaunchKernelGGL(k_cell_start, dim3(grid(e->nbricks, 4)), dim3(256), 0, e->stream, e->nbricks, n, e->bstart,
                        e->cnt, e->cstart);
+    hipLaunchKernelGGL(k_place, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S.bead, e->nkey, e->rank,
+                       e->cstart, e->place);
     hipLaunchKernelGGL((k_permute<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, D, e->frc,
-                       e->vp, e->dboxes, B0, e->nkey, e->rank, e->cstart, e->slot_of);
+                       e->vp, e->dboxes, B0, e->nkey, e->place, e->cstart, e->slot_of);
     e->cur ^= 1;
 }
 
@@ -915,24 +902,33 @@ int finish_obs(dpd_engine* e)
 }
 
 // the steps of one dpd_step call, enqueued on the stream (capturable)
-// steps whose pair launch is bracketed by events: every step, or every
-// kTimingStride-th with DPD_STEP_TIMING_SAMPLED
+// steps whose pair launch is bracketed by events: every step, or the last of every
+// kTimingStride with DPD_STEP_TIMING_SAMPLED (never the first step of a call, whose
+// launch follows the host's enqueue rather than the previous step)
 constexpr int64_t kTimingStride = 16;
-inline bool timed_step(int64_t t, int timing) { return timing == 1 || (timing == 2 && t % kTimingStride == 0); }
+inline bool timed_step(int64_t t, int timing)
+{
+    return timing == 1 || (timing == 2 && t % kTimingStride == kTimingStride - 1);
+}
+inline int64_t timed_steps(int64_t nsteps, int timing)
+{
+    return timing == 1 ? nsteps : (timing == 2 ? nsteps / kTimingStride : 0);
+}
 
 // timing: 0 none, 1 every pair launch, 2 every kTimingStride-th
 int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, int timing)
 {
     int rc;
     launch_zero(e, false);
+    int64_t ti = 0;   // timed launches so far (event pair 2 ti, 2 ti + 1)
     for (int64_t t = 0; t < nsteps; ++t) {
         rc = (t == 0) ? sort_step<kBegin>(e) : sort_step<kEndBegin>(e);
         if (rc) return rc;
         const bool timed = timed_step(t, timing);
-        if (timed) HIPCHK(hipEventRecord(e->pev[2 * t], e->stream));
+        if (timed) HIPCHK(hipEventRecord(e->pev[2 * ti], e->stream));
         bool terms_done = false;
         if ((rc = launch_pair(e, obs && t == nsteps - 1, &terms_done))) return rc;
-        if (timed) HIPCHK(hipEventRecord(e->pev[2 * t + 1], e->stream));
+        if (timed) HIPCHK(hipEventRecord(e->pev[2 * ti++ + 1], e->stream));
         // the bonded sum of every step but the last is added by the next step's k_advance
         if (obs && t == nsteps - 1) launch_bonded<true>(e, terms_done);
         else launch_bonded<false>(e, terms_done, t == nsteps - 1);
@@ -994,7 +990,8 @@ int commit_beads(dpd_engine* e, int64_t step0)
         if ((rc = dalloc(e, &e->img, n))) return rc;
         for (int b = 0; b < 2; ++b) HIPCHK(hipMemsetAsync(e->code[b] + n, 0, sizeof(uint32_t) * kCodePad, e->stream));
         if ((rc = dalloc(e, &e->frc, n)) || (rc = dalloc(e, &e->vp, n)) || (rc = dalloc(e, &e->slot_of, n)) ||
-            (rc = dalloc(e, &e->nkey, n)) || (rc = dalloc(e, &e->rank, n)) || (rc = dalloc(e, &e->gather, 12 * n)))
+            (rc = dalloc(e, &e->nkey, n)) || (rc = dalloc(e, &e->rank, n)) || (rc = dalloc(e, &e->place, n)) ||
+            (rc = dalloc(e, &e->gather, 12 * n)))
             return rc;
         e->n = n;
     }
@@ -1264,7 +1261,8 @@ void dpd_destroy(dpd_engine* e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
     void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->bead[0], e->bead[1], e->img,
-                    e->key[0], e->key[1], e->code[0], e->code[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank, e->cnt, e->bcnt,
+                    e->key[0], e->key[1], e->code[0], e->code[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank,
+                    e->place, e->cnt, e->bcnt,
                     e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
                     e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj, e->hitblk};
@@ -1347,7 +1345,7 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
     int rc;
     e->jumps = 0;
     if (timing) {
-        while ((int64_t)e->pev.size() < 2 * nsteps) {
+        while ((int64_t)e->pev.size() < 2 * timed_steps(nsteps, timing_mode)) {
             hipEvent_t ev;
             HIPCHK(hipEventCreate(&ev));
             e->pev.push_back(ev);
@@ -1402,11 +1400,9 @@ int dpd_step(dpd_engine* e, int64_t nsteps, uint32_t flags)
     if (timing) {
         double tp = 0.0;
         int64_t np = 0;
-        for (int64_t t = 0; t < nsteps; ++t) {
-            if (!timed_step(t, timing_mode)) continue;
-            HIPCHK(hipEventElapsedTime(&ms, e->pev[2 * t], e->pev[2 * t + 1]));
+        for (; np < timed_steps(nsteps, timing_mode); ++np) {
+            HIPCHK(hipEventElapsedTime(&ms, e->pev[2 * np], e->pev[2 * np + 1]));
             tp += ms;
-            ++np;
         }
         e->t_pair = tp;
         e->n_pair = np;
@@ -1428,8 +1424,15 @@ int dpd_compute_forces(dpd_engine* e, int64_t step, uint32_t flags, double* f_ou
         if ((rc = upload_tables(e))) return rc;
     }
     launch_zero(e, true);
+    // the force the next dpd_step starts from is kept aside (the inactive SoA buffers are
+    // scratch between calls) and restored below, so a diagnostic evaluation does not change
+    // the trajectory
+    const size_t f4 = sizeof(float4) * (size_t)e->n;
+    float4* keep = e->fold[e->cur ^ 1];
+    HIPCHK(hipMemcpyAsync(keep, e->frc, f4, hipMemcpyDeviceToDevice, e->stream));
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, step);
-    // the uploaded state is sorted with vp = v: evaluate at it
+    // the uploaded state is sorted with vp = v: evaluate at it (after dpd_step, vp is the
+    // last step's predicted velocity)
     bool terms_done = false;
     if ((rc = launch_pair(e, true, &terms_done))) return rc;
     launch_bonded<true>(e, terms_done);
@@ -1449,6 +1452,7 @@ int dpd_compute_forces(dpd_engine* e, int64_t step, uint32_t flags, double* f_ou
                                e->frc, e->dboxes, e->boxes[0].B, g, g + 3 * n, g + 6 * n, g + 9 * n);
         HIPCHK(hipMemcpyAsync(f_out, g + 9 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, e->stream));
     }
+    HIPCHK(hipMemcpyAsync(e->frc, keep, f4, hipMemcpyDeviceToDevice, e->stream));
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, e->stream, e->dstep, e->step);
     HIPCHK(hipStreamSynchronize(e->stream));
     // totals over all boxes
@@ -1565,6 +1569,32 @@ int64_t dpd_last_timing(dpd_engine* e, double* ms3)
 }
 
 int64_t dpd_get_step(const dpd_engine* e) { return e ? e->step : -1; }
+
+int dpd_time_pair(dpd_engine* e, int32_t nlaunch, double* ms_per_launch)
+{
+    if (!e || !ms_per_launch) return DPD_ERR_PARAM;
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (nlaunch < 1 || nlaunch > 100000) return fail(e, DPD_ERR_PARAM, "nlaunch must be in [1, 100000]");
+    HIPCHK(hipSetDevice(e->device));
+    int rc;
+    // the launches rewrite this step's pair force: the force the next step starts from is
+    // kept in the inactive SoA buffer (scratch between calls) and restored afterwards
+    const size_t f4 = sizeof(float4) * (size_t)e->n;
+    float4* keep = e->fold[e->cur ^ 1];
+    HIPCHK(hipMemcpyAsync(keep, e->frc, f4, hipMemcpyDeviceToDevice, e->stream));
+    bool terms_done = false;
+    if ((rc = launch_pair(e, false, &terms_done))) return rc;   // untimed first launch
+    HIPCHK(hipEventRecord(e->ev[0], e->stream));
+    for (int32_t i = 0; i < nlaunch; ++i)
+        if ((rc = launch_pair(e, false, &terms_done))) return rc;
+    HIPCHK(hipEventRecord(e->ev[1], e->stream));
+    HIPCHK(hipMemcpyAsync(e->frc, keep, f4, hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+    *ms_per_launch = (double)ms / nlaunch;
+    return DPD_OK;
+}
 
 int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel)
 {
@@ -1736,6 +1766,9 @@ int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_o
     a.b0 = 0;
     for (int32_t b = 0; b < box; ++b) a.b0 += (int)e->boxes[b].bh.size();
     a.b1 = a.b0 + (int)h.bh.size();
+    a.ang_bead = e->ang_bead;
+    a.ang_par = e->ang_par;
+    a.a0 = a.a1 = 0;
     a.nslice = nslices;
     a.ntypes = nt;
     a.nidx = nt * nt;
@@ -1743,6 +1776,42 @@ int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_o
     if (pair_out) HIPCHK(hipMemcpyAsync(pair_out, e->slice, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
     if (bond_out)
         HIPCHK(hipMemcpyAsync(bond_out, e->slice + ntab, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DPD_OK;
+}
+
+int dpd_slice_stress_bondpairs(dpd_engine* e, int32_t box, int32_t nslices, double* out)
+{
+    if (!e || !out) return DPD_ERR_PARAM;
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (box < 0 || box >= (int32_t)e->boxes.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    if (nslices < 1 || nslices > 65536) return fail(e, DPD_ERR_PARAM, "nslices must be in [1, 65536]");
+    HIPCHK(hipSetDevice(e->device));
+    const BoxHost& h = e->boxes[box];
+    const int nt = h.B.ntypes;
+    const size_t ntab = (size_t)nslices * nt * nt * nt * 9;
+    if (ntab > e->slice_cap) {
+        int rc;
+        if ((rc = dalloc(e, &e->slice, ntab))) { e->slice_cap = 0; return rc; }
+        e->slice_cap = ntab;
+    }
+    HIPCHK(hipMemsetAsync(e->slice, 0, sizeof(double) * ntab, e->stream));
+    SliceArgs a{};
+    a.B = h.B;
+    a.pos = e->pos[e->cur];
+    a.img = e->img;
+    a.key = e->key[e->cur];
+    a.slot_of = e->slot_of;
+    a.ang_bead = e->ang_bead;
+    a.ang_par = e->ang_par;
+    a.a0 = 0;
+    for (int32_t b = 0; b < box; ++b) a.a0 += (int)e->boxes[b].pf.size();
+    a.a1 = e->nangles > 0 ? a.a0 + (int)h.pf.size() : a.a0;
+    a.nslice = nslices;
+    a.ntypes = nt;
+    a.nidx = nt * nt * nt;
+    HIPCHK(launch_slice_bondpairs(a, e->slice, e->stream));
+    HIPCHK(hipMemcpyAsync(out, e->slice, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return DPD_OK;
 }

@@ -189,6 +189,50 @@ __device__ inline void advance(const float o[3], const int c[3], const int img[3
     }
 }
 
+// CBondPair::AddForce (src/BondPair.cpp:138-286) for one angle in FP64.  first = bond
+// (b -> c), second = bond (a -> b); f[0] is the force on c (tail of first), f[1] on the
+// shared bead b, f[2] on a (head of second); pe (may be null) gets
+// CBondPair::AddPotentialEnergy (src/BondPair.cpp:320-378).
+__device__ inline void angle_forces(const double ua[3], const double ub[3], const double uc[3], const double4 par,
+                                    double f[3][3], double* pe)
+{
+    double first[3], second[3];
+    for (int d = 0; d < 3; ++d) { first[d] = ub[d] - uc[d]; second[d] = ua[d] - ub[d]; }
+    const double l1 = sqrt(first[0] * first[0] + first[1] * first[1] + first[2] * first[2]);
+    const double l2 = sqrt(second[0] * second[0] + second[1] * second[1] + second[2] * second[2]);
+    for (int r = 0; r < 3; ++r) f[r][0] = f[r][1] = f[r][2] = 0.0;
+    const double magProduct = l1 * l2;
+    if (!(magProduct > 0.0001)) { if (pe) *pe = 0.0; return; }
+    const double k = par.x, phi0 = par.y, cosPhi0 = par.z, sinPhi0 = par.w;
+    const double b1Dotb2 = first[0] * second[0] + first[1] * second[1] + first[2] * second[2];
+    const double b1b2Overb1Sq = b1Dotb2 / (l1 * l1);
+    const double b1b2Overb2Sq = b1Dotb2 / (l2 * l2);
+    const double cosPhiSq = b1b2Overb1Sq * b1b2Overb2Sq;
+    double forceMag;
+    if (fabs(b1Dotb2) > 0.000001) {
+        const double prefactor = sqrt(1.0 / cosPhiSq - 1.0);
+        forceMag = (phi0 > 0.0) ? k * (cosPhi0 - sinPhi0 / prefactor) / magProduct : k / magProduct;
+    } else {
+        forceMag = k / magProduct;
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double ft = forceMag * (b1b2Overb1Sq * first[d] - second[d]);
+        const double fh = forceMag * (first[d] - b1b2Overb2Sq * second[d]);
+        f[0][d] = ft;
+        f[2][d] = fh;
+        f[1][d] = -ft - fh;
+    }
+    if (pe) {
+        const double cps = (b1Dotb2 * b1Dotb2) / (magProduct * magProduct);
+        const double sinPhi = sqrt(fmax(0.0, 1.0 - cps));
+        double e;
+        if (b1Dotb2 > 0.000001) e = phi0 > 0.0 ? k * (1.0 - sqrt(cps) * cosPhi0 - sinPhi * sinPhi0) : k * (1.0 - sqrt(cps));
+        else if (b1Dotb2 < -0.000001) e = phi0 > 0.0 ? k * (1.0 + sqrt(cps) * cosPhi0 - sinPhi * sinPhi0) : k * (1.0 + sqrt(cps));
+        else e = k * (1.0 - sinPhi0);
+        *pe = e;
+    }
+}
+
 // wave-level helpers (64 lanes)
 __device__ inline double wave_sum(double v)
 {

@@ -1,5 +1,6 @@
 // slice_stress.hip -- z-slice stress profile of one box (CMonitor::AddBeadStress,
-// src/Monitor.cpp:1900-2104, and CMonitor::AddBondStress, :2283-2518).
+// src/Monitor.cpp:1900-2104, CMonitor::AddBondStress, :2283-2518, and
+// CMonitor::AddBondPairStress, :2520-2739).
 //
 // The reference zeroes its slice tables every step and adds F (x) dx of every pair (and,
 // when a stress analysis asks for it, of every bond) into the z-slices spanned by the two
@@ -152,7 +153,59 @@ __global__ __launch_bounds__(256) void k_slice_bonds(SliceArgs A, double* __rest
               wrapped_z(A, st), F, d, 1.0);
 }
 
+__device__ inline void unwrapped(const SliceArgs& A, int bead, int slot, double u[3])
+{
+    const BoxDev& B = A.B;
+    int c[3];
+    key_cell(A.key[slot] - B.koff, B, c);
+    const float4 p = A.pos[slot];
+    const int4 a = A.img[bead];
+    u[0] = (double)(c[0] + a.x * B.n[0]) * B.wd[0] + (double)p.x;
+    u[1] = (double)(c[1] + a.y * B.n[1]) * B.wd[1] + (double)p.y;
+    u[2] = (double)(c[2] + a.z * B.n[2]) * B.wd[2] + (double)p.z;
+}
+
+// CMonitor::AddBondPairStress (src/Monitor.cpp:2520-2739), thread per bond pair: the bond
+// pair's forces (FP64, as in the step), then its three bead pairs first tail -> first
+// head -> second head -> first tail, each adding 0.33333333 (F[j+1] - F[j]) (x)
+// (u[j+1] - u[j]) over the slices it spans, keyed by the bead types (second head, first
+// head, first tail) where the reference keys by bond-pair type.
+__global__ __launch_bounds__(256) void k_slice_bondpairs(SliceArgs A, double* __restrict__ out)
+{
+    const int q = A.a0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.a1) return;
+    const int4 abc = A.ang_bead[q];
+    const int sa = A.slot_of[abc.x], sb = A.slot_of[abc.y], sc = A.slot_of[abc.z];
+    double ua[3], ub[3], uc[3], f[3][3];
+    unwrapped(A, abc.x, sa, ua);
+    unwrapped(A, abc.y, sb, ub);
+    unwrapped(A, abc.z, sc, uc);
+    angle_forces(ua, ub, uc, A.ang_par[q], f, nullptr);   // f[0] on c, f[1] on b, f[2] on a
+    const double* u[4] = {uc, ub, ua, uc};
+    const double* F[4] = {f[0], f[1], f[2], f[0]};
+    const int slot[4] = {sc, sb, sa, sc};
+    const int nt = A.ntypes;
+    const int idx = (bead_type(__float_as_int(A.pos[sa].w)) * nt + bead_type(__float_as_int(A.pos[sb].w))) * nt +
+                    bead_type(__float_as_int(A.pos[sc].w));
+    const double Lz = (double)A.B.n[2] * A.B.wd[2];
+    for (int j = 0; j < 3; ++j) {
+        double dx[3], fj[3];
+        for (int d = 0; d < 3; ++d) {
+            dx[d] = u[j + 1][d] - u[j][d];
+            fj[d] = 0.33333333 * (F[j + 1][d] - F[j][d]);
+        }
+        slice_add(out, A.nslice, Lz, idx, nt * nt * nt, wrapped_z(A, slot[j + 1]), wrapped_z(A, slot[j]), fj, dx, 1.0);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_slice_bondpairs(const SliceArgs& a, double* tab, hipStream_t s)
+{
+    if (a.a1 > a.a0)
+        hipLaunchKernelGGL(k_slice_bondpairs, dim3((a.a1 - a.a0 + 255) / 256), dim3(256), 0, s, a, tab);
+    return hipGetLastError();
+}
 
 hipError_t launch_slice_stress(const SliceArgs& a, double* pair_tab, double* bond_tab, hipStream_t s)
 {

@@ -19,6 +19,9 @@ struct SliceArgs {
     const int2* bond_bead;    // (head, tail) bead index per bond
     const double2* bond_par;  // (k, l0) per bond
     int b0, b1;               // this box's bonds
+    const int4* ang_bead;     // (second head, first head, first tail) bead index per bond pair
+    const double4* ang_par;   // (k, phi0, cos phi0, sin phi0) per bond pair
+    int a0, a1;               // this box's bond pairs
     int nslice;
     int ntypes;
     int nidx;                 // ntypes * ntypes
@@ -30,5 +33,7 @@ constexpr size_t kSliceLdsBytes = 32768;
 constexpr int kSliceBlocks = 1024;
 
 hipError_t launch_slice_stress(const SliceArgs& a, double* pair_tab, double* bond_tab, hipStream_t s);
+// bond pairs: tab[nslice][ntypes^3][9] (cleared by the caller)
+hipError_t launch_slice_bondpairs(const SliceArgs& a, double* tab, hipStream_t s);
 
 }  // namespace dpd

@@ -68,6 +68,7 @@ SIGNATURES = {
     "dpd_toggle": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "dpd_last_timing": (C.c_int64, [_P, _P]),
     "dpd_get_step": (C.c_int64, [_P]),
+    "dpd_time_pair": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     "dpd_set_pair_kernel": (C.c_int, [_P, C.c_int32]),
     "dpd_get_pair_kernel": (C.c_int32, [_P]),
     "dpd_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P]),
@@ -81,6 +82,7 @@ SIGNATURES = {
     "dpd_download_box": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     "dpd_set_box_interactions": (C.c_int, [_P, C.c_int32, _P, _P]),
     "dpd_slice_stress": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P]),
+    "dpd_slice_stress_bondpairs": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
     "dpd_zero_total_momentum": (C.c_int, [_P, C.c_int32]),
     "dpd_renormalise_momenta": (C.c_int, [_P, C.c_int32]),
 }
@@ -221,6 +223,13 @@ class Engine:
         n = lib().dpd_last_timing(self.h, _ptr(ms))
         return int(n), ms
 
+    def time_pair(self, nlaunch=20):
+        """Average duration (ms) of the step's pair-force launch on the current state, from
+        `nlaunch` back-to-back launches between two HIP events (dpd_time_pair)."""
+        ms = C.c_double()
+        self._check(lib().dpd_time_pair(self.h, int(nlaunch), C.byref(ms)))
+        return ms.value
+
     @property
     def step_count(self):
         return lib().dpd_get_step(self.h)
@@ -258,6 +267,18 @@ class Engine:
         bond = np.zeros((int(nslices), nt * nt, 9)) if bonds else None
         self._check(lib().dpd_slice_stress(self.h, int(box), int(nslices), _ptr(pair), _ptr(bond)))
         return pair, bond
+
+    def slice_stress_bondpairs(self, nslices, box=0):
+        """Bond-pair stress profile (CMonitor::AddBondPairStress, src/Monitor.cpp:2520-2739):
+        [nslices, ntypes**3, 9] keyed by the bead types (second head, first head, first tail)."""
+        return _slice_bondpairs(self, nslices, box)
+
+
+def _slice_bondpairs(eng, nslices, box):
+    nt = eng._ntypes(box)
+    tab = np.zeros((int(nslices), nt ** 3, 9))
+    eng._check(lib().dpd_slice_stress_bondpairs(eng.h, int(box), int(nslices), _ptr(tab)))
+    return tab
 
 
 def _params(cells, a, gamma, dt, lam, kT, seed, widths, device, pair_kernel, keep):

@@ -72,6 +72,37 @@ def test_pair_timing_every_launch_or_sampled():
     assert n == 20 and 0 < ms[1] < ms[0] and abs(ms[2] - (ms[0] - ms[1])) < 1e-9
     e.step(40, timing="sampled")
     n2, ms2 = e.last_timing()
-    assert n2 == 3                                  # steps 0, 16, 32 of the call
+    assert n2 == 2                                  # steps 15 and 31 of the call
     say(f"  pair per launch: all {ms[1] / n * 1e3:.1f} us, sampled {ms2[1] / n2 * 1e3:.1f} us")
     assert 0.5 < (ms2[1] / n2) / (ms[1] / n) < 2.0
+
+
+@pytest.mark.parametrize("kernel", [D.PAIR_HITLIST, D.PAIR_FULLSHELL])
+def test_pair_launch_timing_and_diagnostics_leave_the_state(kernel):
+    """dpd_time_pair (back-to-back launches between two events) and dpd_compute_forces after
+    steps are diagnostics: the trajectory continues exactly as in an engine that ran neither.
+    With chains, so that the hit-list launch carries the bonded terms."""
+    from helpers import chain_state
+    cells = (12, 12, 12)
+    ids, t, x, u, v, (h, tl, fi, se) = chain_state(cells, 40, 16, 4544, seed=12)
+    k, l0 = np.full(len(h), 128.0), np.full(len(h), 0.5)
+    mod, ph = np.full(len(fi), 20.0), np.zeros(len(fi))
+    A3 = np.array([[25.0, 25.0, 25.0], [25.0, 10.0, 25.0], [25.0, 25.0, 25.0]])
+    G3 = np.full((3, 3), 4.5)
+    runs = []
+    for diag in (False, True):
+        e = D.Engine(cells, A3, G3, seed=6, pair_kernel=kernel)
+        e.upload(ids, t, x, v, unpbc=u)
+        e.upload_topology(h, tl, k, l0, fi, se, mod, ph)
+        e.step(7)
+        if diag:
+            ms = e.time_pair(5)
+            say(f"  kernel {kernel}: {ms * 1e3:.1f} us per pair launch")
+            assert ms > 0
+            e.compute_forces(step=3)
+            e.compute_forces(step=7, thermostat=False)
+        e.step(5, observe=True)
+        runs.append((e.download(), e.observables()))
+    for a, b in zip(runs[0][0], runs[1][0]):
+        assert np.array_equal(a, b)
+    assert runs[0][1]["kT"] == runs[1][1]["kT"]

@@ -482,3 +482,73 @@ def test_oracle_commands_match_reference():
     assert six_digit_close(u, ref["u"]).all()
     assert six_digit_close(v, ref["v"]).all()
     assert six_digit_close(f, ref["f"]).all()
+
+
+# ---- bond-pair slice stress (CMonitor::AddBondPairStress) ----
+
+def _angle_only(cells, ntypes, ids, types, x, topo):
+    """Forces of the bond pairs alone (a = gamma = 0, bond springs of zero strength)."""
+    z = np.zeros((ntypes, ntypes))
+    box = O.Box(cells, z, z, seed=1)
+    t0 = O.Topo(topo.head, topo.tail, np.zeros(len(topo.head)), topo.l0, topo.first, topo.second,
+                topo.modulus, topo.phi0)
+    f, _, _ = O.forces(box, ids, types, x, np.zeros_like(x), step=1, topo=t0)
+    return box, t0, f
+
+
+def test_bondpair_slices_closed_form():
+    """One bond pair a - b - c (types 0, 1, 2) spanning three of four slices (width 2): the
+    pair c -> b lies in slice 0; b -> a spans 1.5 .. 5.0 (slice 0 gets 0.5/3.5 of it,
+    slice 1 2/3.5, slice 2 1/3.5); a -> c spans 1.0 .. 5.0 (1/4, 2/4, 1/4).  Each pair adds
+    (F[j+1] - F[j]) / 3 (x) (u[j+1] - u[j]), F the bond pair's force on each bead."""
+    cells = (8, 8, 8)
+    x = np.array([[3.0, 4.0, 5.0],      # a: second bond's head
+                  [3.4, 3.8, 1.5],      # b: shared (first bond's head)
+                  [3.9, 3.3, 1.0]])     # c: first bond's tail
+    ids = np.array([1, 2, 3], np.int32)
+    types = np.array([0, 1, 2], np.int32)
+    topo = O.Topo(head=[0, 1], tail=[1, 2], k=[0.0, 0.0], l0=[0.5, 0.5], first=[1], second=[0],
+                  modulus=[17.0], phi0=[np.radians(40.0)])
+    box, t0, f = _angle_only(cells, 3, ids, types, x, topo)
+    assert np.abs(f).max() > 1.0
+    tab = O.slice_stress_bondpairs(box, ids, types, x, 4, topo=t0)
+    idx = (0 * 3 + 1) * 3 + 2
+    assert np.abs(np.delete(tab, idx, axis=1)).max() == 0.0
+
+    def T(p, q):   # bead p -> bead q
+        return np.outer(0.33333333 * (f[q] - f[p]), x[q] - x[p]).ravel()
+    a, b, c = 0, 1, 2
+    want = np.zeros((4, 9))
+    want[0] = T(c, b) + (0.5 / 3.5) * T(b, a) + 0.25 * T(a, c)
+    want[1] = (2.0 / 3.5) * T(b, a) + 0.5 * T(a, c)
+    want[2] = (1.0 / 3.5) * T(b, a) + 0.25 * T(a, c)
+    assert np.allclose(tab[:, idx, :], want, rtol=1e-12, atol=1e-12)
+    # a plausible mistake fails: the factor of a third dropped, or the pair order reversed
+    # in the force but not in the separation
+    assert not np.allclose(tab[:, idx, :], 3 * want)
+    bad = np.zeros((4, 9))
+    bad[0] = -T(c, b)
+    assert not np.allclose(tab[:, idx, :], want + 2 * bad)
+
+
+def test_bondpair_slices_sum_to_the_bondpair_virial():
+    """Summed over slices and types, the table is the bond pairs' virial sum_i F_i (x) u_i
+    (times 0.99999999, the reference's 0.33333333 times three) -- on the reference's branched
+    lipid snapshot with its bond pairs (phi0 = 0, 30 and 60 degrees)."""
+    from dpd_amd import host as H
+    inp = H.Input(os.path.join(GOLDEN, "lipidphi8", "case.dmpci"))
+    (h, t, k, l0), (fi, se, mo, ph) = inp.topology()
+    d = load_beads("lipidphi8", "in")
+    topo = O.Topo(h, t, k, l0, fi, se, mo, ph)
+    # wrapped positions consistent with the unwrapped ones (the snapshot's writer moved beads
+    # within 0.001 of a face without moving their unwrapped coordinate; the slice split
+    # takes heights from the one and separations from the other)
+    x = np.mod(d["u"], np.array(inp.cells, float))
+    box, t0, f = _angle_only(inp.cells, len(inp.a), d["id"], d["type"], x, topo)
+    tab = O.slice_stress_bondpairs(box, d["id"], d["type"], x, 16, unpbc=d["u"], topo=t0)
+    f, _, _ = O.forces(box, d["id"], d["type"], x, np.zeros_like(x), step=1, unpbc=d["u"], topo=t0)
+    vir = np.einsum("ia,ib->ab", f, d["u"]).ravel()
+    assert np.allclose(tab.sum(axis=(0, 1)), 0.99999999 * vir, rtol=1e-9, atol=1e-9)
+    assert np.abs(vir).max() > 1.0
+    # the profile sees where the lipids are
+    assert (np.abs(tab).sum(axis=(1, 2)) > 0).sum() >= 8
