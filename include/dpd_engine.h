@@ -225,6 +225,26 @@ int dpd_renormalise_momenta(dpd_engine* e, int32_t box);
 int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_out,
                      double* bond_out);
 
+/* dpd_slice_stress without device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The device tables live in a static scratch of
+ * 2^19 doubles (4 MiB) that each GPU holds until the process ends: sizes with
+ * 2 * nslices * ntypes^2 * 9 above 2^19 are refused with DPD_ERR_PARAM.  The scratch is
+ * cleared by a kernel on every call (no memset node); all calls on one GPU share it, so
+ * calls that overlap on one GPU race.  Same arguments and results as dpd_slice_stress. */
+int dpd_slice_stress_noalloc(dpd_engine* e, int32_t box, int32_t nslices, double* pair_out,
+                             double* bond_out);
+
+/* The potential energy the reference writes with ToggleEnergyOutput
+ * (CCNTCell::UpdateTotalEnergy, src/CNTCell.cpp:2909-3229; src/Bond.cpp:279-288;
+ * src/BondPair.cpp:320-378) at the current positions of box `box`, FP64: out3[0] the pair
+ * sum in the reference's order -- every pair inside a cell, a pair across cells only from
+ * the cell whose 13-cell half shell holds the other and only when the neighbour's id is
+ * larger, so about half of the cross-cell pairs are left out, as in the reference --,
+ * out3[1] the bonds 1/2 k (L - l0)^2, out3[2] the bond pairs.  (dpd_obs.pe_pair is the
+ * true pair energy.) */
+int dpd_energy_output(dpd_engine* e, int32_t box, double* out3);
+
 /* Bond-pair stress profile across `nslices` equal z-slices of box `box` (replaces the
  * per-step tables of CMonitor::AddBondPairStress, src/Monitor.cpp:2520-2739, which
  * CPolymer::AddBondPairForces fills for every bond pair, src/Polymer.cpp:703).  Each bond

@@ -56,6 +56,12 @@ int  dpd_input_topology(const dpd_input* in, int32_t* head, int32_t* tail, doubl
  * two names match its beads in either order (src/InputData.cpp:1489-1498); the bond
  * slice-stress tables of the reference are keyed by it. */
 int  dpd_input_bond_types(const dpd_input* in, int32_t* count, int32_t* head_type, int32_t* tail_type);
+/* Write the initial-state summary dmpcis.<runId> the reference writes before its first
+ * step (CInitialState::SerializeAsciiFile, src/InitialState.cpp:1153-1347): numbers of
+ * polymers, beads, bonds and bond pairs created, polymers per type, bead types (radius,
+ * conservative and dissipative rows), bond types, bond-pair types (modulus, phi0 in
+ * radians) and each polymer type's fraction and bead, bond and bond-pair types. */
+int  dpd_input_write_summary(const dpd_input* in, const char* path);
 /* Engine parameters of this input (pointers stay valid while `in` lives). */
 int  dpd_input_params(const dpd_input* in, int32_t device, dpd_params* out);
 /* Starting state for State random / lamella: positions (wrapped and unwrapped) and

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1197,6 +1198,83 @@ int stage_topology(dpd_engine* e, int32_t box, int64_t nbonds, const int32_t* he
 
 }  // namespace
 
+// Static scratch of dpd_slice_stress_noalloc: the pair and bond slice tables of one call,
+// held by every device the library runs on until the process ends.  Its host-side name is
+// a stand-in, not a device address: scratch_ptr looks up each device's own copy once.
+constexpr size_t kSliceScratchDoubles = (size_t)1 << 19;   // 4 MiB
+static __device__ __align__(256) double scratch[kSliceScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// the device address of this engine's device's copy of `scratch` (looked up on the first
+// call on that device, with that device current)
+int scratch_ptr(dpd_engine* e, double** out)
+{
+    if (e->device < 0 || e->device >= kMaxDevices) return fail(e, DPD_ERR_PARAM, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[e->device]) {
+        HIPCHK(hipSetDevice(e->device));
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(e, DPD_ERR_HIP, "hipGetSymbolAddress returned no address for the slice scratch");
+        g_scratch[e->device] = (double*)p;
+    }
+    *out = g_scratch[e->device];
+    return DPD_OK;
+}
+
+int slice_check(dpd_engine* e, int32_t box, int32_t nslices, size_t* ntab)
+{
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (box < 0 || box >= (int32_t)e->boxes.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    if (nslices < 1 || nslices > 65536) return fail(e, DPD_ERR_PARAM, "nslices must be in [1, 65536]");
+    HIPCHK(hipSetDevice(e->device));
+    const int nt = e->boxes[box].B.ntypes;
+    *ntab = (size_t)nslices * nt * nt * 9;
+    return DPD_OK;
+}
+
+// pair and bond tables of box `box` into the cleared device tables tab[0 .. 2 ntab), then
+// to the host outputs (either may be null)
+int slice_tables(dpd_engine* e, int32_t box, int32_t nslices, double* tab, size_t ntab, double* pair_out,
+                 double* bond_out)
+{
+    const BoxHost& h = e->boxes[box];
+    const int nt = h.B.ntypes;
+    SliceArgs a{};
+    a.B = h.B;
+    a.pos = e->pos[e->cur];
+    a.vp = e->vp;
+    a.img = e->img;
+    a.key = e->key[e->cur];
+    a.cstart = e->cstart;
+    a.table = e->table + (size_t)box * kTableStride;
+    a.step = e->dstep;
+    a.slot_of = e->slot_of;
+    a.bond_bead = e->bond_bead;
+    a.bond_par = e->bond_par;
+    a.b0 = 0;
+    for (int32_t b = 0; b < box; ++b) a.b0 += (int)e->boxes[b].bh.size();
+    a.b1 = a.b0 + (int)h.bh.size();
+    a.ang_bead = e->ang_bead;
+    a.ang_par = e->ang_par;
+    a.a0 = a.a1 = 0;
+    a.nslice = nslices;
+    a.ntypes = nt;
+    a.nidx = nt * nt;
+    HIPCHK(launch_slice_stress(a, pair_out ? tab : nullptr, bond_out ? tab + ntab : nullptr, e->stream));
+    if (pair_out) HIPCHK(hipMemcpyAsync(pair_out, tab, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
+    if (bond_out) HIPCHK(hipMemcpyAsync(bond_out, tab + ntab, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DPD_OK;
+}
+
+}  // namespace
+
 // ------------------------------------------------------------------ C ABI ------
 
 extern "C" {
@@ -1738,20 +1816,50 @@ int dpd_renormalise_momenta(dpd_engine* e, int32_t box) { return adjust_momenta(
 int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_out, double* bond_out)
 {
     if (!e) return DPD_ERR_PARAM;
-    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
-    if (box < 0 || box >= (int32_t)e->boxes.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
-    if (nslices < 1 || nslices > 65536) return fail(e, DPD_ERR_PARAM, "nslices must be in [1, 65536]");
-    HIPCHK(hipSetDevice(e->device));
-    const BoxHost& h = e->boxes[box];
-    const int nt = h.B.ntypes;
-    const size_t ntab = (size_t)nslices * nt * nt * 9;
+    int rc;
+    size_t ntab = 0;
+    if ((rc = slice_check(e, box, nslices, &ntab))) return rc;
     if (2 * ntab > e->slice_cap) {
-        int rc;
         if ((rc = dalloc(e, &e->slice, 2 * ntab))) { e->slice_cap = 0; return rc; }
         e->slice_cap = 2 * ntab;
     }
     HIPCHK(hipMemsetAsync(e->slice, 0, sizeof(double) * 2 * ntab, e->stream));
-    SliceArgs a;
+    return slice_tables(e, box, nslices, e->slice, ntab, pair_out, bond_out);
+}
+
+int dpd_slice_stress_noalloc(dpd_engine* e, int32_t box, int32_t nslices, double* pair_out, double* bond_out)
+{
+    if (!e) return DPD_ERR_PARAM;
+    int rc;
+    size_t ntab = 0;
+    if ((rc = slice_check(e, box, nslices, &ntab))) return rc;
+    if (2 * ntab > kSliceScratchDoubles)
+        return fail(e, DPD_ERR_PARAM, "slice tables of " + std::to_string(2 * ntab) + " doubles exceed the " +
+                                          std::to_string(kSliceScratchDoubles) + " of the static scratch");
+    double* tab = nullptr;
+    if ((rc = scratch_ptr(e, &tab))) return rc;
+    // the scratch keeps the last call's tables: clear the part this call uses (a kernel,
+    // not a memset node, so that the call may be captured)
+    const int nclr = (int)(2 * ntab);
+    hipLaunchKernelGGL(k_zero, dim3(grid(std::max(nclr, 1), 256)), dim3(256), 0, e->stream, tab, nclr, e->flags, 0);
+    HIPCHK(hipGetLastError());
+    return slice_tables(e, box, nslices, tab, ntab, pair_out, bond_out);
+}
+
+int dpd_energy_output(dpd_engine* e, int32_t box, double* out3)
+{
+    if (!e || !out3) return DPD_ERR_PARAM;
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    if (box < 0 || box >= (int32_t)e->boxes.size()) return fail(e, DPD_ERR_PARAM, "box index out of range");
+    HIPCHK(hipSetDevice(e->device));
+    const BoxHost& h = e->boxes[box];
+    if (e->slice_cap < 3) {
+        int rc;
+        if ((rc = dalloc(e, &e->slice, 3))) { e->slice_cap = 0; return rc; }
+        e->slice_cap = 3;
+    }
+    HIPCHK(hipMemsetAsync(e->slice, 0, sizeof(double) * 3, e->stream));
+    SliceArgs a{};
     a.B = h.B;
     a.pos = e->pos[e->cur];
     a.vp = e->vp;
@@ -1759,23 +1867,18 @@ int dpd_slice_stress(dpd_engine* e, int32_t box, int32_t nslices, double* pair_o
     a.key = e->key[e->cur];
     a.cstart = e->cstart;
     a.table = e->table + (size_t)box * kTableStride;
-    a.step = e->dstep;
     a.slot_of = e->slot_of;
     a.bond_bead = e->bond_bead;
     a.bond_par = e->bond_par;
-    a.b0 = 0;
-    for (int32_t b = 0; b < box; ++b) a.b0 += (int)e->boxes[b].bh.size();
-    a.b1 = a.b0 + (int)h.bh.size();
     a.ang_bead = e->ang_bead;
     a.ang_par = e->ang_par;
-    a.a0 = a.a1 = 0;
-    a.nslice = nslices;
-    a.ntypes = nt;
-    a.nidx = nt * nt;
-    HIPCHK(launch_slice_stress(a, pair_out ? e->slice : nullptr, bond_out ? e->slice + ntab : nullptr, e->stream));
-    if (pair_out) HIPCHK(hipMemcpyAsync(pair_out, e->slice, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
-    if (bond_out)
-        HIPCHK(hipMemcpyAsync(bond_out, e->slice + ntab, sizeof(double) * ntab, hipMemcpyDeviceToHost, e->stream));
+    a.b0 = a.a0 = 0;
+    for (int32_t b = 0; b < box; ++b) { a.b0 += (int)e->boxes[b].bh.size(); a.a0 += (int)e->boxes[b].pf.size(); }
+    a.b1 = e->nbonds > 0 ? a.b0 + (int)h.bh.size() : a.b0;
+    a.a1 = e->nangles > 0 ? a.a0 + (int)h.pf.size() : a.a0;
+    a.ntypes = h.B.ntypes;
+    HIPCHK(launch_energy_output(a, e->slice, e->stream));
+    HIPCHK(hipMemcpyAsync(out3, e->slice, sizeof(double) * 3, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return DPD_OK;
 }

@@ -71,15 +71,6 @@ void maxwell(std::mt19937_64& rng, double kT, size_t n, double* v)
     for (size_t i = 0; i < 3 * n; ++i) v[i] *= scale;
 }
 
-// parent bead of each polymer bead (the head of the bond whose tail it is), -1 for roots
-std::vector<int64_t> parents(const System& s)
-{
-    std::vector<int64_t> p(s.id.size(), -1);
-    for (size_t b = 0; b < s.bond_head.size(); ++b)
-        if (p[s.bond_tail[b]] < 0 && s.bond_head[b] < s.bond_tail[b]) p[s.bond_tail[b]] = s.bond_head[b];
-    return p;
-}
-
 void wrap(const InputData& d, size_t n, const double* u, double* x)
 {
     for (size_t i = 0; i < n; ++i)
@@ -89,30 +80,6 @@ void wrap(const InputData& d, size_t n, const double* u, double* x)
             if (y >= L) y = 0.0;
             x[3 * i + a] = y;
         }
-}
-
-// random state: free beads uniform; polymer beads a random unit vector times the bond
-// length from their parent (a random walk along the polymer)
-void build_random(const dpd_input* in, std::mt19937_64& rng, double* u)
-{
-    const InputData& d = in->d;
-    const System& s = in->s;
-    std::uniform_real_distribution<double> U(0.0, 1.0);
-    std::normal_distribution<double> N(0.0, 1.0);
-    const std::vector<int64_t> par = parents(s);
-    std::vector<double> blen(s.id.size(), 0.5);
-    for (size_t b = 0; b < s.bond_head.size(); ++b)
-        if (par[s.bond_tail[b]] == s.bond_head[b]) blen[s.bond_tail[b]] = s.bond_l0[b] > 0 ? s.bond_l0[b] : 0.5;
-    for (size_t i = 0; i < s.id.size(); ++i) {
-        if (par[i] < 0) {
-            for (int a = 0; a < 3; ++a) u[3 * i + a] = U(rng) * box_len(d, a);
-        } else {
-            double r[3], nn = 0;
-            for (int a = 0; a < 3; ++a) { r[a] = N(rng); nn += r[a] * r[a]; }
-            nn = std::sqrt(nn);
-            for (int a = 0; a < 3; ++a) u[3 * i + a] = u[3 * par[i] + a] + blen[i] * r[a] / nn;
-        }
-    }
 }
 
 // lamella: the named polymer in two leaflets about the mid-plane normal to `Normal`
@@ -194,6 +161,91 @@ bool build_lamella(const dpd_input* in, std::mt19937_64& rng, double* u, std::st
         i = j;
     }
     return true;
+}
+
+// The reference's uniform stream for building states: its 64-bit LCG (CCNTCell::Randf,
+// src/CNTCell.cpp:2454-2465) started from |RNGSeed| (CCNTCell::SetRNGSeed, :134-137)
+struct RefUniform {
+    uint64_t state;
+    double next()
+    {
+        state = 6364136223846793005ull * state + 1ull;
+        return (double)(uint32_t)(state >> 32) * (1.0 / 4294967296.0);
+    }
+};
+
+// CBuilder::MaxwellDist (src/Builder.cpp:104-129): a table of max(n, 10^7) speeds, each
+// from the factor the previous one left; the last entry repeats the one before it
+std::vector<double> maxwell_table(int64_t n)
+{
+    const int64_t m = std::max<int64_t>(n, 10000000);
+    std::vector<double> t((size_t)m, 0.0);
+    const double dm = (double)m;
+    double factor = 1.0;
+    for (int64_t j = 0; j < m - 1; ++j) {
+        const double vel = std::sqrt(2.0 * std::log(dm * factor / (dm - factor)));
+        factor = std::exp(0.5 * vel * vel);
+        t[(size_t)j] = vel;
+    }
+    t[(size_t)(m - 1)] = t[(size_t)(m - 2)];
+    return t;
+}
+
+// CRandomBuilder::Assemble (src/RandomBuilder.cpp:71-425) without walls: per bead, in
+// creation order, a speed drawn from the Maxwell table, a position uniform in the box
+// (a coordinate within 1e-4 of the upper face set to 0) and a direction uniform on the
+// sphere; then, bond by bond in each polymer, a tail further than 1 from its head along an
+// axis is moved to the head plus that axis' component of the unit head-tail vector; then
+// the centre-of-mass velocity is removed and the velocities scaled to kT exactly.
+// Wrapped, unwrapped and initial positions are the same.
+void build_reference_random(const dpd_input* in, uint64_t seed, double* x, double* v)
+{
+    const InputData& d = in->d;
+    const System& s = in->s;
+    const int64_t n = (int64_t)s.id.size();
+    RefUniform U{seed};
+    const std::vector<double> table = maxwell_table(n);
+    const double L[3] = {box_len(d, 0), box_len(d, 1), box_len(d, 2)};
+    double vcm[3] = {0.0, 0.0, 0.0};
+    const double two_pi = 2.0 * 3.141592653589793238;
+    for (int64_t i = 0; i < n; ++i) {
+        const size_t index = (size_t)((double)table.size() * U.next());
+        const double vmag = table[index];
+        double xp[3];
+        xp[0] = L[0] * U.next();
+        xp[1] = L[1] * U.next();
+        xp[2] = L[2] * U.next();
+        for (int a = 0; a < 3; ++a)
+            if (L[a] - xp[a] < 0.0001) xp[a] = 0.0;
+        const double vtheta = std::acos(1.0 - 2.0 * U.next());
+        const double vphi = two_pi * U.next();
+        const double vp[3] = {vmag * std::sin(vtheta) * std::cos(vphi), vmag * std::sin(vtheta) * std::sin(vphi),
+                              vmag * std::cos(vtheta)};
+        for (int a = 0; a < 3; ++a) {
+            vcm[a] += vp[a];
+            v[3 * i + a] = vp[a];
+            x[3 * i + a] = xp[a];
+        }
+    }
+    // bonds in polymer order are the engine's bond order (polymers in creation order)
+    for (size_t b = 0; b < s.bond_head.size(); ++b) {
+        const int64_t h = s.bond_head[b], t = s.bond_tail[b];
+        double rd[3];
+        for (int a = 0; a < 3; ++a) rd[a] = x[3 * t + a] - x[3 * h + a];
+        const double rmag = std::sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
+        for (int a = 0; a < 3; ++a)
+            if (std::fabs(rd[a]) > 1.0) x[3 * t + a] = x[3 * h + a] + rd[a] / rmag;
+    }
+    for (int a = 0; a < 3; ++a) vcm[a] /= (double)n;
+    double vtotal = 0.0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            v[3 * i + a] -= vcm[a];
+            vtotal += v[3 * i + a] * v[3 * i + a];
+        }
+    vtotal = std::sqrt(vtotal / (double)(3 * n));
+    const double sk = std::sqrt(d.kT);
+    for (int64_t i = 0; i < 3 * n; ++i) v[i] = sk * v[i] / vtotal;
 }
 
 std::string fmt6(double v)
@@ -283,6 +335,57 @@ int dpd_input_beads(const dpd_input* in, int32_t* id, int32_t* type, int32_t* po
     return DPD_OK;
 }
 
+int dpd_input_write_summary(const dpd_input* in, const char* path)
+{
+    if (!in || !path) return DPD_ERR_PARAM;
+    const InputData& d = in->d;
+    const System& s = in->s;
+    std::ofstream o(path);
+    if (!o) return DPD_ERR_IO;
+    // CInitialState::SerializeAsciiFile (src/InitialState.cpp:1153-1347): counts, then the
+    // bead, bond, bond-pair and polymer types in declaration order, 6 significant digits
+    // (the fraction in a left-justified field of 12 at 8 digits, which stays the stream's
+    // precision)
+    o << "Number of polymers created\t\t" << s.npolymers << "\n";
+    o << "Number of beads created\t\t\t" << s.id.size() << "\n";
+    o << "Number of bonds created\t\t\t" << s.bond_head.size() << "\n";
+    o << "Number of bond pairs created\t" << s.pair_first.size() << "\n";
+    for (size_t t = 0; t < s.polymers_per_type.size(); ++t)
+        o << "  # of type " << t << "  " << s.polymers_per_type[t] << "\n";
+    const int nt = d.ntypes();
+    for (int t = 0; t < nt; ++t) {
+        o << "\nBead type " << t << "\n  Radius: " << d.bead_radius[t] << "\n  Cons Int: ";
+        for (int j = 0; j < nt; ++j) o << d.a[t * nt + j] << " ";
+        o << "\n  Diss Int: ";
+        for (int j = 0; j < nt; ++j) o << d.g[t * nt + j] << " ";
+        o << "\n";
+    }
+    for (size_t b = 0; b < d.bonds.size(); ++b)
+        o << "\nBond type " << b << "\n  Spring Const/Unstr length " << d.bonds[b].k << " " << d.bonds[b].l0 << "\n";
+    o << "\n";
+    for (size_t b = 0; b < d.bondpairs.size(); ++b)
+        o << "\nStiff bond type " << b << "\n  Modulus/Angle " << d.bondpairs[b].k << " "
+          << d.bondpairs[b].phi0_deg * 3.14159265358979323846 / 180.0 << "\n";
+    o << "\n";
+    for (size_t t = 0; t < s.types.size(); ++t) {
+        const auto& pt = s.types[t];
+        o << "Polymer type " << t << "\n";
+        o << "  with fraction          " << std::setw(12) << std::setprecision(8) << std::left << pt.fraction << "\n";
+        o << "  containing beads       " << pt.bead_type.size() << "\n";
+        o << "  and bonds              " << pt.bond_type.size() << "\n";
+        o << "  Bead types\n";
+        for (int bt : pt.bead_type) o << "  " << bt << " ";
+        if (!pt.bond_type.empty()) {
+            o << "\n  Bond types\n";
+            for (int bt : pt.bond_type) o << "  " << bt << " ";
+            o << "\n  Stiff bond types\n";
+            for (int bt : pt.pair_type) o << "  " << bt << " ";
+        }
+        o << "\n\n";
+    }
+    return o ? DPD_OK : DPD_ERR_IO;
+}
+
 int dpd_input_bond_types(const dpd_input* in, int32_t* count, int32_t* head_type, int32_t* tail_type)
 {
     if (!in) return DPD_ERR_PARAM;
@@ -334,14 +437,15 @@ int dpd_input_build_state(const dpd_input* in, uint64_t seed, double* x, double*
 {
     if (!in || !x || !v) return DPD_ERR_PARAM;
     const size_t n = in->s.id.size();
+    if (in->d.state != "lamella") {
+        build_reference_random(in, seed, x, v);
+        if (unpbc) std::copy(x, x + 3 * n, unpbc);
+        return DPD_OK;
+    }
     std::mt19937_64 rng(seed);
     std::vector<double> u(3 * n);
     std::string err;
-    if (in->d.state == "lamella") {
-        if (!build_lamella(in, rng, u.data(), err)) return DPD_ERR_UNSUPPORTED;
-    } else {
-        build_random(in, rng, u.data());
-    }
+    if (!build_lamella(in, rng, u.data(), err)) return DPD_ERR_UNSUPPORTED;
     wrap(in->d, n, u.data(), x);
     if (unpbc) std::copy(u.begin(), u.end(), unpbc);
     maxwell(rng, in->d.kT, n, v);
@@ -457,9 +561,73 @@ int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* 
                                       s.pair_first.data(), s.pair_second.data(), s.pair_k.data(),
                                       s.pair_phi0.data())))
             return fail(rc);
+    if ((rc = dpd_input_write_summary(in, (base + "dmpcis." + run_id).c_str()))) {
+        put_err(err, errlen, "cannot write the dmpcis summary");
+        return fail(rc);
+    }
     std::ofstream hist(base + "dmpchs." + run_id);
-    hist << std::setprecision(6);
     const double L3[3] = {d.nx * d.wx, d.ny * d.wy, d.nz * d.wz};
+    // history rows (CMonitor::SaveHistoryState, src/Monitor.cpp:1750-1823): time, kT, P,
+    // KE, PE, total energy (zero unless ToggleEnergyOutput is on), the mean squared
+    // displacement of each bead type from its starting unwrapped position over 6 dt t
+    // (src/Monitor.cpp:828-945), the mean end-to-end distance of each polymer type (head
+    // and tail beads, wrapped positions with one minimum-image correction,
+    // src/Monitor.cpp:707-778); each value left-justified in 12 columns, 6 digits, then a
+    // space (src/TimeSeriesData.cpp:84-89)
+    const std::vector<double> u0 = u;
+    double dt = d.dt;   // as SetTimeStepSize leaves it
+    const int ntypes = d.ntypes();
+    std::vector<int64_t> per_type(ntypes, 0);
+    for (int64_t i = 0; i < n; ++i) per_type[s.type[i]]++;
+    std::vector<int64_t> head_of, tail_of;   // first and last bead of every polymer
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t p = s.poly_id[i] - 1;
+        if (p >= (int64_t)head_of.size()) { head_of.resize(p + 1, -1); tail_of.resize(p + 1, -1); }
+        if (head_of[p] < 0) head_of[p] = i;
+        tail_of[p] = i;
+    }
+    bool energy_out = false;
+    auto history_row = [&](int64_t t, const dpd_obs& o) {
+        std::vector<double> row = {(double)t, o.kT, o.pressure, 0.0, 0.0, 0.0};
+        if (energy_out) {
+            double pe3[3] = {0, 0, 0};
+            if (dpd_energy_output(e, 0, pe3)) return false;
+            const double ke = 0.5 * (o.sum_v2[0] + o.sum_v2[1] + o.sum_v2[2]);
+            row[3] = ke;
+            row[4] = pe3[0] + pe3[1] + pe3[2];
+            row[5] = ke + row[4];
+        }
+        if (dpd_download(e, x.data(), u.data(), nullptr, nullptr)) return false;
+        std::vector<double> msd(ntypes, 0.0);
+        for (int64_t i = 0; i < n; ++i) {
+            double q = 0.0;
+            for (int a = 0; a < 3; ++a) q += (u[3 * i + a] - u0[3 * i + a]) * (u[3 * i + a] - u0[3 * i + a]);
+            msd[s.type[i]] += q;
+        }
+        for (int b = 0; b < ntypes; ++b) {
+            if (per_type[b] > 0) msd[b] /= 6.0 * dt * (double)t * (double)per_type[b];
+            row.push_back(msd[b]);
+        }
+        std::vector<double> ee(s.polymers_per_type.size(), 0.0);
+        for (size_t p = 0; p < head_of.size(); ++p) {
+            const int64_t hb = head_of[p], tb = tail_of[p];
+            double q = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                double dx = x[3 * hb + a] - x[3 * tb + a];
+                if (dx > 0.5 * L3[a]) dx -= L3[a];
+                else if (dx < -0.5 * L3[a]) dx += L3[a];
+                q += dx * dx;
+            }
+            ee[s.poly_type[hb]] += std::sqrt(q);
+        }
+        for (size_t p = 0; p < ee.size(); ++p) {
+            if (s.polymers_per_type[p] > 0) ee[p] /= (double)s.polymers_per_type[p];
+            row.push_back(ee[p]);
+        }
+        for (double val : row) hist << std::setw(12) << std::setprecision(6) << std::left << val << " ";
+        hist << "\n";
+        return true;
+    };
     dpd_run_stats st{};
     st.nbeads = n;
     size_t next_cmd = 0;
@@ -486,7 +654,6 @@ int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* 
                                    bk.data(), bl0.data(), (int64_t)s.pair_first.size(), s.pair_first.data(),
                                    s.pair_second.data(), pk.data(), pphi.data());
     };
-    double dt = d.dt;
     std::vector<double> a = in->a, g = in->g;
     double seconds = 0.0;
     int64_t t = 0;
@@ -526,6 +693,10 @@ int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* 
                 for (size_t q = 0; q < s.pair_type.size(); ++q)
                     if (s.pair_type[q] == pt) { pk[q] = k; pphi[q] = phi0; }
                 rc = upload_topology();
+            } else if (c.name == "ToggleEnergyOutput") {
+                // CMonitor::InternalToggleEnergyOutput (src/Monitor.cpp:3171-3177)
+                energy_out = !energy_out;
+                rc = DPD_OK;
             } else if (c.name == "ZeroTotalMom") {
                 rc = dpd_zero_total_momentum(e, -1);
             } else if (c.name == "RenormaliseMomenta" && c.args.size() >= 2) {
@@ -556,7 +727,7 @@ int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* 
         if (sample) {
             dpd_obs o;
             dpd_observables(e, &o);
-            hist << t << " " << o.kT << " " << o.pressure << "\n";
+            if (!history_row(t, o)) return fail(DPD_ERR_HIP);
             st.last_kT = o.kT;
             st.last_pressure = o.pressure;
             st.samples++;

@@ -198,7 +198,97 @@ __global__ __launch_bounds__(256) void k_slice_bondpairs(SliceArgs A, double* __
     }
 }
 
+// The 13 half-shell neighbours of the reference's cells (CSimBox::MakeCNTCells,
+// src/SimBox.cpp:830-867) as bits of the neighbour code (dx+1) + 3 (dy+1) + 9 (dz+1):
+// (1,-1,0), (1,0,0), (0,1,0), (1,1,0) and the nine cells of the layer above.
+constexpr uint32_t kHalfShellMask = (1u << 11) | (1u << 14) | (1u << 16) | (1u << 17) | (0x1FFu << 18);
+
+// CCNTCell::UpdateTotalEnergy's pair sum (src/CNTCell.cpp:2909-3229): every pair inside a
+// cell once, a pair across cells only from the cell whose half shell holds the other cell
+// and only when the neighbour's id is the larger (src/CNTCell.cpp:3081), so the
+// reference's energy output leaves out about half of the cross-cell pairs.  Thread per bead
+// over its 27 neighbour cells, 1/2 a (1 - r)^2 in FP64; out[0] += the sum.
+__global__ __launch_bounds__(256) void k_energy_pairs(SliceArgs A, double* __restrict__ out)
+{
+    const BoxDev& B = A.B;
+    const int s0 = A.cstart[B.koff], s1 = A.cstart[B.koff + B.nbricks * kBrickCells];
+    double e = 0.0;
+    for (int i = s0 + blockIdx.x * blockDim.x + threadIdx.x; i < s1; i += gridDim.x * blockDim.x) {
+        const float4 pi = A.pos[i];
+        const int ti = bead_type(__float_as_int(pi.w));
+        const int idi = __float_as_int(A.vp[i].w);
+        int c[3];
+        key_cell(A.key[i] - B.koff, B, c);
+        for (int nb = 0; nb < 27; ++nb) {
+            const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - 1;
+            const bool same = nb == 13, half = (kHalfShellMask >> nb) & 1u;
+            if (!same && !half) continue;
+            int cx = c[0] + ox, cy = c[1] + oy, cz = c[2] + oz;
+            cx += (cx < 0) ? B.n[0] : ((cx >= B.n[0]) ? -B.n[0] : 0);
+            cy += (cy < 0) ? B.n[1] : ((cy >= B.n[1]) ? -B.n[1] : 0);
+            cz += (cz < 0) ? B.n[2] : ((cz >= B.n[2]) ? -B.n[2] : 0);
+            const int k = B.koff + cell_key(cx, cy, cz, B);
+            const float sx = (float)ox * B.w[0], sy = (float)oy * B.w[1], sz = (float)oz * B.w[2];
+            for (int j = A.cstart[k], j1 = A.cstart[k + 1]; j < j1; ++j) {
+                if (j == i) continue;
+                if (half && !(__float_as_int(A.vp[j].w) > idi)) continue;
+                const float4 pj = A.pos[j];
+                const float dx = (pi.x - pj.x) - sx, dy = (pi.y - pj.y) - sy, dz = (pi.z - pj.z) - sz;
+                const double dr2 = (double)(dx * dx + dy * dy + dz * dz);
+                if (!(dr2 < 1.0)) continue;
+                const double dr = sqrt(dr2);
+                if (!(dr > 0.000000001)) continue;
+                const int tt = ti * kMaxTypes + bead_type(__float_as_int(pj.w));
+                // a pair inside a cell is seen from both of its beads: half from each
+                e += (same ? 0.25 : 0.5) * (double)A.table[tt] * (1.0 - dr) * (1.0 - dr);
+            }
+        }
+    }
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0 && e != 0.0) atomicAdd(out, e);
+}
+
+// bond and bond-pair potential energies of the box (src/Bond.cpp:279-288,
+// src/BondPair.cpp:320-378): thread per bond, then per bond pair; out[1], out[2]
+__global__ __launch_bounds__(256) void k_energy_bonded(SliceArgs A, double* __restrict__ out)
+{
+    const int nb = A.b1 - A.b0, q = blockIdx.x * blockDim.x + threadIdx.x;
+    double eb = 0.0, ea = 0.0;
+    if (q < nb) {
+        const int2 hb = A.bond_bead[A.b0 + q];
+        double uh[3], ut[3];
+        unwrapped(A, hb.x, A.slot_of[hb.x], uh);
+        unwrapped(A, hb.y, A.slot_of[hb.y], ut);
+        const double2 par = A.bond_par[A.b0 + q];
+        const double L = sqrt((uh[0] - ut[0]) * (uh[0] - ut[0]) + (uh[1] - ut[1]) * (uh[1] - ut[1]) +
+                              (uh[2] - ut[2]) * (uh[2] - ut[2]));
+        eb = 0.5 * par.x * (L - par.y) * (L - par.y);
+    } else if (q < nb + A.a1 - A.a0) {
+        const int g = A.a0 + q - nb;
+        const int4 abc = A.ang_bead[g];
+        double ua[3], ub[3], uc[3], f[3][3];
+        unwrapped(A, abc.x, A.slot_of[abc.x], ua);
+        unwrapped(A, abc.y, A.slot_of[abc.y], ub);
+        unwrapped(A, abc.z, A.slot_of[abc.z], uc);
+        angle_forces(ua, ub, uc, A.ang_par[g], f, &ea);
+    }
+    eb = wave_sum(eb);
+    ea = wave_sum(ea);
+    if ((threadIdx.x & 63) == 0) {
+        if (eb != 0.0) atomicAdd(out + 1, eb);
+        if (ea != 0.0) atomicAdd(out + 2, ea);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_energy_output(const SliceArgs& a, double* out3, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_energy_pairs, dim3(kSliceBlocks), dim3(256), 0, s, a, out3);
+    const int nq = (a.b1 - a.b0) + (a.a1 - a.a0);
+    if (nq > 0) hipLaunchKernelGGL(k_energy_bonded, dim3((nq + 255) / 256), dim3(256), 0, s, a, out3);
+    return hipGetLastError();
+}
 
 hipError_t launch_slice_bondpairs(const SliceArgs& a, double* tab, hipStream_t s)
 {

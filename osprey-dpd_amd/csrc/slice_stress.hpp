@@ -33,6 +33,8 @@ constexpr size_t kSliceLdsBytes = 32768;
 constexpr int kSliceBlocks = 1024;
 
 hipError_t launch_slice_stress(const SliceArgs& a, double* pair_tab, double* bond_tab, hipStream_t s);
+// the reference's energy-output sums: out3[0] pairs, [1] bonds, [2] bond pairs (added to)
+hipError_t launch_energy_output(const SliceArgs& a, double* out3, hipStream_t s);
 // bond pairs: tab[nslice][ntypes^3][9] (cleared by the caller)
 hipError_t launch_slice_bondpairs(const SliceArgs& a, double* tab, hipStream_t s);
 

@@ -83,6 +83,8 @@ SIGNATURES = {
     "dpd_set_box_interactions": (C.c_int, [_P, C.c_int32, _P, _P]),
     "dpd_slice_stress": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P]),
     "dpd_slice_stress_bondpairs": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "dpd_slice_stress_noalloc": (C.c_int, [_P, C.c_int32, C.c_int32, _P, _P]),
+    "dpd_energy_output": (C.c_int, [_P, C.c_int32, _P]),
     "dpd_zero_total_momentum": (C.c_int, [_P, C.c_int32]),
     "dpd_renormalise_momenta": (C.c_int, [_P, C.c_int32]),
 }
@@ -267,6 +269,13 @@ class Engine:
         bond = np.zeros((int(nslices), nt * nt, 9)) if bonds else None
         self._check(lib().dpd_slice_stress(self.h, int(box), int(nslices), _ptr(pair), _ptr(bond)))
         return pair, bond
+
+    def energy_output(self, box=0):
+        """[pair, bond, bond-pair] potential energy as the reference's energy output sums it
+        (cross-cell pairs only when the neighbour's id is larger)."""
+        out = np.zeros(3)
+        self._check(lib().dpd_energy_output(self.h, int(box), _ptr(out)))
+        return out
 
     def slice_stress_bondpairs(self, nslices, box=0):
         """Bond-pair stress profile (CMonitor::AddBondPairStress, src/Monitor.cpp:2520-2739):

@@ -38,6 +38,7 @@ HOST_SIGNATURES = {
     "dpd_input_topology": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dpd_input_bond_types": (C.c_int, [_P, C.POINTER(C.c_int32), _P, _P]),
     "dpd_input_params": (C.c_int, [_P, C.c_int32, _P]),
+    "dpd_input_write_summary": (C.c_int, [_P, C.c_char_p]),
     "dpd_input_build_state": (C.c_int, [_P, C.c_uint64, _P, _P, _P]),
     "dpd_input_command": (C.c_int, [_P, C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int64),
                                     C.c_char_p, C.c_int32]),
@@ -146,6 +147,12 @@ class Input:
             hlib().dpd_input_command(self.h, i, name, 256, C.byref(t), args, 1024)
             out.append((name.value.decode(), t.value, args.value.decode().split()))
         return out
+
+    def write_summary(self, path):
+        """Write the dmpcis initial-state summary (CInitialState::SerializeAsciiFile)."""
+        rc = hlib().dpd_input_write_summary(self.h, str(path).encode())
+        if rc != 0:
+            raise InputError(f"cannot write {path}")
 
     def engine(self, device=0, pair_kernel=PAIR_AUTO):
         c = self.counts

@@ -33,6 +33,10 @@ schedule COMMANDS (time step changes, thermostat, conservative-force and force t
 interaction changes: src/CNTCell.cpp:223-377, :434-456): ``cmd14.dat.gz`` is the state
 after the last step and ``cmd_history.txt`` the ``dmpchs`` rows of every step.
 
+``io/`` (``make_golden.py io``): the dmpcis summary of every case, the lipidphi8 restart
+file as the reference wrote it, and a deterministic 10-step run from it (thermostat off,
+energy output on) with its dmpchs rows and final snapshot.
+
 ``rng.txt`` holds the first 1000 draws of the reference LCG for seed |RNGSeed|, logged
 by ``Command GenerateRNGUniform`` (src/mcGenerateRNGUniformImpl.cpp:44-62).
 """
@@ -324,12 +328,47 @@ def make_rng(work):
         fh.write("\n".join(vals) + "\n")
 
 
+def make_io(work):
+    """Osprey file fixtures: the dmpcis summary the reference writes for every case
+    (CInitialState::SerializeAsciiFile, src/InitialState.cpp:1153-1347), a restart file as
+    the reference writes it (flag line, 17 columns, src/RestartState.cpp:100-156), and a
+    deterministic run of the lipidphi8 snapshot (thermostat off from step 1, energy output
+    on) with its full dmpchs rows (src/Monitor.cpp:1750-1823) and final snapshot."""
+    out = os.path.join(HERE, "io")
+    os.makedirs(out, exist_ok=True)
+    for name in CASES:
+        w, restart = warm_restart(name, work)
+        run(work, name + "s", dmpci(name, restart, 1))
+        shutil.copy(os.path.join(work, f"dmpcis.{name}s"), os.path.join(out, f"dmpcis.{name}"))
+    # the reference's random starting state (CRandomBuilder::Assemble, Maxwell speeds of
+    # CBuilder::MaxwellDist): one step with the forces toggled off moves every bead by
+    # dt v and leaves v alone, so the time-1 snapshot carries the built state
+    for name in ("water6", "chains7", "lipid8"):
+        run(work, name + "r", dmpci(name, "random", 1, "Command ToggleDPDBeadForces 1\n"))
+        write_gz(os.path.join(out, f"random_{name}.dat.gz"), bead_lines(os.path.join(work, f"dmpcrs.{name}r.con.1.dat")))
+    w, restart = warm_restart("lipidphi8", work)
+    src = os.path.join(work, f"dmpcrs.{w}.con.{WARM}.dat")
+    with open(src, "rb") as fi, gzip.open(os.path.join(out, "dmpcrs.lipidphi8.con.0.dat.gz"), "wb") as fo:
+        fo.write(fi.read())
+    cmds = "Command ToggleDPDBeadThermostat 1\nCommand ToggleEnergyOutput 1 0\n"
+    text = dmpci("lipidphi8", restart, 10, cmds)
+    run(work, "nfe", text)
+    shutil.copy(os.path.join(work, "dmpchs.nfe"), os.path.join(out, "dmpchs.lipidphi8_nf10"))
+    write_gz(os.path.join(out, "lipidphi8_nf10.dat.gz"), bead_lines(os.path.join(work, "dmpcrs.nfe.con.10.dat")))
+    with open(os.path.join(out, "dmpci.lipidphi8_nf10"), "w") as fh:
+        fh.write(dmpci("lipidphi8", "restart RunId lipidphi8 StateId 0", 10, cmds))
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
     work = tempfile.mkdtemp(prefix="dpd_golden_")
     try:
         args = sys.argv[1:]
+        if args[:1] == ["io"]:
+            make_io(work)
+            print("wrote io fixtures")
+            return
         if args[:1] == ["extras"]:
             # only the energy and command fixtures (the snapshots stay as committed)
             for name in CASES:

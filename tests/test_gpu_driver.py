@@ -60,7 +60,7 @@ def test_run_writes_history_and_restarts(tmp_path):
           flush=True)
     assert st["steps"] == 400 and st["nbeads"] == 1536 and st["samples"] == 20
     h = history(tmp_path, "w")
-    assert h.shape == (20, 3)
+    assert h.shape == (20, 8)   # time kT P KE PE E, MSD of W, end-to-end of Water
     assert np.array_equal(h[:, 0], np.arange(20, 401, 20))
     late = h[h[:, 0] > 200]
     # water at a = 25, rho = 3, dt = 0.02: kT ~1.01 and P ~23.7 (reference dmpchs rows)
@@ -188,3 +188,41 @@ def test_bond_commands(tmp_path):
         lengths.append(np.linalg.norm(r["u"][h] - r["u"][t], axis=1).mean())
     print(f"  mean bond length {lengths[0]:.3f} -> {lengths[1]:.3f}", flush=True)
     assert lengths[0] < 0.75 and lengths[1] > 0.8
+
+
+def test_history_and_summary_match_the_reference(tmp_path):
+    """dpd_run from the restart file the reference wrote, thermostat off from step 1 (a
+    deterministic trajectory) and ToggleEnergyOutput on: the dmpcis summary is the
+    reference's byte for byte, and every dmpchs row -- time, kT, P, kinetic, potential
+    (the reference's pair sum) and total energy, MSD per bead type, end-to-end distance per
+    polymer type -- matches the reference's within 1e-4 relative (FP32 trajectory against
+    the reference's FP64 over ten steps), in the reference's column layout."""
+    import gzip
+    import shutil
+    from helpers import GOLDEN
+    io = os.path.join(GOLDEN, "io")
+    with gzip.open(os.path.join(io, "dmpcrs.lipidphi8.con.0.dat.gz"), "rb") as fi, \
+            open(os.path.join(tmp_path, "dmpcrs.lipidphi8.con.0.dat"), "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    shutil.copy(os.path.join(io, "dmpci.lipidphi8_nf10"), os.path.join(tmp_path, "dmpci.nf"))
+    st = H.run(tmp_path, "nf")
+    assert st["steps"] == 10 and st["samples"] == 10
+    assert open(os.path.join(tmp_path, "dmpcis.nf")).read() == open(os.path.join(io, "dmpcis.lipidphi8")).read()
+    got_txt = open(os.path.join(tmp_path, "dmpchs.nf")).read().splitlines()
+    ref_txt = open(os.path.join(io, "dmpchs.lipidphi8_nf10")).read().splitlines()
+    got = np.array([[float(t) for t in l.split()] for l in got_txt])
+    ref = np.array([[float(t) for t in l.split()] for l in ref_txt])
+    assert got.shape == ref.shape == (10, 11)
+    # fields of 12 columns and a space, as the reference writes them
+    assert all(len(l) == 13 * 11 for l in got_txt)
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-12)
+    err[ref == 0] = np.abs(got[ref == 0])
+    for c, name in enumerate(["t", "kT", "P", "KE", "PE", "E", "MSD H", "MSD T", "MSD W", "EE water", "EE lipid"]):
+        print(f"  {name:9s} max rel diff {err[:, c].max():.2e}", flush=True)
+    assert err.max() < 1e-4
+    # the final snapshot against the reference's
+    ref_state = np.loadtxt(os.path.join(io, "lipidphi8_nf10.dat.gz"))
+    r = H.read_restart(os.path.join(tmp_path, "dmpcrs.nf.con.10.dat"), len(ref_state))
+    assert np.allclose(r["v"], ref_state[:, 10:13], rtol=1e-4, atol=1e-4)
+    assert np.allclose(r["u"], ref_state[:, 7:10], rtol=1e-5, atol=1e-5)
