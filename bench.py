@@ -31,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "osprey-dpd_amd"))
 import numpy as np  # noqa: E402
 
 import dpd_amd as D  # noqa: E402
+from dpd_amd import tune as T  # noqa: E402
 from dpd_amd import workloads as W  # noqa: E402
 
 METRIC = "bead-steps/s at rho=3 (1 box and batched replicas) vs host CPU; force rel err"
@@ -95,6 +96,9 @@ def build_engine(wl, kernel):
         h, t, k, l0 = wl.bonds if wl.bonds is not None else ((), (), (), ())
         fi, se, mo, ph = wl.pairs if wl.pairs is not None else ((), (), (), ())
         e.upload_topology(h, t, k, l0, fi, se, mo, ph)
+    if kernel == D.PAIR_AUTO:
+        # the schedule the autotuner recorded for this shape (osprey-dpd_amd/tune.json)
+        T.apply(e)
     return e
 
 
@@ -115,6 +119,8 @@ def bench_batched(nbox=256, steps=100, warmup=20, nsingle=16, seed=20261202, pai
     sizes = rng.integers(8, 25, size=nbox)
     seeds = rng.integers(1, 2 ** 31, size=nbox)
     e, meta = S.build_batch(params, sizes, seeds, pair_kernel=pair_kernel)
+    if pair_kernel == D.PAIR_AUTO:
+        T.apply(e)
     _warm_graphs(e, warmup, steps)
     t0 = time.perf_counter()
     e.step(steps, observe=True, graph=True)
@@ -126,6 +132,8 @@ def bench_batched(nbox=256, steps=100, warmup=20, nsingle=16, seed=20261202, pai
     tot_bs, tot_t = 0.0, 0.0
     for i in range(nsingle):
         es, _ = S.build_batch(params[i:i + 1], sizes[i:i + 1], seeds[i:i + 1], pair_kernel=pair_kernel)
+        if pair_kernel == D.PAIR_AUTO:
+            T.apply(es)
         _warm_graphs(es, warmup, steps)
         t0 = time.perf_counter()
         es.step(steps, observe=True, graph=True)
@@ -152,7 +160,8 @@ def bench_configs(steps=100, warmup=20):
         o = e.observables()
         out[wl.name] = {"beads": wl.n, "bonds": wl.nbonds, "angles": wl.nangles, "steps": steps,
                         "value": wl.n * steps / wall, "ms_per_step": wall / steps * 1e3,
-                        "pair_kernel": e.pair_kernel, "kT": o["kT"], "pressure": o["pressure"]}
+                        "pair_kernel": e.pair_kernel, "schedule": D.schedule_name(e.schedule), "kT": o["kT"],
+                        "pressure": o["pressure"]}
         e.close()
     return out
 
@@ -428,7 +437,8 @@ def main():
                      "measured_copy_gbs": copy_gbs,
                      "flops_per_bead_step": FLOPS_PER_BEAD_STEP,
                      "step_tflops": FLOPS_PER_BEAD_STEP * value / 1e12,
-                     "kernel": kname, "pair_kernel": e.pair_kernel, "pair_ms_per_launch": t_pair * 1e3,
+                     "kernel": kname, "pair_kernel": e.pair_kernel, "schedule": D.schedule_name(e.schedule),
+                     "pair_ms_per_launch": t_pair * 1e3,
                      "pair_launches_timed": args.pair_launches,
                      "pair_within_step": consistent,
                      "alg_bytes_per_launch": alg_bytes,

@@ -59,19 +59,22 @@ typedef struct dpd_params {
 
 #define DPD_MAX_TYPES 16
 
-#define DPD_PAIR_AUTO      0    /* by bead count, re-chosen at each commit: the LDS  */
-                                /* stream kernel (1024) below 56k beads, the hit-   */
-                                /* list kernel above (measured crossover on MI355X)  */
-#define DPD_PAIR_BRICK     1    /* LDS cell-brick kernel, 256 threads per brick      */
+#define DPD_PAIR_AUTO      0    /* re-chosen at each commit: the hit-list kernel up   */
+                                /* to 2^26 beads, the full-shell kernel beyond       */
 #define DPD_PAIR_FULLSHELL 2    /* thread-per-bead 27-cell kernel                    */
-#define DPD_PAIR_BRICK512  3    /* LDS cell-brick kernel, 512 threads per brick      */
-#define DPD_PAIR_STREAM    4    /* LDS cell-stream kernel, 512 threads per brick     */
-#define DPD_PAIR_STREAM1024 5   /* LDS cell-stream kernel, 1024 threads per brick    */
-#define DPD_PAIR_COMPACT   6    /* thread-per-bead 27-cell kernel, compacted pairs   */
-#define DPD_PAIR_CELLS     7    /* wavefront-per-cell kernel, candidates loaded once */
-#define DPD_PAIR_HITLIST   8    /* thread-per-bead 27-cell scan into an LDS list of  */
+#define DPD_PAIR_HITLIST   8    /* thread-per-bead scan of the 9 neighbour rows      */
+                                /* (cells x-1..x+1 as one range) into an LDS list of */
                                 /* partners in range, then one dense evaluation loop */
-#define DPD_PAIR_KINDS     8    /* kernels are numbered 1..DPD_PAIR_KINDS            */
+/* Ids 1 and 3-7 named LDS brick, cell-stream, compacted and wavefront-per-cell kernels that
+ * measured slower than the hit-list kernel at every size; they are retired and refused
+ * with DPD_ERR_UNSUPPORTED. */
+#define DPD_PAIR_BRICK     1
+#define DPD_PAIR_BRICK512  3
+#define DPD_PAIR_STREAM    4
+#define DPD_PAIR_STREAM1024 5
+#define DPD_PAIR_COMPACT   6
+#define DPD_PAIR_CELLS     7
+#define DPD_PAIR_KINDS     8    /* kernel ids are 1..DPD_PAIR_KINDS                  */
 
 /* FP64 observables of the last completed step (CMonitor::CalculateSingleBeadData,
  * src/Monitor.cpp:790-1041).  stress = (sum over pairs and bonds of r (x) F) / V,
@@ -192,10 +195,29 @@ int dpd_time_pair(dpd_engine* e, int32_t nlaunch, double* ms_per_launch);
 /* Pair-kernel choice.  dpd_autotune times `nsteps` steps of every pair kernel on the
  * current state (restoring it after each trial, so the simulation does not advance),
  * keeps the fastest and returns it; ms_per_step[DPD_PAIR_KINDS] (may be NULL) receives
- * the time per step of kernel k at index k - 1. */
+ * the time per step of kernel k at index k - 1 (0 for retired ids). */
 int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel);
 int32_t dpd_get_pair_kernel(const dpd_engine* e);
 int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step);
+
+/* Schedules of the pair launch: the hit-list kernel with its workgroup size, per-thread
+ * list capacity and flush threshold, and the full-shell kernel; dpd_schedule_name gives
+ * each one's name ("hitlist-t512-c24", ...), NULL past the last.  dpd_autotune_schedules
+ * times `nsteps` steps of every schedule on the current state as dpd_autotune does (the
+ * state is restored after each trial), keeps the fastest (the choice then stays across
+ * commits) and returns it; ms_per_step[dpd_schedule_count()] (may be NULL) receives each
+ * schedule's time per step.  No counterpart in the reference, which has one loop. */
+int32_t dpd_schedule_count(void);
+const char* dpd_schedule_name(int32_t sched);
+int dpd_set_schedule(dpd_engine* e, int32_t sched);
+int32_t dpd_get_schedule(const dpd_engine* e);
+int dpd_autotune_schedules(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step);
+/* Apply a schedule recorded by the autotuner (a file of one JSON object per line, written
+ * by dpd_amd.tune, keyed by the number of boxes (1, or 2 for any batch), beads per box as a
+ * power-of-two range [beads_lo, beads_hi), bead types and whether there are bonds) to a
+ * committed engine of that shape; *chosen (may be NULL) gets the schedule set, -1 when the
+ * file holds none for this shape (the engine is then left as it was). */
+int dpd_apply_tuning(dpd_engine* e, const char* path, int32_t* chosen);
 
 /* Momentum commands on the current (closed) state of box `box` (-1: every box, each on
  * its own), executed between dpd_step calls as the reference executes commands before

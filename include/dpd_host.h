@@ -92,8 +92,12 @@ typedef struct dpd_run_stats {
 } dpd_run_stats;
 
 /* Run dmpci.<run_id> found in `dir` to completion on `device`, as `dpd <runId>` does:
- * commands at their times, a sample every SamplePeriod appended to dmpchs.<run_id>
- * (time kT P rows), a restart snapshot every RestartPeriod. */
+ * the dmpcis.<run_id> summary first, commands at their times, a sample every SamplePeriod
+ * appended to dmpchs.<run_id> (the reference's history rows: time, kT, P, the energies
+ * under ToggleEnergyOutput, MSD per bead type, end-to-end distance per polymer type), a
+ * restart snapshot every RestartPeriod.  A pair-launch schedule recorded by the autotuner
+ * for the box's shape is applied from the file DPD_TUNE_FILE names, else from
+ * dpd_tune.json in `dir` when it exists. */
 int  dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* stats, char* err,
              int32_t errlen);
 

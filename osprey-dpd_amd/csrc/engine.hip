@@ -6,7 +6,7 @@
 //   k_advance   UpdateMom of the previous step fused with UpdatePos of this one
 //               (src/CNTCell.cpp:2280-2310, :1704-1757); computes each bead's new cell
 //               key and its rank in that cell
-//   k_brick_scan / k_cell_start   counting-sort prefix sums (bricks, then cells)
+//   k_group_scan / k_cell_start   counting-sort prefix sums (groups of 64 keys, then cells)
 //   k_place / k_permute   counting-sort placement into the new cell-sorted SoA buffers,
 //               beads of a cell in ascending bead index (deterministic layout)
 //   pair kernel DPD pair forces (src/CNTCell.cpp:1144-1555)
@@ -184,18 +184,43 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
 // workgroups come first, the bond and angle workgroups after them, so the bonded work fills
 // the CUs the pair kernel's last workgroups leave idle (the two write disjoint buffers).
 // (one box; held to 80 VGPRs like the hit-list kernel, six wavefronts per SIMD)
-template <bool OBS>
-__global__ __launch_bounds__(kHitThreads) __attribute__((amdgpu_waves_per_eu(OBS ? 4 : 6)))
+template <bool OBS, int THREADS, int CAP, int FLUSH>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(OBS ? 4 : hit_waves(THREADS, CAP))))
 void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buffers S,
                            const int* __restrict__ slot_of, Topo T)
 {
     const int npair = 8 * per_xcd;
     if ((int)blockIdx.x >= npair) {
-        const int q0 = ((int)blockIdx.x - npair) * kHitThreads;
-        bonded_body<OBS, false, kHitThreads / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
+        const int q0 = ((int)blockIdx.x - npair) * THREADS;
+        bonded_body<OBS, false, THREADS / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
         return;
     }
-    hitlist_body<OBS, false>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, false, THREADS, CAP, FLUSH>(A, B0, nblocks, per_xcd);
+}
+
+// the fused launch of hit-list schedule S with `nq` bonds and angles
+template <int S, bool OBS>
+void launch_hitlist_bonded_s(const PairArgs& a, const BoxDev& B0, int nq, const Buffers& Sb, const int* slot_of,
+                             const Topo& T, hipStream_t st)
+{
+    constexpr Schedule q = kSchedule[S];
+    const int gh = (a.n + q.threads - 1) / q.threads;
+    const int ph = (gh + 7) / 8;
+    const dim3 gr(8 * ph + (nq + q.threads - 1) / q.threads), bl(q.threads);
+    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush>), gr, bl, 0, st, a, B0, gh, ph, Sb,
+                       slot_of, T);
+}
+
+template <bool OBS>
+void launch_hitlist_bonded(int sched, const PairArgs& a, const BoxDev& B0, int nq, const Buffers& Sb,
+                           const int* slot_of, const Topo& T, hipStream_t st)
+{
+    switch (sched) {
+    case 1: launch_hitlist_bonded_s<1, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 2: launch_hitlist_bonded_s<2, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 3: launch_hitlist_bonded_s<3, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
+    default: launch_hitlist_bonded_s<0, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
+    }
 }
 
 // The bonded flag in pos.w: cleared on every slot (and the slot of every bead rewritten,
@@ -241,7 +266,7 @@ __global__ __launch_bounds__(256) void k_bonded_sum(Topo T, const int* __restric
 template <int MODE, bool MULTI>
 __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __restrict__ frc, Topo T,
                                                  const BoxDev* boxes,
-                                                 BoxDev B0, int* __restrict__ cnt, int* __restrict__ bcnt,
+                                                 BoxDev B0, int* __restrict__ cnt, int* __restrict__ gcnt,
                                                  int* __restrict__ nkey, int* __restrict__ rank,
                                                  int* __restrict__ flags, int64_t* __restrict__ step)
 {
@@ -287,8 +312,8 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
         if (m.bad) atomicAdd(flags + 0, 1);
     }
     // Counting-sort ranks.  Beads arrive cell-sorted, so a wavefront's keys repeat: one
-    // atomic per distinct cell (and per distinct brick) instead of one per bead, which
-    // would serialise ~192 same-address atomics per brick counter.
+    // atomic per distinct cell (and per distinct scan group) instead of one per bead, which
+    // would serialise ~192 same-address atomics per group counter.
     const int lane = threadIdx.x & 63;
     int r, size, leader;
     wave_groups(k, active, r, size, leader);
@@ -297,19 +322,19 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
     base = __shfl(base, leader, 64);
     if (active) rank[s] = base + r;
     wave_groups(k >> 6, active, r, size, leader);
-    if (active && lane == leader) atomicAdd(bcnt + (k >> 6), size);
+    if (active && lane == leader) atomicAdd(gcnt + (k >> 6), size);
 }
 
-// exclusive scan of brick counts (one workgroup: each thread a contiguous run of bricks,
-// wavefront shuffle scans, one cross-wavefront step), clears the counts
-__global__ __launch_bounds__(1024) void k_brick_scan(int nbricks, int* __restrict__ bcnt, int* __restrict__ bstart)
+// exclusive scan of the scan groups' counts (one workgroup: each thread a contiguous run of
+// groups, wavefront shuffle scans, one cross-wavefront step), clears the counts
+__global__ __launch_bounds__(1024) void k_group_scan(int ngroups, int* __restrict__ gcnt, int* __restrict__ gstart)
 {
     __shared__ int wsum[16];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int per = (nbricks + 1023) / 1024;
-    const int b0 = min(nbricks, t * per), b1 = min(nbricks, b0 + per);
+    const int per = (ngroups + 1023) / 1024;
+    const int b0 = min(ngroups, t * per), b1 = min(ngroups, b0 + per);
     int s = 0;
-    for (int b = b0; b < b1; ++b) s += bcnt[b];
+    for (int b = b0; b < b1; ++b) s += gcnt[b];
     int incl = s;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -331,21 +356,64 @@ __global__ __launch_bounds__(1024) void k_brick_scan(int nbricks, int* __restric
     __syncthreads();
     int run = wsum[wave] + incl - s;
     for (int b = b0; b < b1; ++b) {
-        const int c = bcnt[b];
-        bstart[b] = run;
+        const int c = gcnt[b];
+        gstart[b] = run;
         run += c;
-        bcnt[b] = 0;
+        gcnt[b] = 0;
     }
 }
 
-// cell starts: one wavefront per brick scans its 64 cell counts; clears the counts
-__global__ __launch_bounds__(256) void k_cell_start(int nbricks, int n, const int* __restrict__ bstart,
+// Engines of up to kSmallScanKeys cell keys: the cell starts in one launch (one workgroup,
+// each thread a contiguous run of keys, wavefront shuffle scans, one cross-wavefront step),
+// which replaces k_group_scan + k_cell_start; clears the cell and group counts.
+constexpr int kSmallScanKeys = 16384;
+__global__ __launch_bounds__(1024) void k_cell_scan_small(int nkeys, int ngroups, int n, int* __restrict__ cnt,
+                                                          int* __restrict__ gcnt, int* __restrict__ cstart)
+{
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int per = (nkeys + 1023) / 1024;
+    const int k0 = min(nkeys, t * per), k1 = min(nkeys, k0 + per);
+    int s = 0;
+    for (int k = k0; k < k1; ++k) s += cnt[k];
+    int incl = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        const int w = lane < 16 ? wsum[lane] : 0;
+        int wi = w;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            const int y = __shfl_up(wi, off, 64);
+            if (lane >= off) wi += y;
+        }
+        if (lane < 16) wsum[lane] = wi - w;
+    }
+    __syncthreads();
+    int run = wsum[wave] + incl - s;
+    for (int k = k0; k < k1; ++k) {
+        const int c = cnt[k];
+        cstart[k] = run;
+        run += c;
+        cnt[k] = 0;
+    }
+    for (int g = t; g < ngroups; g += 1024) gcnt[g] = 0;
+    if (t == 0) cstart[nkeys] = n;
+}
+
+// cell starts: one wavefront per scan group scans its 64 cell counts; clears the counts
+__global__ __launch_bounds__(256) void k_cell_start(int ngroups, int n, const int* __restrict__ gstart,
                                                     int* __restrict__ cnt, int* __restrict__ cstart)
 {
-    const int brick = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (brick >= nbricks) return;
-    const int k = brick * 64 + lane;
+    if (g >= ngroups) return;
+    const int k = g * 64 + lane;
     const int c = cnt[k];
     int incl = c;
 #pragma unroll
@@ -353,9 +421,9 @@ __global__ __launch_bounds__(256) void k_cell_start(int nbricks, int n, const in
         const int y = __shfl_up(incl, off, 64);
         if (lane >= off) incl += y;
     }
-    cstart[k] = bstart[brick] + incl - c;
+    cstart[k] = gstart[g] + incl - c;
     cnt[k] = 0;
-    if (brick == nbricks - 1 && lane == 63) cstart[nbricks * 64] = n;
+    if (g == ngroups - 1 && lane == 63) cstart[ngroups * 64] = n;
 }
 
 // Counting-sort placement, first half: every source slot s writes (bead index, s) to the
@@ -370,37 +438,31 @@ __global__ __launch_bounds__(256) void k_place(int n, const int* __restrict__ be
     tmp[cstart[nkey[s]] + rank[s]] = make_int2(bead[s], s);
 }
 
-// Counting-sort placement, second half: slot d of the new order takes the bead whose index
-// is the (d - start)-th smallest of its cell's beads, so the order inside every cell is
-// ascending bead index, a function of the state alone (reruns, and a state downloaded and
-// uploaded again, lay the beads out identically).  O(m^2) reads of the cell's m entries per
-// slot, from L1.  Then the move itself (recomputing the same advance as k_advance).
+// Counting-sort placement, second half: the bead placed at d goes to the start of its cell
+// plus the number of the cell's beads with a smaller bead index, so the order inside every
+// cell is ascending bead index, a function of the state alone (reruns, and a state
+// downloaded and uploaded again, lay the beads out identically).  One pass over the cell's m
+// entries per bead, from L1.  Then the move itself (recomputing the same advance as
+// k_advance).
 template <int MODE, bool MULTI>
 __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, float4* __restrict__ frc,
                                                  float4* __restrict__ vp, const BoxDev* boxes, BoxDev B0,
                                                  const int* __restrict__ nkey, const int2* __restrict__ tmp,
                                                  const int* __restrict__ cstart, int* __restrict__ slot_of)
 {
-    const int dst = blockIdx.x * blockDim.x + threadIdx.x;
-    if (dst >= n) return;
-    const int2 t0 = tmp[dst];
-    const int k = nkey[t0.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 t0 = tmp[i];
+    const int s = t0.y;
+    const int k = nkey[s];
     const int c0 = cstart[k], c1 = cstart[k + 1];
-    int s = t0.y;
-    if (c1 - c0 > 1) {
-        const int r = dst - c0;
-        for (int e = c0; e < c1; ++e) {
-            const int2 te = tmp[e];
-            int below = 0;
-            for (int x = c0; x < c1; ++x) below += tmp[x].x < te.x;
-            if (below == r) { s = te.y; break; }
-        }
-    }
+    int dst = c0;
+    for (int e = c0; e < c1; ++e) dst += tmp[e].x < t0.x;
     const float4 p = S.pos[s];
     const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
     const float4 v = S.vel[s];
     const float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
-    const int q = S.bead[s];
+    const int q = t0.x;   // bead index of the source slot
     int c[3];
     key_cell(S.key[s] - B.koff, B, c);
     const float o[3] = {p.x, p.y, p.z};
@@ -425,7 +487,7 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
         S.img[q] = im;
     }
     D.key[dst] = k;
-    D.code[dst] = scan_code(m.o, B);
+    D.code[dst] = scan_code(m.o, m.c[0], B);
     if (MODE == kRebin) {
         vp[dst] = v;
     } else {
@@ -588,11 +650,12 @@ struct dpd_engine {
     std::vector<BoxHost> boxes;
     int device = 0;
     hipStream_t stream = nullptr;
-    int nbricks = 0, nkeys = 0;
+    int ngroups = 0, nkeys = 0;   // scan groups of 64 keys, keys (all boxes)
     int64_t n = 0;
     int64_t step = 0;
     int cur = 0;
-    int pair_kernel = DPD_PAIR_FULLSHELL;
+    int pair_kernel = DPD_PAIR_FULLSHELL;   // the kernel of schedule `sched`
+    int sched = kFullShellSchedule;         // pair-launch schedule (kSchedule)
     bool pair_auto = false;   // choose by bead count at each commit
     bool committed = false;
     // device buffers
@@ -605,7 +668,7 @@ struct dpd_engine {
     float4 *frc = nullptr, *vp = nullptr;
     int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
     int2* place = nullptr;    // counting-sort placement: (bead index, source slot) per new slot
-    int *cnt = nullptr, *bcnt = nullptr, *bstart = nullptr, *cstart = nullptr;
+    int *cnt = nullptr, *gcnt = nullptr, *gstart = nullptr, *cstart = nullptr;
     int* flags = nullptr;
     int64_t* dstep = nullptr;
     double* acc = nullptr;
@@ -613,8 +676,8 @@ struct dpd_engine {
     double* gather = nullptr;   // 12n doubles for downloads
     double* slice = nullptr;    // slice stress tables (pair | bond)
     double4* madj = nullptr;    // per-box velocity shift and scale
-    int4* hitblk = nullptr;     // hit-list workgroups of a batch (box, first slot, end slot)
-    int nhitblk = 0;
+    int4* hitblk[2] = {};       // hit-list workgroups of a batch (box, first slot, end slot)
+    int nhitblk[2] = {};        // for workgroups of 512 [0] and 256 [1] threads
     size_t slice_cap = 0;
     // topology
     int *bond_ptr = nullptr, *ang_ptr = nullptr;
@@ -673,6 +736,21 @@ void drop_graphs(dpd_engine* e)
     e->graph_src.clear();
 }
 
+// switch the pair launch to schedule `s` (captured graphs hold the old launch)
+void set_schedule(dpd_engine* e, int s)
+{
+    if (s != e->sched) drop_graphs(e);
+    e->sched = s;
+    e->pair_kernel = kSchedule[s].kernel;
+}
+
+// the automatic choice: the hit-list kernel's default schedule for every engine whose slots
+// fit its 26-bit list entries, the full-shell kernel beyond
+int default_schedule(const dpd_engine* e)
+{
+    return e->n <= (int64_t)kHitSlotMask ? kDefaultSchedule : kFullShellSchedule;
+}
+
 void set_step_constants(BoxHost& b)
 {
     // CCNTCell::SetTimeStepConstants / ChangeTimeStepConstants (src/CNTCell.cpp:180-203, :357-377)
@@ -689,8 +767,8 @@ void set_step_constants(BoxHost& b)
 int check_params(dpd_engine* e, const dpd_params* p)
 {
     if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(e, DPD_ERR_PARAM, "need at least 3 cells per axis");
-    if ((int64_t)p->nx * p->ny * p->nz > (int64_t)1 << 28)
-        return fail(e, DPD_ERR_PARAM, "more than 2^28 cells in one box");
+    if ((int64_t)p->nx * p->ny * p->nz > (int64_t)1 << 24)
+        return fail(e, DPD_ERR_PARAM, "more than 2^24 cells in one box");
     if (p->wx < 1.0 || p->wy < 1.0 || p->wz < 1.0) return fail(e, DPD_ERR_PARAM, "cell widths must be >= 1");
     if (p->ntypes < 1 || p->ntypes > DPD_MAX_TYPES) return fail(e, DPD_ERR_PARAM, "ntypes out of range");
     if (!(p->dt > 0.0) || p->lambda < 0.0 || p->lambda > 1.0 || p->kT < 0.0)
@@ -759,10 +837,15 @@ void sort_step_t(dpd_engine* e)
     Buffers S = bufs(e, e->cur), D = bufs(e, e->cur ^ 1);
     const BoxDev B0 = e->boxes[0].B;
     hipLaunchKernelGGL((k_advance<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
-                       topo(e), e->dboxes, B0, e->cnt, e->bcnt, e->nkey, e->rank, e->flags, e->dstep);
-    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, e->stream, e->nbricks, e->bcnt, e->bstart);
-    hipLaunchKernelGGL(k_cell_start, dim3(grid(e->nbricks, 4)), dim3(256), 0, e->stream, e->nbricks, n, e->bstart,
-                       e->cnt, e->cstart);
+                       topo(e), e->dboxes, B0, e->cnt, e->gcnt, e->nkey, e->rank, e->flags, e->dstep);
+    if (e->nkeys <= kSmallScanKeys) {
+        hipLaunchKernelGGL(k_cell_scan_small, dim3(1), dim3(1024), 0, e->stream, e->nkeys, e->ngroups, n, e->cnt,
+                           e->gcnt, e->cstart);
+    } else {
+        hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e->stream, e->ngroups, e->gcnt, e->gstart);
+        hipLaunchKernelGGL(k_cell_start, dim3(grid(e->ngroups, 4)), dim3(256), 0, e->stream, e->ngroups, n, e->gstart,
+                           e->cnt, e->cstart);
+    }
     hipLaunchKernelGGL(k_place, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S.bead, e->nkey, e->rank,
                        e->cstart, e->place);
     hipLaunchKernelGGL((k_permute<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, D, e->frc,
@@ -787,7 +870,7 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr)
     PairArgs a;
     a.n = (int)e->n;
     a.nbox = (int)e->boxes.size();
-    a.nbricks = e->nbricks;
+    a.ngroups = e->ngroups;
     a.boxes = e->dboxes;
     a.pos = e->pos[e->cur];
     a.vp = e->vp;
@@ -798,24 +881,22 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr)
     a.step = e->dstep;
     a.frc = e->frc;
     a.acc = e->acc;
-    a.hitblk = e->hitblk;
-    a.nhitblk = e->nhitblk;
+    const int tb = kSchedule[e->sched].threads == 512 ? 0 : 1;
+    a.hitblk = e->hitblk[tb];
+    a.nhitblk = e->nhitblk[tb];
     const int nq = e->nbonds + e->nangles;
     // one box only: in a batch the bonded path's per-lane box raises the fused kernel to 87
     // VGPRs and costs more than the launch it saves (3.59e9 against 3.72e9 bead-steps/s for
     // 256 replicas)
     if (bonded && nq > 0 && e->pair_kernel == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask && !multi(e)) {
-        const int gh = (a.n + kHitThreads - 1) / kHitThreads;
-        const int ph = (gh + 7) / 8;
-        const dim3 gr(8 * ph + grid(nq, kHitThreads)), bl(kHitThreads);
         const BoxDev B0 = e->boxes[0].B;
         const Buffers S = bufs(e, e->cur);
-        if (obs) hipLaunchKernelGGL((k_pair_hitlist_bonded<true>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
-        else hipLaunchKernelGGL((k_pair_hitlist_bonded<false>), gr, bl, 0, e->stream, a, B0, gh, ph, S, e->slot_of, topo(e));
+        if (obs) launch_hitlist_bonded<true>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
+        else launch_hitlist_bonded<false>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
         *bonded = true;
         return hipGetLastError() == hipSuccess ? DPD_OK : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
     }
-    return launch_pair_kernel(e->pair_kernel, a, e->boxes[0].B, obs, e->stream) == hipSuccess
+    return launch_pair_kernel(e->sched, a, e->boxes[0].B, obs, e->stream) == hipSuccess
                ? DPD_OK
                : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
 }
@@ -949,38 +1030,42 @@ int commit_beads(dpd_engine* e, int64_t step0)
         BoxHost& h = e->boxes[b];
         if (h.n == 0) return fail(e, DPD_ERR_STATE, "box " + std::to_string(b) + " has no beads");
         h.bead_off = n;
-        h.B.koff = nbr * kBrickCells;
-        h.B.boff = nbr;
+        h.B.koff = nbr * kGroupKeys;
+        h.B.goff = nbr;
         h.B.box = (int)b;
-        nbr += h.B.nbricks;
+        nbr += h.B.ngroups;
         n += h.n;
     }
-    if (n > ((int64_t)1 << 30) || (int64_t)nbr * kBrickCells > ((int64_t)1 << 30))
+    if (n > ((int64_t)1 << 30) || (int64_t)nbr * kGroupKeys > ((int64_t)1 << 30))
         return fail(e, DPD_ERR_PARAM, "batch too large");
     drop_graphs(e);
     int rc;
     // hit-list workgroups of a batch: slots of one box each (a box's beads are the slots
     // [bead_off, bead_off + n) since the sort keys are ordered by box)
     if (e->boxes.size() > 1) {
-        std::vector<int4> hb;
-        for (size_t b = 0; b < e->boxes.size(); ++b) {
-            const int64_t s0 = e->boxes[b].bead_off, s1 = s0 + e->boxes[b].n;
-            for (int64_t s = s0; s < s1; s += kHitThreads)
-                hb.push_back(make_int4((int)b, (int)s, (int)std::min<int64_t>(s + kHitThreads, s1), 0));
+        for (int tb = 0; tb < 2; ++tb) {
+            const int64_t th = tb ? 256 : 512;
+            std::vector<int4> hb;
+            for (size_t b = 0; b < e->boxes.size(); ++b) {
+                const int64_t s0 = e->boxes[b].bead_off, s1 = s0 + e->boxes[b].n;
+                for (int64_t s = s0; s < s1; s += th)
+                    hb.push_back(make_int4((int)b, (int)s, (int)std::min<int64_t>(s + th, s1), 0));
+            }
+            if ((rc = dalloc(e, &e->hitblk[tb], hb.size()))) return rc;
+            HIPCHK(hipMemcpyAsync(e->hitblk[tb], hb.data(), sizeof(int4) * hb.size(), hipMemcpyHostToDevice,
+                                  e->stream));
+            HIPCHK(hipStreamSynchronize(e->stream));
+            e->nhitblk[tb] = (int)hb.size();
         }
-        if ((rc = dalloc(e, &e->hitblk, hb.size()))) return rc;
-        HIPCHK(hipMemcpyAsync(e->hitblk, hb.data(), sizeof(int4) * hb.size(), hipMemcpyHostToDevice, e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
-        e->nhitblk = (int)hb.size();
     }
-    if (nbr != e->nbricks) {
-        e->nbricks = nbr;
-        e->nkeys = nbr * kBrickCells;
+    if (nbr != e->ngroups) {
+        e->ngroups = nbr;
+        e->nkeys = nbr * kGroupKeys;
         if ((rc = dalloc(e, &e->cnt, e->nkeys)) || (rc = dalloc(e, &e->cstart, e->nkeys + 1)) ||
-            (rc = dalloc(e, &e->bcnt, e->nbricks)) || (rc = dalloc(e, &e->bstart, e->nbricks)))
+            (rc = dalloc(e, &e->gcnt, e->ngroups)) || (rc = dalloc(e, &e->gstart, e->ngroups)))
             return rc;
         HIPCHK(hipMemsetAsync(e->cnt, 0, sizeof(int) * e->nkeys, e->stream));
-        HIPCHK(hipMemsetAsync(e->bcnt, 0, sizeof(int) * e->nbricks, e->stream));
+        HIPCHK(hipMemsetAsync(e->gcnt, 0, sizeof(int) * e->ngroups, e->stream));
     }
     if (n != e->n) {
         for (int b = 0; b < 2; ++b)
@@ -1050,9 +1135,7 @@ int commit_beads(dpd_engine* e, int64_t step0)
     e->obs_valid = false;
     e->committed = true;
     if (e->pair_auto) {
-        const int k = default_pair_kernel(e->n);
-        if (k != e->pair_kernel) drop_graphs(e);
-        e->pair_kernel = k;
+        set_schedule(e, default_schedule(e));
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     return read_flags(e);
@@ -1302,14 +1385,13 @@ int dpd_create_batch(int32_t nbox, const dpd_params* p, dpd_engine** out)
         h.dt = p[b].dt; h.lambda = p[b].lambda; h.kT = p[b].kT;
         BoxDev& B = h.B;
         B.n[0] = p[b].nx; B.n[1] = p[b].ny; B.n[2] = p[b].nz;
-        for (int d = 0; d < 3; ++d) B.nb[d] = (B.n[d] + kBrick - 1) / kBrick;
-        for (int d = 0; d < 2; ++d) B.inb[d] = 1.0f / (float)B.nb[d];
+        for (int d = 0; d < 2; ++d) B.inv[d] = 1.0f / (float)B.n[d];
         B.wd[0] = p[b].wx; B.wd[1] = p[b].wy; B.wd[2] = p[b].wz;
         for (int d = 0; d < 3; ++d) B.w[d] = (float)B.wd[d];
         B.ntypes = nt;
         B.seed0 = (uint32_t)p[b].seed;
         B.seed1 = (uint32_t)(p[b].seed >> 32);
-        B.nbricks = B.nb[0] * B.nb[1] * B.nb[2];
+        B.ngroups = (int)(((int64_t)B.n[0] * B.n[1] * B.n[2] + kGroupKeys - 1) / kGroupKeys);
         B.box = b;
         set_step_constants(h);
     }
@@ -1317,9 +1399,7 @@ int dpd_create_batch(int32_t nbox, const dpd_params* p, dpd_engine** out)
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     for (auto& ev : e->ev) HIPCHK(hipEventCreate(&ev));
-    e->pair_auto = p[0].pair_kernel == DPD_PAIR_AUTO;
-    e->pair_kernel = e->pair_auto ? DPD_PAIR_FULLSHELL : p[0].pair_kernel;
-    if (e->pair_kernel < 1 || e->pair_kernel > DPD_PAIR_KINDS) return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
+    if ((rc = dpd_set_pair_kernel(e, p[0].pair_kernel))) return rc;
     if ((rc = dalloc(e, &e->flags, 4)) || (rc = dalloc(e, &e->dstep, 1)) ||
         (rc = dalloc(e, &e->acc, (size_t)nbox * kObsStride)) || (rc = dalloc(e, &e->table, (size_t)nbox * kTableStride)) ||
         (rc = dalloc(e, &e->dboxes, nbox)))
@@ -1340,10 +1420,10 @@ void dpd_destroy(dpd_engine* e)
     drop_graphs(e);
     void* ptrs[] = {e->pos[0], e->pos[1], e->vel[0], e->vel[1], e->fold[0], e->fold[1], e->bead[0], e->bead[1], e->img,
                     e->key[0], e->key[1], e->code[0], e->code[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank,
-                    e->place, e->cnt, e->bcnt,
-                    e->bstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
+                    e->place, e->cnt, e->gcnt,
+                    e->gstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
-                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj, e->hitblk};
+                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj, e->hitblk[0], e->hitblk[1]};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& ev : e->ev)
@@ -1677,19 +1757,83 @@ int dpd_time_pair(dpd_engine* e, int32_t nlaunch, double* ms_per_launch)
 int dpd_set_pair_kernel(dpd_engine* e, int32_t kernel)
 {
     if (!e) return DPD_ERR_PARAM;
+    if (kernel != DPD_PAIR_AUTO && (kernel < 1 || kernel > DPD_PAIR_KINDS))
+        return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
+    if (kernel != DPD_PAIR_AUTO && !pair_kernel_available(kernel))
+        return fail(e, DPD_ERR_UNSUPPORTED, "pair kernel " + std::to_string(kernel) + " is retired");
     e->pair_auto = kernel == DPD_PAIR_AUTO;
-    if (e->pair_auto) kernel = e->committed ? default_pair_kernel(e->n) : DPD_PAIR_FULLSHELL;
-    if (kernel < 1 || kernel > DPD_PAIR_KINDS) return fail(e, DPD_ERR_PARAM, "unknown pair kernel");
-    if (kernel != e->pair_kernel) drop_graphs(e);
-    e->pair_kernel = kernel;
+    if (e->pair_auto) set_schedule(e, e->committed ? default_schedule(e) : kFullShellSchedule);
+    else if (kernel == DPD_PAIR_FULLSHELL) set_schedule(e, kFullShellSchedule);
+    else if (kSchedule[e->sched].kernel != DPD_PAIR_HITLIST) set_schedule(e, kDefaultSchedule);
     return DPD_OK;
 }
 
 int32_t dpd_get_pair_kernel(const dpd_engine* e) { return e ? e->pair_kernel : -1; }
 
-int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step)
+int32_t dpd_schedule_count(void) { return kSchedules; }
+
+const char* dpd_schedule_name(int32_t sched)
+{
+    return (sched >= 0 && sched < kSchedules) ? kSchedule[sched].name : nullptr;
+}
+
+int dpd_set_schedule(dpd_engine* e, int32_t sched)
 {
     if (!e) return DPD_ERR_PARAM;
+    if (sched < 0 || sched >= kSchedules) return fail(e, DPD_ERR_PARAM, "unknown schedule");
+    e->pair_auto = false;
+    set_schedule(e, sched);
+    return DPD_OK;
+}
+
+int32_t dpd_get_schedule(const dpd_engine* e) { return e ? e->sched : -1; }
+
+int dpd_apply_tuning(dpd_engine* e, const char* path, int32_t* chosen)
+{
+    if (!e || !path) return DPD_ERR_PARAM;
+    if (chosen) *chosen = -1;
+    if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
+    FILE* fh = std::fopen(path, "r");
+    if (!fh) return fail(e, DPD_ERR_IO, std::string("cannot open ") + path);
+    // the engine's shape, as dpd_amd.tune keys it
+    const int64_t nbox = (int64_t)e->boxes.size();
+    const int64_t per = std::max<int64_t>(1, e->n / nbox);
+    int64_t lo = 1;
+    while (lo * 2 <= per) lo *= 2;
+    const int64_t key[5] = {nbox == 1 ? 1 : 2, lo, 2 * lo, e->boxes[0].p.ntypes, (e->nbonds + e->nangles) > 0};
+    const char* names[6] = {"\"nbox\":", "\"beads_lo\":", "\"beads_hi\":", "\"ntypes\":", "\"bonded\":", "\"schedule\":"};
+    char line[4096];
+    int found = -1;
+    while (found < 0 && std::fgets(line, sizeof(line), fh)) {
+        int64_t v[6];
+        bool ok = true;
+        for (int k = 0; k < 6 && ok; ++k) {
+            const char* p = std::strstr(line, names[k]);
+            if (!p) { ok = false; break; }
+            v[k] = std::strtoll(p + std::strlen(names[k]), nullptr, 10);
+        }
+        if (!ok) continue;
+        bool match = true;
+        for (int k = 0; k < 5; ++k) match = match && v[k] == key[k];
+        if (match && v[5] >= 0 && v[5] < kSchedules) found = (int)v[5];
+    }
+    std::fclose(fh);
+    if (found >= 0) {
+        if (kSchedule[found].kernel == DPD_PAIR_HITLIST && e->n > (int64_t)kHitSlotMask) found = kFullShellSchedule;
+        e->pair_auto = false;
+        set_schedule(e, found);
+    }
+    if (chosen) *chosen = found;
+    return DPD_OK;
+}
+
+namespace {
+
+// Time `nsteps` steps of each candidate schedule from the current state, restoring the
+// state after every trial (the simulation does not advance); ms[k] gets the time per step of
+// trial k.  The fastest is left set; *best is its schedule.
+int tune_trials(dpd_engine* e, int32_t nsteps, const std::vector<int>& trials, double* ms, int* best)
+{
     if (!e->committed) return fail(e, DPD_ERR_STATE, "no beads uploaded");
     if (nsteps < 1) return fail(e, DPD_ERR_PARAM, "nsteps must be positive");
     HIPCHK(hipSetDevice(e->device));
@@ -1742,24 +1886,63 @@ int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_
         return DPD_OK;
     };
     if ((rc = copy(true))) return rc;
-    double best = 1e300;
-    int32_t bestk = e->pair_kernel;
-    for (int v = 0; v < DPD_PAIR_KINDS; ++v) {
-        if ((rc = dpd_set_pair_kernel(e, v + 1))) break;
+    double bestms = 1e300;
+    int bests = e->sched;
+    for (size_t v = 0; v < trials.size(); ++v) {
+        set_schedule(e, trials[v]);
         if ((rc = dpd_step(e, 1, 0))) break;          // warm-up (code objects, caches)
         if ((rc = copy(false))) break;
         if ((rc = dpd_step(e, nsteps, 0))) break;
-        const double ms = e->t_total / nsteps;
-        if (ms_per_step) ms_per_step[v] = ms;
-        if (ms < best) { best = ms; bestk = v + 1; }
+        const double t = e->t_total / nsteps;
+        if (ms) ms[v] = t;
+        if (t < bestms) { bestms = t; bests = trials[v]; }
         if ((rc = copy(false))) break;
     }
     void* tmp[] = {sp, sv, sfo, sf, svp, sa, sk, sso, scs, scd, sbd};
     for (void* q : tmp) (void)hipFree(q);
     if (rc) return rc;
     e->obs_valid = false;
-    if (chosen) *chosen = bestk;
-    return dpd_set_pair_kernel(e, bestk);
+    e->pair_auto = false;   // the tuned choice stays across commits
+    set_schedule(e, bests);
+    if (best) *best = bests;
+    return DPD_OK;
+}
+
+}  // namespace
+
+int dpd_autotune(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step)
+{
+    if (!e) return DPD_ERR_PARAM;
+    const std::vector<int> trials = {kDefaultSchedule, kFullShellSchedule};
+    double ms[2] = {0.0, 0.0};
+    int best = 0;
+    int rc = tune_trials(e, nsteps, trials, ms, &best);
+    if (rc) return rc;
+    if (ms_per_step) {
+        for (int k = 0; k < DPD_PAIR_KINDS; ++k) ms_per_step[k] = 0.0;
+        ms_per_step[kSchedule[kDefaultSchedule].kernel - 1] = ms[0];
+        ms_per_step[kSchedule[kFullShellSchedule].kernel - 1] = ms[1];
+    }
+    if (chosen) *chosen = kSchedule[best].kernel;
+    return DPD_OK;
+}
+
+int dpd_autotune_schedules(dpd_engine* e, int32_t nsteps, int32_t* chosen, double* ms_per_step)
+{
+    if (!e) return DPD_ERR_PARAM;
+    std::vector<int> trials;
+    for (int k = 0; k < kSchedules; ++k)
+        if (kSchedule[k].kernel != DPD_PAIR_HITLIST || e->n <= (int64_t)kHitSlotMask) trials.push_back(k);
+    std::vector<double> ms(trials.size(), 0.0);
+    int best = 0;
+    int rc = tune_trials(e, nsteps, trials, ms.data(), &best);
+    if (rc) return rc;
+    if (ms_per_step) {
+        for (int k = 0; k < kSchedules; ++k) ms_per_step[k] = 0.0;
+        for (size_t v = 0; v < trials.size(); ++v) ms_per_step[trials[v]] = ms[v];
+    }
+    if (chosen) *chosen = best;
+    return DPD_OK;
 }
 
 namespace {

@@ -2,10 +2,11 @@
 //
 // Layout in HBM: an engine holds one or more boxes (replicas).  Beads of all boxes are
 // sorted together every step by a global cell key.  Within a box the key orders cells
-// brick-major: the box is tiled into 4x4x4-cell bricks (partial bricks at the far faces),
-// local key = brick * 64 + local cell (x fastest); a box's keys start at its key offset.
-// A brick's beads are therefore one contiguous slot range, which is what the LDS pair
-// kernel stages, and a box's beads are one contiguous slot range too.
+// row-major, x fastest: local key = cx + nx (cy + ny cz); a box's keys start at its key
+// offset, a multiple of 64 (its key range is padded to whole groups of 64 keys, the unit
+// of the two-level counting-sort scan).  A row of cells along x is therefore one
+// contiguous slot range -- the hit-list kernel scans the three cells cx-1..cx+1 of a
+// neighbour row as one range -- and a box's beads are one contiguous slot range too.
 //
 // Positions are FP32 offsets inside the bead's cell, o in [0, w), plus the integer cell
 // (from the key) and integer periodic image counts.  Differences of positions are
@@ -19,8 +20,7 @@
 
 namespace dpd {
 
-constexpr int kBrick = 4;        // cells per brick edge
-constexpr int kBrickCells = 64;  // cells per brick
+constexpr int kGroupKeys = 64;   // cell keys per scan group
 constexpr int kMaxTypes = 16;
 constexpr int kTypeStride = kMaxTypes * kMaxTypes;   // one matrix of a box's table
 constexpr int kTableStride = 3 * kTypeStride;        // a | gamma | sqrt(gamma) per box
@@ -30,25 +30,23 @@ constexpr int kObsStride = 32;                       // FP64 accumulators per bo
 // SetTimeStepConstants (src/CNTCell.cpp:143-203).
 struct BoxDev {
     int n[3];          // cells per axis
-    int nb[3];         // bricks per axis
-    float inb[2];      // 1 / nb[0], 1 / nb[1] (division by reciprocal in key_cell)
+    float inv[2];      // 1 / n[0], 1 / n[1] (division by reciprocal in key_cell)
     float w[3];        // cell widths
     double wd[3];
     float dt, hdt, hdt2, lamdt;   // dt, dt/2, dt^2/2, lambda*dt
     float invrootdt;              // sqrt(24 kT / dt)
     int ntypes;
     uint32_t seed0, seed1;        // key of the pair stream
-    int koff;                     // first global cell key of this box
-    int boff;                     // first global brick of this box
-    int nbricks;
+    int koff;                     // first global cell key of this box (a multiple of 64)
+    int goff;                     // first global scan group of this box
+    int ngroups;                  // scan groups of 64 keys (cells, padded)
     int box;                      // index of this box
 };
 
 // local key of cell (cx, cy, cz) of a box
 __host__ __device__ inline int cell_key(int cx, int cy, int cz, const BoxDev& B)
 {
-    const int brick = (cx >> 2) + B.nb[0] * ((cy >> 2) + B.nb[1] * (cz >> 2));
-    return brick * kBrickCells + (((cz & 3) * 4 + (cy & 3)) * 4 + (cx & 3));
+    return cx + B.n[0] * (cy + B.n[1] * cz);
 }
 
 // x / d for 0 <= x < 2^24, d >= 1, given inv = 1/d rounded to FP32: the FP32 estimate is
@@ -61,17 +59,13 @@ __host__ __device__ inline int div_small(int x, int d, float inv)
     return q;
 }
 
-// cell of a local key
+// cell of a local key (boxes hold at most 2^24 cells, so div_small is exact)
 __host__ __device__ inline void key_cell(int key, const BoxDev& B, int c[3])
 {
-    const int brick = key >> 6, l = key & 63;
-    const int t = div_small(brick, B.nb[0], B.inb[0]);
-    const int bx = brick - t * B.nb[0];
-    const int bz = div_small(t, B.nb[1], B.inb[1]);
-    const int by = t - bz * B.nb[1];
-    c[0] = bx * 4 + (l & 3);
-    c[1] = by * 4 + ((l >> 2) & 3);
-    c[2] = bz * 4 + (l >> 4);
+    const int t = div_small(key, B.n[0], B.inv[0]);
+    c[0] = key - t * B.n[0];
+    c[2] = div_small(t, B.n[1], B.inv[1]);
+    c[1] = t - c[2] * B.n[1];
 }
 
 __host__ __device__ inline int floordiv(int a, int n)
@@ -140,19 +134,19 @@ __device__ inline float pair_scalar(float dr2, float dxdv, float a, float sg, fl
     return (con + diss + rnd) * inv;
 }
 
-// Scan code of a bead: its offsets in the cell to 8 bits each, q = floor(256 o / w) in
-// byte 0 (x), 1 (y), 2 (z).  The hit-list kernel tests candidates on these 4 bytes instead
-// of the 16-byte position; the centre of code q, (q + 1/2) w / 256, is within w / 512 of
-// the offset on each axis.
-__device__ inline uint32_t scan_code(const float o[3], const BoxDev& B)
+// Scan code of a bead: its offsets in the cell to 8 bits each, q = floor(256 o / w), with
+// byte 0 the x offset, byte 1 the cell's x index modulo 256, byte 2 the y offset and byte 3
+// the z offset.  The hit-list kernel tests candidates on these 4 bytes instead of the
+// 16-byte position: bytes 0-1 read as one 16-bit number less 256 times a row segment's
+// first cell index give the x position in that segment in units of w/256, so a row of
+// three cells is scanned as one range.  The centre of code q, (q + 1/2) w / 256, is within
+// w / 512 of the offset on each axis.
+__device__ inline uint32_t scan_code(const float o[3], int cx, const BoxDev& B)
 {
-    uint32_t c = 0;
+    uint32_t q[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int q = (int)(o[a] * (256.0f / B.w[a]));
-        c |= (uint32_t)min(max(q, 0), 255) << (8 * a);
-    }
-    return c;
+    for (int a = 0; a < 3; ++a) q[a] = (uint32_t)min(max((int)(o[a] * (256.0f / B.w[a])), 0), 255);
+    return q[0] | ((uint32_t)(cx & 0xFF) << 8) | (q[1] << 16) | (q[2] << 24);
 }
 
 // Move a bead by displacement d from offset o in cell c (image img): new offset in

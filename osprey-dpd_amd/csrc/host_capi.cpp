@@ -561,6 +561,14 @@ int dpd_run(const char* dir, const char* run_id, int32_t device, dpd_run_stats* 
                                       s.pair_first.data(), s.pair_second.data(), s.pair_k.data(),
                                       s.pair_phi0.data())))
             return fail(rc);
+    // a schedule recorded by the autotuner for this shape: the file named by DPD_TUNE_FILE,
+    // else dpd_tune.json in the run's directory
+    {
+        const char* tf = std::getenv("DPD_TUNE_FILE");
+        const std::string tune = tf ? std::string(tf) : base + "dpd_tune.json";
+        std::ifstream probe(tune);
+        if (probe.good() && (rc = dpd_apply_tuning(e, tune.c_str(), nullptr))) return fail(rc);
+    }
     if ((rc = dpd_input_write_summary(in, (base + "dmpcis." + run_id).c_str()))) {
         put_err(err, errlen, "cannot write the dmpcis summary");
         return fail(rc);

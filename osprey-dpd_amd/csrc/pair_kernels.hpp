@@ -1,22 +1,16 @@
 // pair_kernels.hpp -- DPD pair-force kernels (CCNTCell::UpdateForce, src/CNTCell.cpp:877-1572).
 //
-// Pair kernels over the same cell-sorted SoA layout (all produce the same forces):
+// Two kernels over the same cell-sorted SoA layout (both produce the same forces):
 //   DPD_PAIR_FULLSHELL  one thread per bead visits all 27 neighbour cells and keeps only
 //                       its own force (no scatter, no atomics).  Simple and exact; the
-//                       cross-check for the brick kernel.
-//   DPD_PAIR_BRICK      one workgroup per 4x4x4-cell brick: the brick and its one-cell
-//                       halo are staged in LDS, candidate pairs are filtered by r < 1 and
-//                       compacted per wavefront, and the surviving pairs are evaluated
-//                       64 at a time with both forces accumulated in LDS (pair_brick.hip).
-//   DPD_PAIR_STREAM     the brick layout with a flat (bead, candidate) stream per home
-//                       cell, so the candidate test runs on full wavefronts (pair_stream.hip).
-//   DPD_PAIR_COMPACT    the full-shell walk with pairs in range compacted per wavefront
-//                       and evaluated 64 at a time (pair_compact.hip).
-//   DPD_PAIR_CELLS      a wavefront per home cell: the 27-cell candidate list is loaded once
-//                       per cell and tested against each of its beads (pair_cells.hip).
+//                       cross-check for the hit-list kernel and the fallback beyond 2^26
+//                       beads.
 //   DPD_PAIR_HITLIST    the full-shell walk in two passes: candidates are tested on 4-byte
-//                       scan codes (cell offsets to 8 bits), partners that may be in range
-//                       are listed in LDS, then evaluated exactly in one dense loop (below).
+//                       scan codes (cell offsets to 8 bits), a neighbour row's three cells
+//                       as one range, partners that may be in range are listed in LDS,
+//                       then evaluated exactly in one dense loop (below).
+// Both accumulate each bead's force in one thread in a fixed order, so forces are
+// deterministic for a given layout.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -28,7 +22,7 @@ namespace dpd {
 struct PairArgs {
     int n;                 // beads (all boxes)
     int nbox;
-    int nbricks;           // bricks (all boxes)
+    int ngroups;           // scan groups of 64 cell keys (all boxes)
     const BoxDev* boxes;   // [nbox]
     const float4* pos;     // offset xyz, type | box << 8 (sorted)
     const float4* vp;      // predicted velocity xyz, id bits (sorted)
@@ -133,10 +127,12 @@ __global__ __launch_bounds__(256) void k_pair_fullshell(PairArgs A, BoxDev B0, i
 // wavefront.  Partners are evaluated in candidate order with the same expressions as
 // k_pair_fullshell; the compiler contracts multiply-adds differently in the two code
 // shapes, so the forces agree to ~1 ulp per term, not bitwise.  Slots must fit in 26 bits.
-constexpr int kHitThreads = 512;
-constexpr int kHitCap = 24;     // list entries per thread (96 B of LDS each: 48 KB per workgroup)
-constexpr int kHitFlush = 24;   // after each neighbour cell, the wavefront evaluates its
-                                // lists together once any of them holds this many
+// A schedule of the hit-list kernel: threads per workgroup, list entries per thread (4 B of
+// LDS each) and the flush threshold (after each row segment the wavefront evaluates its
+// lists together once any of them holds this many; FLUSH = CAP never flushes).
+// dpd_autotune_schedules picks among these by timing them.
+constexpr int kHitThreads = 512;   // the default schedule's
+constexpr int kHitCap = 24;
 constexpr int kHitSlotBits = 26;
 constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
 // slots of padding after the last scan code: the scan reads candidates two at a time,
@@ -155,15 +151,15 @@ typedef uint32_t code2_t __attribute__((ext_vector_type(2), aligned(4)));
 // several boxes each workgroup takes slots of one box only (A.hitblk[blk] = box, first
 // slot, end slot), so the box, its constants and its table are uniform: scalar loads and
 // the table in LDS, as for one box.
-template <bool OBS, bool MULTI>
+template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH>
 __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArgs& A, const BoxDev& B0, int nblocks,
                                                                    int per_xcd)
 {
-    __shared__ uint32_t hits[kHitCap][kHitThreads];
+    __shared__ uint32_t hits[CAP][THREADS];
     __shared__ float stab[kTableStride];
     const int blk = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (blk >= nblocks) return;   // whole workgroups
-    int box = 0, i0 = blk * kHitThreads, i1 = A.n;
+    int box = 0, i0 = blk * THREADS, i1 = A.n;
     if (MULTI) {
         const int4 hb = A.hitblk[blk];
         box = hb.x; i0 = hb.y; i1 = hb.z;
@@ -172,7 +168,7 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
     // the box's interaction table in LDS, and the shift of each neighbour-cell code
     __shared__ float4 shtab[64];
     const float* gtab = A.table + (size_t)box * kTableStride;
-    for (int q = threadIdx.x; q < kTableStride; q += kHitThreads) stab[q] = gtab[q];
+    for (int q = threadIdx.x; q < kTableStride; q += THREADS) stab[q] = gtab[q];
     if (threadIdx.x < 64) {
         const int q = threadIdx.x;
         shtab[q] = make_float4((float)(1 - (q & 3)) * B.w[0], (float)(1 - ((q >> 2) & 3)) * B.w[1],
@@ -229,10 +225,10 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         // evaluate and clear this thread's list, two entries per pass (both loads in
         // flight, two independent random-stream chains), in list order
         auto drain = [&]() {
-            const int nh = (int)(wp - laddr) / (4 * kHitThreads);
+            const int nh = (int)(wp - laddr) / (4 * THREADS);
             int h = 0;
             for (; h + 1 < nh; h += 2) {
-                const uint32_t ea = lbase[h * kHitThreads], eb = lbase[(h + 1) * kHitThreads];
+                const uint32_t ea = lbase[h * THREADS], eb = lbase[(h + 1) * THREADS];
                 const int ja = (int)(ea & kHitSlotMask), jb = (int)(eb & kHitSlotMask);
                 const float4 pa = A.pos[ja], va = A.vp[ja];
                 const float4 pb = A.pos[jb], vb = A.vp[jb];
@@ -240,7 +236,7 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                 eval(eb, pb, vb);
             }
             if (h < nh) {
-                const uint32_t ea = lbase[h * kHitThreads];
+                const uint32_t ea = lbase[h * THREADS];
                 const int ja = (int)(ea & kHitSlotMask);
                 eval(ea, A.pos[ja], A.vp[ja]);
             }
@@ -253,14 +249,19 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         const float ux = B.w[0] * (1.0f / 256.0f), uy = B.w[1] * (1.0f / 256.0f), uz = B.w[2] * (1.0f / 256.0f);
         const float m = sqrtf(B.w[0] * B.w[0] + B.w[1] * B.w[1] + B.w[2] * B.w[2]) * (1.01f / 512.0f) + 1e-5f;
         const uint32_t thr = __float_as_uint((1.0f + m) * (1.0f + m));
-        auto test = [&](uint32_t jc, uint32_t cj, float qx, float qy, float qz, bool in_cell) {
-            const float ddx = qx - (float)(cj & 0xFFu) * ux;
-            const float ddy = qy - (float)((cj >> 8) & 0xFFu) * uy;
-            const float ddz = qz - (float)((cj >> 16) & 0xFFu) * uz;
+        // one candidate of a row segment whose first cell has x index fx (fxbits = (fx mod
+        // 256) << 8): code bytes 0-1 less fxbits are r * 256 + q_x, r the candidate's cell
+        // counted from the segment's first one, so its x in the segment is that times ux
+        auto test = [&](uint32_t jc, uint32_t cj, uint32_t fxbits, float qx, float qy, float qz, bool in_seg) {
+            const uint32_t x16 = (cj - fxbits) & 0xFFFFu;
+            const float ddx = qx - (float)x16 * ux;
+            const float ddy = qy - (float)((cj >> 16) & 0xFFu) * uy;
+            const float ddz = qz - (float)(cj >> 24) * uz;
             const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if ((__float_as_uint(dr2) < thr) & in_cell) {
-                *(lds_u32*)(size_t)wp = jc;
-                wp += 4 * kHitThreads;
+            if ((__float_as_uint(dr2) < thr) & in_seg) {
+                // the candidate's neighbour-cell code: the segment's base plus r in the x field
+                *(lds_u32*)(size_t)wp = jc + ((x16 >> 8) << kHitSlotBits);
+                wp += 4 * THREADS;
             }
         };
         const char* cbase = reinterpret_cast<const char*>(A.code);
@@ -270,39 +271,46 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             for (int dy = -1; dy <= 1; ++dy) {
                 int cy = c[1] + dy;
                 cy += (cy < 0) ? B.n[1] : ((cy >= B.n[1]) ? -B.n[1] : 0);
-                for (int dx = -1; dx <= 1; ++dx) {
-                    int cx = c[0] + dx;
-                    cx += (cx < 0) ? B.n[0] : ((cx >= B.n[0]) ? -B.n[0] : 0);
-                    const int k = B.koff + cell_key(cx, cy, cz, B);
-                    const int j0 = A.cstart[k], j1 = A.cstart[k + 1];
-                    // the bead's position seen from this neighbour cell (same shift table as
-                    // the evaluation), less half a code unit
-                    const uint32_t q = (uint32_t)((dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4));
-                    const uint32_t code = q << kHitSlotBits;
-                    const float4 sh = shtab[q];
-                    const float qx = (pi.x + sh.x) - 0.5f * ux, qy = (pi.y + sh.y) - 0.5f * uy,
-                                qz = (pi.z + sh.z) - 0.5f * uz;
+                // the row's cells are consecutive keys: cells cx-1 .. cx+1 are one range, or two
+                // where the row wraps through the periodic x face.  Segment s covers the cells
+                // fx .. lx (wrapped indices), the first of them at fu = -1, 0 or +1 cells from
+                // the bead's along x (unwrapped).  Candidates come in the order of a cell-by-cell
+                // walk (x innermost).
+                const int krow = B.koff + B.n[0] * (cy + B.n[1] * cz);
+                const uint32_t qrow = (uint32_t)(((dy + 1) << 2) | ((dz + 1) << 4));
+                // the bead's y and z seen from this row (the shifts of the evaluation), less
+                // half a code unit
+                const float qy = (pi.y - (float)dy * B.w[1]) - 0.5f * uy;
+                const float qz = (pi.z - (float)dz * B.w[2]) - 0.5f * uz;
+                const int cx = c[0], nx = B.n[0];
+                for (int sgm = 0; sgm < 2; ++sgm) {
+                    int fx, lx, fu;
+                    if (cx >= 1 && cx <= nx - 2) { fx = cx - 1; lx = sgm ? cx - 2 : cx + 1; fu = -1; }
+                    else if (cx == 0) { fx = sgm ? 0 : nx - 1; lx = sgm ? 1 : nx - 1; fu = sgm ? 0 : -1; }
+                    else { fx = sgm ? 0 : nx - 2; lx = sgm ? 0 : nx - 1; fu = sgm ? 1 : -1; }
+                    const int j0 = A.cstart[krow + fx], j1 = (lx >= fx) ? A.cstart[krow + lx + 1] : j0;
+                    const uint32_t fxbits = (uint32_t)(fx & 0xFF) << 8;
+                    const uint32_t code = ((uint32_t)(fu + 1) | qrow) << kHitSlotBits;
+                    const float qx = (pi.x - (float)fu * B.w[0]) - 0.5f * ux;
                     // two consecutive candidates per pass from one 8-byte load, the next
-                    // pass's two loaded before these are tested.  (Four per pass waste more
-                    // slots: a wavefront runs each cell to its lanes' longest count rounded
-                    // up to the pass width, 223 candidate slots per lane against 194 for two
-                    // and 82 candidates; one per pass loses the overlap of loads.)  The loads
-                    // run up to three codes past the cell (the next cells' beads, or the
-                    // padding after the last slot); none of those is listed.  Slots are below
-                    // 2^26, so byte offsets fit 32 bits and the loads address from the array
-                    // base.
+                    // pass's two loaded before these are tested.  (A wavefront runs each
+                    // segment to its lanes' longest count rounded up to the pass width.)  The
+                    // loads run up to three codes past the segment (the next cells' beads, or
+                    // the padding after the last slot); none of those is listed.  Slots are
+                    // below 2^26, so byte offsets fit 32 bits and the loads address from the
+                    // array base.
                     code2_t c2 = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j0 << 2));
                     for (int j = j0; j < j1; j += 2) {
                         const code2_t cn = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j << 2) + 8);
                         const uint32_t jc = (uint32_t)j | code;
-                        // room for the two: a list past kHitCap - 2 entries is evaluated
-                        // first (overflow inside a cell: this lane alone)
-                        if (wp > laddr + 4 * (kHitCap - 2) * kHitThreads) drain();
-                        test(jc, c2.x, qx, qy, qz, true);
-                        test(jc + 1, c2.y, qx, qy, qz, j + 1 < j1);
+                        // room for the two: a list past CAP - 2 entries is evaluated
+                        // first (overflow inside a segment: this lane alone)
+                        if (wp > laddr + 4 * (CAP - 2) * THREADS) drain();
+                        test(jc, c2.x, fxbits, qx, qy, qz, true);
+                        test(jc + 1, c2.y, fxbits, qx, qy, qz, j + 1 < j1);
                         c2 = cn;
                     }
-                    if (kHitFlush < kHitCap && __ballot(wp >= laddr + 4 * kHitFlush * kHitThreads) != 0) drain();
+                    if (FLUSH < CAP && __ballot(wp >= laddr + 4 * FLUSH * THREADS) != 0) drain();
                 }
             }
         }
@@ -312,66 +320,98 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             for (int q = 0; q < 10; ++q) vir[q] = 0.5 * (double)sv[q];
     }
     if (OBS) {
-        accumulate_boxes<10, kHitThreads / 64>(vir, valid ? box : -1, box, kObsVir, A.acc);
+        accumulate_boxes<10, THREADS / 64>(vir, valid ? box : -1, box, kObsVir, A.acc);
     }
 }
 
-// Six wavefronts per SIMD (three 512-thread workgroups per CU, which the 52 KB of LDS
-// also allows) need at most 80 VGPRs: the force-only kernel is held to that; the kernel
-// with observables runs once per batch of steps and is left unconstrained.
-template <bool OBS, bool MULTI>
-__global__ __launch_bounds__(kHitThreads) __attribute__((amdgpu_waves_per_eu(OBS ? 4 : 6)))
-void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
+// Six wavefronts per SIMD (three 512-thread or six 256-thread workgroups per CU, which the
+// LDS of the 24-entry lists also allows) need at most 80 VGPRs: the force-only kernel is held
+// to that; the kernel with observables runs once per batch of steps and is left freer.
+// wavefronts per SIMD that the LDS of a schedule allows (160 KB per CU, four SIMDs), at most 6
+constexpr int hit_waves(int threads, int cap)
 {
-    hitlist_body<OBS, MULTI>(A, B0, nblocks, per_xcd);
+    const int lds = cap * threads * 4 + kTableStride * 4 + 64 * 16;
+    const int w = (160 * 1024 / lds) * (threads / 64) / 4;
+    return w < 6 ? w : 6;
 }
 
-// The automatic choice by the number of beads the engine holds (all boxes).  A thread
-// per bead leaves most of the 256 CUs idle for small boxes (24k beads are 375 wavefronts),
-// where the LDS stream kernel with 1024-thread workgroups is faster (1.25x at 24k, 1.25x
-// at 41k beads); the two break even at ~57k beads, and above that the hit-list kernel is
-// fastest (1.11x at 66k, 1.13x at 98k beads; 192k-bead box 1.28x, 786k-bead box 1.48x,
-// 4M-bead batch of 256 replicas 1.85x the next best kernel before its scan took codes and
-// two candidates per pass; measured on MI355X with graph replay).
-constexpr int64_t kHitListMinBeads = 56000;
+template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(OBS ? 4 : hit_waves(THREADS, CAP))))
+void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
+{
+    hitlist_body<OBS, MULTI, THREADS, CAP, FLUSH>(A, B0, nblocks, per_xcd);
+}
 
+// The schedules of the pair launch: hit-list kernels with their parameters, and the
+// full-shell kernel.  A schedule id is what the autotuner stores.
+struct Schedule {
+    int kernel;    // DPD_PAIR_HITLIST or DPD_PAIR_FULLSHELL
+    int threads, cap, flush;
+    const char* name;
+};
+constexpr int kSchedules = 5;
+constexpr Schedule kSchedule[kSchedules] = {
+    {DPD_PAIR_HITLIST, 512, 24, 24, "hitlist-t512-c24"},
+    {DPD_PAIR_HITLIST, 512, 24, 16, "hitlist-t512-c24-f16"},
+    {DPD_PAIR_HITLIST, 256, 24, 24, "hitlist-t256-c24"},
+    {DPD_PAIR_HITLIST, 256, 32, 32, "hitlist-t256-c32"},
+    {DPD_PAIR_FULLSHELL, 256, 0, 0, "fullshell-t256"},
+};
+constexpr int kDefaultSchedule = 0;
+constexpr int kFullShellSchedule = 4;
+
+// hit-list launch of schedule S on a grid of workgroups of its thread count
+template <int S, bool OBS, bool MULTI>
+inline void launch_hitlist_s(const PairArgs& a, const BoxDev& B0, int nblocks, hipStream_t s)
+{
+    constexpr Schedule q = kSchedule[S];
+    const int ph = (nblocks + 7) / 8;
+    hipLaunchKernelGGL((k_pair_hitlist<OBS, MULTI, q.threads, q.cap, q.flush>), dim3(8 * ph), dim3(q.threads), 0, s,
+                       a, B0, nblocks, ph);
+}
+
+template <bool OBS, bool MULTI>
+inline void launch_hitlist(int sched, const PairArgs& a, const BoxDev& B0, int nblocks, hipStream_t s)
+{
+    switch (sched) {
+    case 1: launch_hitlist_s<1, OBS, MULTI>(a, B0, nblocks, s); break;
+    case 2: launch_hitlist_s<2, OBS, MULTI>(a, B0, nblocks, s); break;
+    case 3: launch_hitlist_s<3, OBS, MULTI>(a, B0, nblocks, s); break;
+    default: launch_hitlist_s<0, OBS, MULTI>(a, B0, nblocks, s); break;
+    }
+}
+
+// The automatic choice: the hit-list kernel for every engine whose slots fit its 26-bit list
+// entries, the full-shell kernel beyond.
 inline int default_pair_kernel(int64_t nbeads)
 {
-    if (nbeads < kHitListMinBeads) return DPD_PAIR_STREAM1024;
     return nbeads <= (int64_t)kHitSlotMask ? DPD_PAIR_HITLIST : DPD_PAIR_FULLSHELL;
 }
 
-hipError_t launch_pair_brick(const PairArgs& a, bool obs, hipStream_t s, int threads);
-hipError_t launch_pair_stream(const PairArgs& a, bool obs, hipStream_t s, int threads);
-hipError_t launch_pair_compact(const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s);
-hipError_t launch_pair_cells(const PairArgs& a, bool obs, hipStream_t s);
+inline bool pair_kernel_available(int which) { return which == DPD_PAIR_FULLSHELL || which == DPD_PAIR_HITLIST; }
 
-inline hipError_t launch_pair_kernel(int which, const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s)
+// The pair launch of schedule `sched`.  A batch's hit-list workgroups take the slots of one
+// box each: hitblk (nhitblk entries) must be the table built for the schedule's thread count.
+inline hipError_t launch_pair_kernel(int sched, const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s)
 {
-    if (which == DPD_PAIR_BRICK) return launch_pair_brick(a, obs, s, 256);
-    if (which == DPD_PAIR_BRICK512) return launch_pair_brick(a, obs, s, 512);
-    if (which == DPD_PAIR_STREAM) return launch_pair_stream(a, obs, s, 512);
-    if (which == DPD_PAIR_STREAM1024) return launch_pair_stream(a, obs, s, 1024);
-    if (which == DPD_PAIR_COMPACT) return launch_pair_compact(a, B0, obs, s);
-    if (which == DPD_PAIR_CELLS) return launch_pair_cells(a, obs, s);
-    const int g = (a.n + 255) / 256;
-    const int per_xcd = (g + 7) / 8;
-    const dim3 grid(8 * per_xcd);
     const bool multi = a.nbox > 1;
-    if (which == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask && (!multi || a.hitblk)) {
-        const int gh = multi ? a.nhitblk : (a.n + kHitThreads - 1) / kHitThreads;
-        const int ph = (gh + 7) / 8;
-        const dim3 gr(8 * ph), bl(kHitThreads);
+    if (sched < 0 || sched >= kSchedules) sched = kDefaultSchedule;
+    if (kSchedule[sched].kernel == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask && (!multi || a.hitblk)) {
+        const int th = kSchedule[sched].threads;
+        const int gh = multi ? a.nhitblk : (a.n + th - 1) / th;
         if (obs) {
-            if (multi) hipLaunchKernelGGL((k_pair_hitlist<true, true>), gr, bl, 0, s, a, B0, gh, ph);
-            else hipLaunchKernelGGL((k_pair_hitlist<true, false>), gr, bl, 0, s, a, B0, gh, ph);
+            if (multi) launch_hitlist<true, true>(sched, a, B0, gh, s);
+            else launch_hitlist<true, false>(sched, a, B0, gh, s);
         } else {
-            if (multi) hipLaunchKernelGGL((k_pair_hitlist<false, true>), gr, bl, 0, s, a, B0, gh, ph);
-            else hipLaunchKernelGGL((k_pair_hitlist<false, false>), gr, bl, 0, s, a, B0, gh, ph);
+            if (multi) launch_hitlist<false, true>(sched, a, B0, gh, s);
+            else launch_hitlist<false, false>(sched, a, B0, gh, s);
         }
         return hipGetLastError();
     }
     // the full-shell kernel (also for hit lists beyond 2^26 beads)
+    const int g = (a.n + 255) / 256;
+    const int per_xcd = (g + 7) / 8;
+    const dim3 grid(8 * per_xcd);
     if (obs) {
         if (multi) hipLaunchKernelGGL((k_pair_fullshell<true, true>), grid, dim3(256), 0, s, a, B0, g, per_xcd);
         else hipLaunchKernelGGL((k_pair_fullshell<true, false>), grid, dim3(256), 0, s, a, B0, g, per_xcd);

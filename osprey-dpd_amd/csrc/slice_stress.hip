@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_slice_pairs(SliceArgs A, double* __rest
     const double Lz = (double)B.n[2] * B.wd[2];
     const RngKey K = rng_key(B, *A.step);
     // the box's beads are the slots of its cell keys
-    const int s0 = A.cstart[B.koff], s1 = A.cstart[B.koff + B.nbricks * kBrickCells];
+    const int s0 = A.cstart[B.koff], s1 = A.cstart[B.koff + B.ngroups * kGroupKeys];
     for (int i = s0 + blockIdx.x * blockDim.x + threadIdx.x; i < s1; i += gridDim.x * blockDim.x) {
         const float4 pi = A.pos[i], vi = A.vp[i];
         const int ti = bead_type(__float_as_int(pi.w));
@@ -211,7 +211,7 @@ constexpr uint32_t kHalfShellMask = (1u << 11) | (1u << 14) | (1u << 16) | (1u <
 __global__ __launch_bounds__(256) void k_energy_pairs(SliceArgs A, double* __restrict__ out)
 {
     const BoxDev& B = A.B;
-    const int s0 = A.cstart[B.koff], s1 = A.cstart[B.koff + B.nbricks * kBrickCells];
+    const int s0 = A.cstart[B.koff], s1 = A.cstart[B.koff + B.ngroups * kGroupKeys];
     double e = 0.0;
     for (int i = s0 + blockIdx.x * blockDim.x + threadIdx.x; i < s1; i += gridDim.x * blockDim.x) {
         const float4 pi = A.pos[i];

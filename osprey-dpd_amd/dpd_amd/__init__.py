@@ -17,10 +17,10 @@ ERR_NAMES = {1: "PARAM", 2: "OOM", 3: "HIP", 4: "NAN", 5: "OUT_OF_BOX", 6: "STAT
 (PAIR_AUTO, PAIR_BRICK, PAIR_FULLSHELL, PAIR_BRICK512, PAIR_STREAM, PAIR_STREAM1024, PAIR_COMPACT, PAIR_CELLS,
  PAIR_HITLIST) = range(9)
 PAIR_KINDS = 8
-# device kernel of each pair-kernel id (BRICK512 and STREAM1024 are workgroup sizes of two)
-PAIR_KERNEL_NAMES = {PAIR_BRICK: "k_pair_brick", PAIR_FULLSHELL: "k_pair_fullshell", PAIR_BRICK512: "k_pair_brick",
-                     PAIR_STREAM: "k_pair_stream", PAIR_STREAM1024: "k_pair_stream", PAIR_COMPACT: "k_pair_compact",
-                     PAIR_CELLS: "k_pair_cells", PAIR_HITLIST: "k_pair_hitlist"}
+# the pair kernels the library carries (ids 1, 3-7 name retired kernels, refused with
+# ERR_UNSUPPORTED) and their device kernel names
+PAIR_AVAILABLE = (PAIR_FULLSHELL, PAIR_HITLIST)
+PAIR_KERNEL_NAMES = {PAIR_FULLSHELL: "k_pair_fullshell", PAIR_HITLIST: "k_pair_hitlist"}
 STEP_OBSERVE, NO_THERMOSTAT, STEP_TIMING, STEP_GRAPH, STEP_TIMING_SAMPLED = 1, 2, 4, 8, 16
 TOGGLE_THERMOSTAT, TOGGLE_CONSERVATIVE, TOGGLE_FORCES = 1, 2, 3
 
@@ -72,6 +72,12 @@ SIGNATURES = {
     "dpd_set_pair_kernel": (C.c_int, [_P, C.c_int32]),
     "dpd_get_pair_kernel": (C.c_int32, [_P]),
     "dpd_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P]),
+    "dpd_schedule_count": (C.c_int32, []),
+    "dpd_schedule_name": (C.c_char_p, [C.c_int32]),
+    "dpd_set_schedule": (C.c_int, [_P, C.c_int32]),
+    "dpd_get_schedule": (C.c_int32, [_P]),
+    "dpd_autotune_schedules": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P]),
+    "dpd_apply_tuning": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int32)]),
     "dpd_create_batch": (C.c_int, [C.c_int32, C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "dpd_upload_box": (C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, _P]),
     "dpd_upload_box_topology": (C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, C.c_int64, _P, _P,
@@ -251,6 +257,29 @@ class Engine:
         self._check(lib().dpd_autotune(self.h, int(nsteps), C.byref(chosen), _ptr(ms)))
         return chosen.value, {k + 1: float(ms[k]) for k in range(PAIR_KINDS)}
 
+    @property
+    def schedule(self):
+        return int(lib().dpd_get_schedule(self.h))
+
+    def set_schedule(self, sched):
+        self._check(lib().dpd_set_schedule(self.h, int(sched)))
+
+    def autotune_schedules(self, nsteps=10):
+        """Time every pair-launch schedule on the current state; keep the fastest.  Returns
+        the choice and {schedule name: ms per step}."""
+        chosen = C.c_int32()
+        n = schedule_count()
+        ms = np.zeros(n)
+        self._check(lib().dpd_autotune_schedules(self.h, int(nsteps), C.byref(chosen), _ptr(ms)))
+        return chosen.value, {schedule_name(k): float(ms[k]) for k in range(n)}
+
+    def apply_tuning(self, path):
+        """Set the schedule the tuning file records for this engine's shape (dpd_apply_tuning);
+        returns it, or None when the file has none."""
+        chosen = C.c_int32()
+        self._check(lib().dpd_apply_tuning(self.h, str(path).encode(), C.byref(chosen)))
+        return chosen.value if chosen.value >= 0 else None
+
     def zero_total_momentum(self, box=-1):
         self._check(lib().dpd_zero_total_momentum(self.h, int(box)))
 
@@ -281,6 +310,15 @@ class Engine:
         """Bond-pair stress profile (CMonitor::AddBondPairStress, src/Monitor.cpp:2520-2739):
         [nslices, ntypes**3, 9] keyed by the bead types (second head, first head, first tail)."""
         return _slice_bondpairs(self, nslices, box)
+
+
+def schedule_count():
+    return int(lib().dpd_schedule_count())
+
+
+def schedule_name(k):
+    name = lib().dpd_schedule_name(int(k))
+    return name.decode() if name else None
 
 
 def _slice_bondpairs(eng, nslices, box):

@@ -20,9 +20,10 @@ Cases:
             while the mixture demixes.
   condensate20  the condensate composition on a 20^3 box (24,000 beads): 16-mers
             (S P P P)x4 at 20% of the beads in water, a_SS = 10, other a = 25, Hookean bonds
-            k 128 l0 0.5, 4000 steps from this build's random states (State restart, as for
-            the bilayer): kT, P, the sticker-sticker g(r) and the mean bond length from
-            snapshots every 250 steps after step 2000.
+            k 128 l0 0.5, 4000 steps from this build's random states (its restatement of
+            the reference's random builder, src/RandomBuilder.cpp, handed over as State
+            restart, as for the bilayer): kT, P, the sticker-sticker g(r) and the mean bond
+            length from snapshots every 250 steps after step 2000.
 Each case runs NRUNS times from independent starting states (RNGSeed differs per run).
 The bilayer runs start from this build's lamella states (dpd_amd.host.Input.build_state
 with the run's seed), handed to the reference as restart snapshots (State restart), so

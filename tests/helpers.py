@@ -106,3 +106,33 @@ def force_errors(f, ref, floor=1.0):
     norm = rel_norm(f, ref)
     per = np.linalg.norm(f - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), floor)
     return norm, float(per.max())
+
+
+def clamped_and_bonded(d, L, head=(), tail=(), first=(), second=()):
+    """Beads whose wrapped and unwrapped coordinates of a reference restart snapshot
+    disagree (its writer moves a coordinate within 0.001 of a face to 0.001 and leaves the
+    unwrapped one, src/RestartState.cpp:492-540), together with every bead that shares a bond
+    or a bond pair with one: the reference computes pair forces from the one and bonds from
+    the other, which no single consistent position reproduces."""
+    img = (d["u"] - d["x"]) / np.asarray(L, float)
+    bad = ~(np.abs(img - np.round(img)) < 1e-5).all(axis=1)
+    head, tail = np.asarray(head, int), np.asarray(tail, int)
+    first, second = np.asarray(first, int), np.asarray(second, int)
+    out = bad.copy()
+    if len(head):
+        hit = bad[head] | bad[tail]
+        out[head[hit]] = True
+        out[tail[hit]] = True
+    if len(first):
+        a, b, c = head[second], head[first], tail[first]
+        hit = bad[a] | bad[b] | bad[c]
+        out[a[hit]] = out[b[hit]] = out[c[hit]] = True
+    return out
+
+
+def consistent(d, L):
+    """The snapshot with its wrapped coordinates taken from the unwrapped ones."""
+    L = np.asarray(L, float)
+    x = np.mod(d["u"], L)
+    x[x >= L] = 0.0
+    return dict(d, x=x)

@@ -11,10 +11,7 @@ from helpers import chain_state, force_errors, random_state, rel_norm
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [pytest.param(D.PAIR_FULLSHELL, id="fullshell"), pytest.param(D.PAIR_BRICK, id="brick"),
-           pytest.param(D.PAIR_STREAM, id="stream"), pytest.param(D.PAIR_STREAM1024, id="stream1024"),
-           pytest.param(D.PAIR_COMPACT, id="compact"), pytest.param(D.PAIR_CELLS, id="cells"),
-           pytest.param(D.PAIR_HITLIST, id="hitlist")]
+KERNELS = [pytest.param(D.PAIR_FULLSHELL, id="fullshell"), pytest.param(D.PAIR_HITLIST, id="hitlist")]
 
 
 def say(*a):

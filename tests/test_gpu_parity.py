@@ -13,10 +13,7 @@ from helpers import (SEEDS, chain_state, force_errors, load_beads, random_state,
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [pytest.param(D.PAIR_FULLSHELL, id="fullshell"), pytest.param(D.PAIR_BRICK, id="brick"),
-           pytest.param(D.PAIR_STREAM, id="stream"), pytest.param(D.PAIR_STREAM1024, id="stream1024"),
-           pytest.param(D.PAIR_COMPACT, id="compact"), pytest.param(D.PAIR_CELLS, id="cells"),
-           pytest.param(D.PAIR_HITLIST, id="hitlist")]
+KERNELS = [pytest.param(D.PAIR_FULLSHELL, id="fullshell"), pytest.param(D.PAIR_HITLIST, id="hitlist")]
 BINARY_A = np.array([[25.0, 80.0], [80.0, 25.0]])
 BINARY_G = np.array([[4.5, 4.5], [4.5, 4.5]])
 C4_A = np.array([[25.0, 25.0, 25.0], [25.0, 10.0, 25.0], [25.0, 25.0, 25.0]])
@@ -281,7 +278,7 @@ def test_hitlist_pairs_at_the_cutoff(widths):
     assert rel_norm(fh, out[D.PAIR_FULLSHELL][0]) < 1e-6
 
 
-@pytest.mark.parametrize("kernel", KERNELS + [pytest.param(D.PAIR_BRICK512, id="brick512")])
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_forces_with_wider_cells(kernel):
     """CNT cells wider than the cutoff (Box ... 1.25 1 1.5 in a dmpci file): the
     neighbour shifts and cell keys use each axis' width."""

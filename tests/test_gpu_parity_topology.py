@@ -22,12 +22,11 @@ import dpd_amd as D
 import oracle as O
 from dpd_amd import host as H
 from dpd_amd import workloads as W
-from helpers import GOLDEN, SEEDS, force_errors, load_beads, rel_norm, restart_clamp
+from helpers import GOLDEN, SEEDS, clamped_and_bonded, consistent, force_errors, load_beads, rel_norm, restart_clamp
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [pytest.param(D.PAIR_HITLIST, id="hitlist"), pytest.param(D.PAIR_FULLSHELL, id="fullshell"),
-           pytest.param(D.PAIR_STREAM1024, id="stream1024")]
+KERNELS = [pytest.param(D.PAIR_HITLIST, id="hitlist"), pytest.param(D.PAIR_FULLSHELL, id="fullshell")]
 TOPO_CASES = ["chains7", "lipid8", "lipidphi8"]
 FORCE_TOL = 1e-5
 
@@ -56,7 +55,16 @@ def engine_for(inp, topo, d, kernel, gamma=None, step0=0, seed=None):
 def test_golden_noise_free_step_with_topology(kernel, case):
     """One noise-free step (gamma = 0) from the reference's snapshot with bonds and bond
     pairs (linear chains, branched H3(T4)2 lipids, bond pairs with phi0 = 0, 30 and 60
-    degrees) against the reference's own result."""
+    degrees): against the FP64 oracle on the same reading of the snapshot within 1e-5, and
+    against the reference's own result within what the snapshot's rounding moves.
+
+    The snapshot's wrapped and unwrapped coordinates are each rounded to six digits, so they
+    disagree by up to 5e-5 for a quarter of the beads.  The reference takes pair distances
+    from the wrapped ones and bond vectors from the unwrapped ones (src/Bond.cpp:111-135,
+    src/BondPair.cpp:138-160); the engine keeps one position per bead (wrapped offset plus
+    image counts), which reads the unwrapped coordinate as x + image * L.  The oracle fed
+    that same reading is the like-for-like check; the oracle fed the snapshot as the
+    reference reads it meets the reference's result to six digits (test_oracle)."""
     inp, topo = golden_input(case)
     d = load_beads(case, "in")
     ref = load_beads(case, "nf1")
@@ -65,21 +73,37 @@ def test_golden_noise_free_step_with_topology(kernel, case):
     e.step(1)
     X, U, V, F = e.download()
     L = np.array(inp.cells, dtype=float)
-    norm, per = force_errors(F, ref["f"])
-    say(f"  force normwise {norm:.2e} per-bead {per:.2e}; v {rel_norm(V, ref['v']):.2e}")
-    assert norm < 2e-5
-    assert np.all(np.abs(V - ref["v"]) <= 2e-5 * np.abs(ref["v"]) + 1e-5)
-    # unwrapped coordinates: beads the reference's restart writer moved off a face carry up
-    # to 0.001 between their wrapped and unwrapped coordinates (see test_gpu_parity)
-    img = (d["u"] - d["x"]) / L
-    consistent = (np.abs(img - np.round(img)) < 1e-5).all(axis=1)
-    ok = np.abs(U - ref["u"]) <= 1e-5 * np.abs(ref["u"]) + 1e-5
-    assert ok[consistent].all()
+    # the oracle on the engine's reading of the snapshot
+    img = np.round((d["u"] - d["x"]) / L)
+    _, s = oracle_system(inp, topo, dict(d, u=d["x"] + img * L), SEEDS[case], gamma=np.zeros_like(inp.gamma))
+    s.run(1)
+    xo, uo, vo, fo = s.state()
+    norm_o, per_o = force_errors(F, fo)
+    # beads the restart writer moved off a face, and their bond partners, are compared
+    # loosely: the reference's pair and bond terms see two positions for them
+    aff = clamped_and_bonded(d, L, topo[0], topo[1], topo[4], topo[5])
+    ok = ~aff
+    norm, per = force_errors(F[ok], ref["f"][ok])
+    norm_ro = force_errors(fo[ok], ref["f"][ok])[0]
+    dv_ro = np.abs(vo[ok] - ref["v"][ok]).max()
+    say(f"  against the oracle: force normwise {norm_o:.2e} per-bead {per_o:.2e}, v {rel_norm(V, vo):.2e}")
+    say(f"  against the reference: force normwise {norm:.2e} per-bead {per:.2e} (oracle on the same "
+        f"reading: {norm_ro:.2e}), v max {np.abs(V[ok] - ref['v'][ok]).max():.2e} (oracle {dv_ro:.2e}); "
+        f"{int(aff.sum())} beads off a face or bonded to one")
+    assert norm_o < FORCE_TOL
+    assert np.abs(V - vo).max() < 1e-5 and np.abs(U - uo).max() < 1e-5
+    assert norm <= norm_ro + FORCE_TOL
+    assert np.abs(V[ok] - ref["v"][ok]).max() <= dv_ro + 1e-5
+    assert aff.sum() < 0.02 * len(aff)
+    # unwrapped coordinates: within the snapshot's own disagreement between the two readings
+    # (beads the restart writer moved off a face carry up to 0.001 of it)
+    gap = np.abs(d["u"] - (d["x"] + img * L))
+    assert np.all(np.abs(U - ref["u"]) <= gap + 1e-5 * np.abs(ref["u"]) + 1e-5)
     assert np.all(np.abs(restart_clamp(X, L) - ref["x"]) <= 1e-5 * np.abs(ref["x"]) + 1e-5)
 
 
-def oracle_system(inp, topo, d, seed, step0=0):
-    box = O.Box(inp.cells, inp.a, inp.gamma, seed=seed, rng_mode=O.RNG_PHILOX)
+def oracle_system(inp, topo, d, seed, step0=0, gamma=None):
+    box = O.Box(inp.cells, inp.a, inp.gamma if gamma is None else gamma, seed=seed, rng_mode=O.RNG_PHILOX)
     return box, O.System(box, d["id"], d["type"], d["x"], d["v"], d["f"], unpbc=d["u"], topo=O.Topo(*topo))
 
 
@@ -91,7 +115,8 @@ def test_golden_thermostatted_steps_track_oracle(kernel, case):
     same snapshot with the same pair stream: positions, velocities, forces and the FP64
     observables after every step."""
     inp, topo = golden_input(case)
-    d = load_beads(case, "in")
+    # both sides start from the same self-consistent state (wrapped = unwrapped mod L)
+    d = consistent(load_beads(case, "in"), inp.cells)
     seed = SEEDS[case]
     say("thermostatted steps", kernel, case)
     e = engine_for(inp, topo, d, kernel, seed=seed)
@@ -114,7 +139,7 @@ def test_golden_thermostatted_steps_track_oracle(kernel, case):
         assert np.abs(dx).max() < 1e-4
         assert abs(ob["kT"] - obr["kT"]) / obr["kT"] < 1e-4
         assert abs(ob["pressure"] - obr["pressure"]) / abs(obr["pressure"]) < 1e-4
-        assert np.allclose(ob["pe"][1:], obr["pe"][1:], rtol=1e-4, atol=1e-6)
+        assert np.allclose(ob["pe"], s.potential_energies(t + 1), rtol=1e-4, atol=1e-6)
         assert ob["step"] == t + 1
 
 
@@ -162,7 +187,7 @@ def test_commands_track_oracle(kernel):
         spec = json.load(fh)
     case = spec["case"]
     inp, topo = golden_input(case)
-    d = load_beads(case, "in")
+    d = consistent(load_beads(case, "in"), inp.cells)
     seed = SEEDS[case]
     e = engine_for(inp, topo, d, kernel, seed=seed)
     _, s = oracle_system(inp, topo, d, seed)

@@ -132,12 +132,14 @@ def test_bondpair_tables_match_oracle(nslices):
     x = np.mod(d["u"], L)
     x[x >= L] = 0.0
     box = O.Box(inp.cells, inp.a, inp.gamma, seed=3)
-    want = O.slice_stress_bondpairs(box, d["id"], d["type"], x, nslices, unpbc=d["u"],
-                                    topo=O.Topo(h, tl, k, l0, fi, se, mo, ph))
     e = D.Engine(inp.cells, inp.a, inp.gamma, seed=3, pair_kernel=D.PAIR_HITLIST)
     e.upload(d["id"], d["type"], x, d["v"], unpbc=d["u"])
     e.upload_topology(h, tl, k, l0, fi, se, mo, ph)
     got = e.slice_stress_bondpairs(nslices)
+    # the oracle at the positions the engine holds (FP32 offsets in their cells)
+    X, U = e.download()[:2]
+    want = O.slice_stress_bondpairs(box, d["id"], d["type"], X, nslices, unpbc=U,
+                                    topo=O.Topo(h, tl, k, l0, fi, se, mo, ph))
     err = rel(got, want)
     say(f"  {len(fi)} bond pairs, {nslices} slices: {err:.2e}")
     assert got.shape == want.shape

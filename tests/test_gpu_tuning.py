@@ -25,13 +25,19 @@ def test_automatic_choice_by_bead_count():
     ids, t, x, v = random_state(cells, 3000, 2, seed=8)
     e = D.Engine(cells, A2, G2, seed=3, pair_kernel=D.PAIR_AUTO)
     e.upload(ids, t, x, v)
-    assert e.pair_kernel == D.PAIR_STREAM1024   # 3000 beads: below the crossover
+    assert e.pair_kernel == D.PAIR_HITLIST
     e.set_pair_kernel(D.PAIR_FULLSHELL)
     assert e.pair_kernel == D.PAIR_FULLSHELL
     e.set_pair_kernel(D.PAIR_AUTO)
-    assert e.pair_kernel == D.PAIR_STREAM1024
+    assert e.pair_kernel == D.PAIR_HITLIST
     with pytest.raises(D.DpdError):
         e.set_pair_kernel(D.PAIR_KINDS + 1)
+    # the retired kernels are refused by name
+    for k in (D.PAIR_BRICK, D.PAIR_BRICK512, D.PAIR_STREAM, D.PAIR_STREAM1024, D.PAIR_COMPACT, D.PAIR_CELLS):
+        with pytest.raises(D.DpdError) as err:
+            e.set_pair_kernel(k)
+        assert err.value.code == 8 and "retired" in str(err.value)
+    assert e.pair_kernel == D.PAIR_HITLIST
 
 
 def test_autotune_restores_state_and_keeps_parity():
@@ -47,7 +53,9 @@ def test_autotune_restores_state_and_keeps_parity():
     chosen, ms = e.autotune(nsteps=3)
     say(f"  autotune chose {chosen}: " + ", ".join(f"{k}: {m:.3f} ms" for k, m in ms.items()))
     assert 1 <= chosen <= D.PAIR_KINDS and e.pair_kernel == chosen
-    assert all(m > 0 for m in ms.values()) and ms[chosen] == min(ms.values())
+    avail = {k: m for k, m in ms.items() if k in (D.PAIR_FULLSHELL, D.PAIR_HITLIST)}
+    assert all(m > 0 for m in avail.values()) and ms[chosen] == min(avail.values())
+    assert all(m == 0 for k, m in ms.items() if k not in avail)
     assert e.step_count == 20
     after = e.download()
     for b, a in zip(before, after):

@@ -1,13 +1,42 @@
 """The hit-list kernel's candidate test on 8-bit scan codes (pair_kernels.hpp, scan_code in
 engine_common.hpp), checked on the CPU in FP32 arithmetic: the decoded offset is within
-w/512 of the true one on each axis, and the tested distance is within the margin m the
-kernel widens its bound by, so no pair inside the cut-off can be missed."""
+w/512 of the true one on each axis, the tested distance is within the margin m the kernel
+widens its bound by, so no pair inside the cut-off can be missed, and the x field of a row
+segment (bytes 0-1 of the code less 256 times the segment's first cell index) gives the
+candidate's x in the segment for any cell index, across multiples of 256 too."""
 import numpy as np
 
 
 def encode(o, w):
-    q = np.floor(o.astype(np.float32) * np.float32(256.0) / w.astype(np.float32)).astype(np.int64)
+    """q = (int)(o * (256 / w)) in FP32, as scan_code computes it"""
+    q = np.floor(o.astype(np.float32) * (np.float32(256.0) / w.astype(np.float32))).astype(np.int64)
     return np.clip(q, 0, 255)
+
+
+def scan_code(q, cx):
+    """the 32-bit code: x offset, cell x index mod 256, y offset, z offset"""
+    q = np.asarray(q, np.uint64)
+    return (q[..., 0] | ((np.asarray(cx, np.uint64) & 0xFF) << 8) | (q[..., 1] << 16) | (q[..., 2] << 24)).astype(np.uint32)
+
+
+def test_row_segment_x_field():
+    """For a candidate in cell fx + r (r = 0, 1, 2) of a row segment starting at cell fx, the
+    kernel's (code - ((fx mod 256) << 8)) mod 2^16 is r * 256 + q_x, for first cells on
+    either side of multiples of 256 and rows of up to 2^12 cells; bytes 2 and 3 stay the y and
+    z offsets of the original code."""
+    rng = np.random.default_rng(6)
+    for fx in [0, 1, 253, 254, 255, 256, 257, 510, 511, 4093] + list(rng.integers(0, 4094, 50)):
+        for r in range(3):
+            q = rng.integers(0, 256, (64, 3))
+            c = scan_code(q, fx + r)
+            x16 = (c.astype(np.int64) - ((fx & 0xFF) << 8)) & 0xFFFF
+            assert np.array_equal(x16, r * 256 + q[:, 0])
+            assert np.array_equal((c >> 16) & 0xFF, q[:, 1]) and np.array_equal(c >> 24, q[:, 2])
+    # a plausible mistake fails: subtracting the index without the modulo breaks past 255
+    q = np.array([[10, 20, 30]])
+    c = scan_code(q, 256)
+    assert ((c.astype(np.int64) - (255 << 8)) & 0xFFFF)[0] == 256 + 10
+    assert ((c.astype(np.int64) - (255 << 8)))[0] != 256 + 10 + (20 << 16) + (30 << 24)
 
 
 def test_decode_error_within_half_a_unit():
