@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
 // workgroups come first, the bond and angle workgroups after them, so the bonded work fills
 // the CUs the pair kernel's last workgroups leave idle (the two write disjoint buffers).
 // (one box; held to 80 VGPRs like the hit-list kernel, six wavefronts per SIMD)
-template <bool OBS, int THREADS, int CAP, int FLUSH>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(OBS ? 4 : hit_waves(THREADS, CAP))))
+template <bool OBS, int THREADS, int CAP, int FLUSH, int SPLIT>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT))))
 void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buffers S,
                            const int* __restrict__ slot_of, Topo T)
 {
@@ -195,7 +195,7 @@ void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buff
         bonded_body<OBS, false, THREADS / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
         return;
     }
-    hitlist_body<OBS, false, THREADS, CAP, FLUSH>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, false, THREADS, CAP, FLUSH, SPLIT>(A, B0, nblocks, per_xcd);
 }
 
 // the fused launch of hit-list schedule S with `nq` bonds and angles
@@ -204,11 +204,11 @@ void launch_hitlist_bonded_s(const PairArgs& a, const BoxDev& B0, int nq, const 
                              const Topo& T, hipStream_t st)
 {
     constexpr Schedule q = kSchedule[S];
-    const int gh = (a.n + q.threads - 1) / q.threads;
+    const int gh = (a.n + q.beads() - 1) / q.beads();
     const int ph = (gh + 7) / 8;
     const dim3 gr(8 * ph + (nq + q.threads - 1) / q.threads), bl(q.threads);
-    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush>), gr, bl, 0, st, a, B0, gh, ph, Sb,
-                       slot_of, T);
+    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush, q.split>), gr, bl, 0, st, a, B0, gh, ph,
+                       Sb, slot_of, T);
 }
 
 template <bool OBS>
@@ -219,6 +219,8 @@ void launch_hitlist_bonded(int sched, const PairArgs& a, const BoxDev& B0, int n
     case 1: launch_hitlist_bonded_s<1, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
     case 2: launch_hitlist_bonded_s<2, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
     case 3: launch_hitlist_bonded_s<3, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 5: launch_hitlist_bonded_s<5, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 6: launch_hitlist_bonded_s<6, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
     default: launch_hitlist_bonded_s<0, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
     }
 }
@@ -676,8 +678,8 @@ struct dpd_engine {
     double* gather = nullptr;   // 12n doubles for downloads
     double* slice = nullptr;    // slice stress tables (pair | bond)
     double4* madj = nullptr;    // per-box velocity shift and scale
-    int4* hitblk[2] = {};       // hit-list workgroups of a batch (box, first slot, end slot)
-    int nhitblk[2] = {};        // for workgroups of 512 [0] and 256 [1] threads
+    int4* hitblk[kSchedules] = {};   // hit-list workgroups of a batch (box, first slot, end slot)
+    int nhitblk[kSchedules] = {};    // per schedule (its beads per workgroup)
     size_t slice_cap = 0;
     // topology
     int *bond_ptr = nullptr, *ang_ptr = nullptr;
@@ -881,9 +883,8 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr)
     a.step = e->dstep;
     a.frc = e->frc;
     a.acc = e->acc;
-    const int tb = kSchedule[e->sched].threads == 512 ? 0 : 1;
-    a.hitblk = e->hitblk[tb];
-    a.nhitblk = e->nhitblk[tb];
+    a.hitblk = e->hitblk[e->sched];
+    a.nhitblk = e->nhitblk[e->sched];
     const int nq = e->nbonds + e->nangles;
     // one box only: in a batch the bonded path's per-lane box raises the fused kernel to 87
     // VGPRs and costs more than the launch it saves (3.59e9 against 3.72e9 bead-steps/s for
@@ -1043,8 +1044,9 @@ int commit_beads(dpd_engine* e, int64_t step0)
     // hit-list workgroups of a batch: slots of one box each (a box's beads are the slots
     // [bead_off, bead_off + n) since the sort keys are ordered by box)
     if (e->boxes.size() > 1) {
-        for (int tb = 0; tb < 2; ++tb) {
-            const int64_t th = tb ? 256 : 512;
+        for (int tb = 0; tb < kSchedules; ++tb) {
+            if (kSchedule[tb].kernel != DPD_PAIR_HITLIST) continue;
+            const int64_t th = kSchedule[tb].beads();
             std::vector<int4> hb;
             for (size_t b = 0; b < e->boxes.size(); ++b) {
                 const int64_t s0 = e->boxes[b].bead_off, s1 = s0 + e->boxes[b].n;
@@ -1423,8 +1425,10 @@ void dpd_destroy(dpd_engine* e)
                     e->place, e->cnt, e->gcnt,
                     e->gstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
                     e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
-                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj, e->hitblk[0], e->hitblk[1]};
+                    e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj};
     for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (void* q : e->hitblk)
         if (q) (void)hipFree(q);
     for (auto& ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);

@@ -145,21 +145,24 @@ __device__ inline bool pair_in_range(float dr2) { return __float_as_uint(dr2) - 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 // two scan codes from one 4-byte-aligned 8-byte load
 typedef uint32_t code2_t __attribute__((ext_vector_type(2), aligned(4)));
+// beads per workgroup of THREADS threads with SPLIT lanes per bead (21 per wavefront for 3)
+constexpr int hit_beads(int threads, int split) { return split == 1 ? threads : (threads / 64) * 21; }
 
 // body of the hit-list kernel for workgroup blockIdx.x (< 8 * per_xcd); also called by the
 // kernel that runs the bonded terms in extra workgroups after the pair workgroups.  With
 // several boxes each workgroup takes slots of one box only (A.hitblk[blk] = box, first
 // slot, end slot), so the box, its constants and its table are uniform: scalar loads and
 // the table in LDS, as for one box.
-template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH>
+template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT>
 __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArgs& A, const BoxDev& B0, int nblocks,
                                                                    int per_xcd)
 {
+    static_assert(SPLIT == 1 || SPLIT == 3, "one or three lanes per bead");
     __shared__ uint32_t hits[CAP][THREADS];
     __shared__ float stab[kTableStride];
     const int blk = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (blk >= nblocks) return;   // whole workgroups
-    int box = 0, i0 = blk * THREADS, i1 = A.n;
+    int box = 0, i0 = blk * hit_beads(THREADS, SPLIT), i1 = A.n;
     if (MULTI) {
         const int4 hb = A.hitblk[blk];
         box = hb.x; i0 = hb.y; i1 = hb.z;
@@ -176,11 +179,17 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
     }
     __syncthreads();
     const int t = threadIdx.x;
-    const int i = i0 + t;
-    const bool valid = i < i1;
+    // SPLIT 3: lanes 3g, 3g+1, 3g+2 of a wavefront share bead g (21 per wavefront, lane 63
+    // idle) and scan the neighbour rows of z layers -1, 0, +1 each
+    const int lane = t & 63;
+    const int grp = SPLIT == 1 ? 0 : lane / 3;
+    const int part = SPLIT == 1 ? 0 : lane - 3 * grp;
+    const int i = SPLIT == 1 ? i0 + t : i0 + (t >> 6) * 21 + grp;
+    const bool valid = i < i1 && (SPLIT == 1 || lane < 63);
     double vir[10];
     if (OBS)
         for (int q = 0; q < 10; ++q) vir[q] = 0.0;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
     if (valid) {
         const float4 pi = A.pos[i];
         const int wi = __float_as_int(pi.w);
@@ -191,7 +200,6 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         const uint32_t idi = (uint32_t)__float_as_int(vi.w);
         int c[3];
         key_cell(A.key[i] - B.koff, B, c);
-        float fx = 0.f, fy = 0.f, fz = 0.f;
         float sv[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         // this thread's list: column t of hits, entry e at hits[e][t]; wp is the next free one
         lds_u32* const lbase = (lds_u32*)&hits[0][t];
@@ -265,7 +273,8 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             }
         };
         const char* cbase = reinterpret_cast<const char*>(A.code);
-        for (int dz = -1; dz <= 1; ++dz) {
+        const int dz0 = SPLIT == 1 ? -1 : part - 1, dz1 = SPLIT == 1 ? 1 : part - 1;
+        for (int dz = dz0; dz <= dz1; ++dz) {
             int cz = c[2] + dz;
             cz += (cz < 0) ? B.n[2] : ((cz >= B.n[2]) ? -B.n[2] : 0);
             for (int dy = -1; dy <= 1; ++dy) {
@@ -315,9 +324,18 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             }
         }
         drain();
-        A.frc[i] = make_float4(fx, fy, fz, 0.f);
+        if (SPLIT == 1) A.frc[i] = make_float4(fx, fy, fz, 0.f);
         if (OBS)
             for (int q = 0; q < 10; ++q) vir[q] = 0.5 * (double)sv[q];
+    }
+    if (SPLIT == 3) {
+        // the bead's force: its z layers -1, 0, +1 added in that order (the whole
+        // wavefront takes part in the shuffles)
+        const int b = 3 * grp;
+        const float ax = __shfl(fx, b), ay = __shfl(fy, b), az = __shfl(fz, b);
+        const float bx = __shfl(fx, b + 1), by = __shfl(fy, b + 1), bz = __shfl(fz, b + 1);
+        const float cx = __shfl(fx, b + 2), cy = __shfl(fy, b + 2), cz = __shfl(fz, b + 2);
+        if (valid && part == 0) A.frc[i] = make_float4((ax + bx) + cx, (ay + by) + cy, (az + bz) + cz, 0.f);
     }
     if (OBS) {
         accumulate_boxes<10, THREADS / 64>(vir, valid ? box : -1, box, kObsVir, A.acc);
@@ -334,12 +352,18 @@ constexpr int hit_waves(int threads, int cap)
     const int w = (160 * 1024 / lds) * (threads / 64) / 4;
     return w < 6 ? w : 6;
 }
+// the occupancy a kernel is held to: the split schedules run small boxes, which never fill
+// the SIMDs, and get the registers of four wavefronts (no spills)
+constexpr int hit_waves_eu(bool obs, int threads, int cap, int split)
+{
+    return (obs || split > 1) ? 4 : hit_waves(threads, cap);
+}
 
-template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(OBS ? 4 : hit_waves(THREADS, CAP))))
+template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT))))
 void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
 {
-    hitlist_body<OBS, MULTI, THREADS, CAP, FLUSH>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, MULTI, THREADS, CAP, FLUSH, SPLIT>(A, B0, nblocks, per_xcd);
 }
 
 // The schedules of the pair launch: hit-list kernels with their parameters, and the
@@ -347,15 +371,19 @@ void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
 struct Schedule {
     int kernel;    // DPD_PAIR_HITLIST or DPD_PAIR_FULLSHELL
     int threads, cap, flush;
+    int split;     // lanes per bead (hit list): 1, or 3 (one z layer of neighbour rows each)
     const char* name;
+    constexpr int beads() const { return hit_beads(threads, split); }   // per workgroup
 };
-constexpr int kSchedules = 5;
+constexpr int kSchedules = 7;
 constexpr Schedule kSchedule[kSchedules] = {
-    {DPD_PAIR_HITLIST, 512, 24, 24, "hitlist-t512-c24"},
-    {DPD_PAIR_HITLIST, 512, 24, 16, "hitlist-t512-c24-f16"},
-    {DPD_PAIR_HITLIST, 256, 24, 24, "hitlist-t256-c24"},
-    {DPD_PAIR_HITLIST, 256, 32, 32, "hitlist-t256-c32"},
-    {DPD_PAIR_FULLSHELL, 256, 0, 0, "fullshell-t256"},
+    {DPD_PAIR_HITLIST, 512, 24, 24, 1, "hitlist-t512-c24"},
+    {DPD_PAIR_HITLIST, 512, 24, 16, 1, "hitlist-t512-c24-f16"},
+    {DPD_PAIR_HITLIST, 256, 24, 24, 1, "hitlist-t256-c24"},
+    {DPD_PAIR_HITLIST, 256, 32, 32, 1, "hitlist-t256-c32"},
+    {DPD_PAIR_FULLSHELL, 256, 0, 0, 1, "fullshell-t256"},
+    {DPD_PAIR_HITLIST, 256, 12, 12, 3, "hitlist-split3-t256-c12"},
+    {DPD_PAIR_HITLIST, 128, 12, 12, 3, "hitlist-split3-t128-c12"},
 };
 constexpr int kDefaultSchedule = 0;
 constexpr int kFullShellSchedule = 4;
@@ -366,8 +394,8 @@ inline void launch_hitlist_s(const PairArgs& a, const BoxDev& B0, int nblocks, h
 {
     constexpr Schedule q = kSchedule[S];
     const int ph = (nblocks + 7) / 8;
-    hipLaunchKernelGGL((k_pair_hitlist<OBS, MULTI, q.threads, q.cap, q.flush>), dim3(8 * ph), dim3(q.threads), 0, s,
-                       a, B0, nblocks, ph);
+    hipLaunchKernelGGL((k_pair_hitlist<OBS, MULTI, q.threads, q.cap, q.flush, q.split>), dim3(8 * ph), dim3(q.threads),
+                       0, s, a, B0, nblocks, ph);
 }
 
 template <bool OBS, bool MULTI>
@@ -377,6 +405,8 @@ inline void launch_hitlist(int sched, const PairArgs& a, const BoxDev& B0, int n
     case 1: launch_hitlist_s<1, OBS, MULTI>(a, B0, nblocks, s); break;
     case 2: launch_hitlist_s<2, OBS, MULTI>(a, B0, nblocks, s); break;
     case 3: launch_hitlist_s<3, OBS, MULTI>(a, B0, nblocks, s); break;
+    case 5: launch_hitlist_s<5, OBS, MULTI>(a, B0, nblocks, s); break;
+    case 6: launch_hitlist_s<6, OBS, MULTI>(a, B0, nblocks, s); break;
     default: launch_hitlist_s<0, OBS, MULTI>(a, B0, nblocks, s); break;
     }
 }
@@ -397,8 +427,8 @@ inline hipError_t launch_pair_kernel(int sched, const PairArgs& a, const BoxDev&
     const bool multi = a.nbox > 1;
     if (sched < 0 || sched >= kSchedules) sched = kDefaultSchedule;
     if (kSchedule[sched].kernel == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask && (!multi || a.hitblk)) {
-        const int th = kSchedule[sched].threads;
-        const int gh = multi ? a.nhitblk : (a.n + th - 1) / th;
+        const int nb = kSchedule[sched].beads();
+        const int gh = multi ? a.nhitblk : (a.n + nb - 1) / nb;
         if (obs) {
             if (multi) launch_hitlist<true, true>(sched, a, B0, gh, s);
             else launch_hitlist<true, false>(sched, a, B0, gh, s);
