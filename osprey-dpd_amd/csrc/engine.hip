@@ -184,7 +184,7 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
 // workgroups come first, the bond and angle workgroups after them, so the bonded work fills
 // the CUs the pair kernel's last workgroups leave idle (the two write disjoint buffers).
 // (one box; held to 80 VGPRs like the hit-list kernel, six wavefronts per SIMD)
-template <bool OBS, int THREADS, int CAP, int FLUSH, int SPLIT>
+template <bool OBS, int THREADS, int CAP, int FLUSH, int SPLIT, bool EPI>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT))))
 void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buffers S,
                            const int* __restrict__ slot_of, Topo T)
@@ -195,11 +195,11 @@ void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buff
         bonded_body<OBS, false, THREADS / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
         return;
     }
-    hitlist_body<OBS, false, THREADS, CAP, FLUSH, SPLIT>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, false, THREADS, CAP, FLUSH, SPLIT, EPI>(A, B0, nblocks, per_xcd);
 }
 
 // the fused launch of hit-list schedule S with `nq` bonds and angles
-template <int S, bool OBS>
+template <int S, bool OBS, bool EPI>
 void launch_hitlist_bonded_s(const PairArgs& a, const BoxDev& B0, int nq, const Buffers& Sb, const int* slot_of,
                              const Topo& T, hipStream_t st)
 {
@@ -207,21 +207,21 @@ void launch_hitlist_bonded_s(const PairArgs& a, const BoxDev& B0, int nq, const 
     const int gh = (a.n + q.beads() - 1) / q.beads();
     const int ph = (gh + 7) / 8;
     const dim3 gr(8 * ph + (nq + q.threads - 1) / q.threads), bl(q.threads);
-    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush, q.split>), gr, bl, 0, st, a, B0, gh, ph,
-                       Sb, slot_of, T);
+    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush, q.split, EPI>), gr, bl, 0, st, a, B0, gh,
+                       ph, Sb, slot_of, T);
 }
 
-template <bool OBS>
+template <bool OBS, bool EPI>
 void launch_hitlist_bonded(int sched, const PairArgs& a, const BoxDev& B0, int nq, const Buffers& Sb,
                            const int* slot_of, const Topo& T, hipStream_t st)
 {
     switch (sched) {
-    case 1: launch_hitlist_bonded_s<1, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
-    case 2: launch_hitlist_bonded_s<2, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
-    case 3: launch_hitlist_bonded_s<3, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
-    case 5: launch_hitlist_bonded_s<5, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
-    case 6: launch_hitlist_bonded_s<6, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
-    default: launch_hitlist_bonded_s<0, OBS>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 1: launch_hitlist_bonded_s<1, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 2: launch_hitlist_bonded_s<2, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 3: launch_hitlist_bonded_s<3, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 5: launch_hitlist_bonded_s<5, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 6: launch_hitlist_bonded_s<6, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
+    default: launch_hitlist_bonded_s<0, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     }
 }
 
@@ -259,6 +259,61 @@ __global__ __launch_bounds__(256) void k_bonded_sum(Topo T, const int* __restric
     float4 f = frc[s];
     f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
     frc[s] = f;
+}
+
+// The step's end and the next step's start for the beads with bonded terms, after a pair
+// launch that carried the epilogue for the others: each bonded bead's bond and angle forces
+// added to its pair force in place, then the arithmetic of k_advance (MODE kEndBegin).  One
+// thread per bead that has bonds.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_advance_bonded(Topo T, const int* __restrict__ slot_of, Buffers S,
+                                                        float4* __restrict__ frc, const BoxDev* boxes, BoxDev B0,
+                                                        int* __restrict__ cnt, int* __restrict__ gcnt,
+                                                        int* __restrict__ nkey, int* __restrict__ rank,
+                                                        int* __restrict__ flags)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = t < T.nbonded;
+    int k = 0, s = 0;
+    if (active) {
+        const int q = T.bonded[t];
+        s = slot_of[q];
+        const float4 p = S.pos[s];
+        const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
+        float4 v = S.vel[s];
+        float4 f = frc[s];
+        double fb[3];
+        bonded_gather(q, T, fb);
+        f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
+        frc[s] = f;
+        const float4 fo = S.fold[s];
+        v.x = v.x + B.hdt * (f.x + fo.x);
+        v.y = v.y + B.hdt * (f.y + fo.y);
+        v.z = v.z + B.hdt * (f.z + fo.z);
+        S.vel[s] = v;
+        if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
+            atomicAdd(flags + 0, 1);
+        int c[3];
+        key_cell(S.key[s] - B.koff, B, c);
+        const float o[3] = {p.x, p.y, p.z};
+        const int img[3] = {0, 0, 0};
+        const float d[3] = {B.dt * v.x + B.hdt2 * f.x, B.dt * v.y + B.hdt2 * f.y, B.dt * v.z + B.hdt2 * f.z};
+        Moved m;
+        advance(o, c, img, d, B, m);
+        k = B.koff + cell_key(m.c[0], m.c[1], m.c[2], B);
+        nkey[s] = k;
+        if (m.jump) atomicAdd(flags + 1, 1);
+        if (m.bad) atomicAdd(flags + 0, 1);
+    }
+    const int lane = threadIdx.x & 63;
+    int r, size, leader;
+    wave_groups(k, active, r, size, leader);
+    int base = 0;
+    if (active && lane == leader) base = atomicAdd(cnt + k, size);
+    base = __shfl(base, leader, 64);
+    if (active) rank[s] = base + r;
+    wave_groups(k >> 6, active, r, size, leader);
+    if (active && lane == leader) atomicAdd(gcnt + (k >> 6), size);
 }
 
 // Close the previous step (kEndBegin) and start the next one: new cell key and rank.
@@ -327,85 +382,89 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
     if (active && lane == leader) atomicAdd(gcnt + (k >> 6), size);
 }
 
-// exclusive scan of the scan groups' counts (one workgroup: each thread a contiguous run of
-// groups, wavefront shuffle scans, one cross-wavefront step), clears the counts
-__global__ __launch_bounds__(1024) void k_group_scan(int ngroups, int* __restrict__ gcnt, int* __restrict__ gstart)
+// Exclusive scan of in[0, m) into out, in[] cleared, by one workgroup of 1024 threads: in
+// rounds of kScanRound entries, wavefront w takes up to kScanChunks consecutive chunks of 64
+// entries, loads all of them at once (one coalesced load per chunk, all in flight), scans each
+// chunk with shuffles in order into LDS, and the 16 wavefront totals are added in order through
+// LDS.  ex: LDS of kScanRound ints.
+constexpr int kScanChunks = 8;
+constexpr int kScanRound = 16 * 64 * kScanChunks;
+__device__ inline void scan_block(int* in, int* __restrict__ out, int m, int* ex, int* wsum, int* carry)
 {
-    __shared__ int wsum[16];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int per = (ngroups + 1023) / 1024;
-    const int b0 = min(ngroups, t * per), b1 = min(ngroups, b0 + per);
-    int s = 0;
-    for (int b = b0; b < b1; ++b) s += gcnt[b];
-    int incl = s;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
-    }
-    if (lane == 63) wsum[wave] = incl;
+    if (t == 0) *carry = 0;
     __syncthreads();
-    if (wave == 0) {
-        const int w = lane < 16 ? wsum[lane] : 0;
-        int wi = w;
+    for (int base = 0; base < m; base += kScanRound) {
+        const int nch = (min(m - base, kScanRound) + 63) >> 6;   // chunks this round
+        const int per = (nch + 15) >> 4;                          // chunks per wavefront
+        const int ch0 = wave * per;
+        int v[kScanChunks];
 #pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            const int y = __shfl_up(wi, off, 64);
-            if (lane >= off) wi += y;
+        for (int j = 0; j < kScanChunks; ++j) {
+            const int k = base + ((ch0 + j) << 6) + lane;
+            v[j] = (j < per && k < m) ? in[k] : 0;
         }
-        if (lane < 16) wsum[lane] = wi - w;
-    }
-    __syncthreads();
-    int run = wsum[wave] + incl - s;
-    for (int b = b0; b < b1; ++b) {
-        const int c = gcnt[b];
-        gstart[b] = run;
-        run += c;
-        gcnt[b] = 0;
+        int run = 0;
+#pragma unroll
+        for (int j = 0; j < kScanChunks; ++j) {
+            int incl = v[j];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += y;
+            }
+            if (j < per) ex[((ch0 + j) << 6) + lane] = run + incl - v[j];
+            run += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) wsum[wave] = run;
+        __syncthreads();
+        int off = *carry;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        for (int j = 0; j < per; ++j) {
+            const int k = base + ((ch0 + j) << 6) + lane;
+            if (k < m) {
+                out[k] = off + ex[((ch0 + j) << 6) + lane];
+                in[k] = 0;
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            int tot = 0;
+            for (int w = 0; w < 16; ++w) tot += wsum[w];
+            *carry += tot;
+        }
+        __syncthreads();
     }
 }
 
-// Engines of up to kSmallScanKeys cell keys: the cell starts in one launch (one workgroup,
-// each thread a contiguous run of keys, wavefront shuffle scans, one cross-wavefront step),
-// which replaces k_group_scan + k_cell_start; clears the cell and group counts.
+// exclusive scan of the scan groups' counts (one workgroup), clears the counts
+// `step` (may be null): the simulation time to advance, when the epilogue of the previous
+// pair launch did the work of k_advance
+__global__ __launch_bounds__(1024) void k_group_scan(int ngroups, int* __restrict__ gcnt, int* __restrict__ gstart,
+                                                     int64_t* __restrict__ step)
+{
+    __shared__ int ex[kScanRound];
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    if (step && threadIdx.x == 0) *step += 1;
+    scan_block(gcnt, gstart, ngroups, ex, wsum, &carry);
+}
+
+// Engines of up to kSmallScanKeys cell keys: the cell starts in one launch (scan_block over
+// the cell counts), which replaces k_group_scan + k_cell_start; clears the cell and group
+// counts.
 constexpr int kSmallScanKeys = 16384;
 __global__ __launch_bounds__(1024) void k_cell_scan_small(int nkeys, int ngroups, int n, int* __restrict__ cnt,
-                                                          int* __restrict__ gcnt, int* __restrict__ cstart)
+                                                          int* __restrict__ gcnt, int* __restrict__ cstart,
+                                                          int64_t* __restrict__ step)
 {
+    __shared__ int ex[kScanRound];
     __shared__ int wsum[16];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int per = (nkeys + 1023) / 1024;
-    const int k0 = min(nkeys, t * per), k1 = min(nkeys, k0 + per);
-    int s = 0;
-    for (int k = k0; k < k1; ++k) s += cnt[k];
-    int incl = s;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    if (wave == 0) {
-        const int w = lane < 16 ? wsum[lane] : 0;
-        int wi = w;
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            const int y = __shfl_up(wi, off, 64);
-            if (lane >= off) wi += y;
-        }
-        if (lane < 16) wsum[lane] = wi - w;
-    }
-    __syncthreads();
-    int run = wsum[wave] + incl - s;
-    for (int k = k0; k < k1; ++k) {
-        const int c = cnt[k];
-        cstart[k] = run;
-        run += c;
-        cnt[k] = 0;
-    }
-    for (int g = t; g < ngroups; g += 1024) gcnt[g] = 0;
-    if (t == 0) cstart[nkeys] = n;
+    __shared__ int carry;
+    if (step && threadIdx.x == 0) *step += 1;
+    scan_block(cnt, cstart, nkeys, ex, wsum, &carry);
+    for (int g = threadIdx.x; g < ngroups; g += 1024) gcnt[g] = 0;
+    if (threadIdx.x == 0) cstart[nkeys] = n;
 }
 
 // cell starts: one wavefront per scan group scans its 64 cell counts; clears the counts
@@ -831,20 +890,25 @@ Topo topo(dpd_engine* e)
 inline int grid(int64_t n, int b) { return (int)((n + b - 1) / b); }
 inline bool multi(dpd_engine* e) { return e->boxes.size() > 1; }
 
-// counting sort of buffer `cur` into buffer `cur^1`; MODE decides the move
+// counting sort of buffer `cur` into buffer `cur^1`; MODE decides the move.  `advanced`: the
+// previous pair launch's epilogue (and k_advance_bonded) already closed the step and computed
+// the new keys and ranks (MODE kEndBegin only), so k_advance is left out and the scan advances
+// the simulation time.
 template <int MODE, bool MULTI>
-void sort_step_t(dpd_engine* e)
+void sort_step_t(dpd_engine* e, bool advanced)
 {
     const int n = (int)e->n;
     Buffers S = bufs(e, e->cur), D = bufs(e, e->cur ^ 1);
     const BoxDev B0 = e->boxes[0].B;
-    hipLaunchKernelGGL((k_advance<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
-                       topo(e), e->dboxes, B0, e->cnt, e->gcnt, e->nkey, e->rank, e->flags, e->dstep);
+    if (!advanced)
+        hipLaunchKernelGGL((k_advance<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
+                           topo(e), e->dboxes, B0, e->cnt, e->gcnt, e->nkey, e->rank, e->flags, e->dstep);
+    int64_t* bump = advanced ? e->dstep : nullptr;
     if (e->nkeys <= kSmallScanKeys) {
         hipLaunchKernelGGL(k_cell_scan_small, dim3(1), dim3(1024), 0, e->stream, e->nkeys, e->ngroups, n, e->cnt,
-                           e->gcnt, e->cstart);
+                           e->gcnt, e->cstart, bump);
     } else {
-        hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e->stream, e->ngroups, e->gcnt, e->gstart);
+        hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e->stream, e->ngroups, e->gcnt, e->gstart, bump);
         hipLaunchKernelGGL(k_cell_start, dim3(grid(e->ngroups, 4)), dim3(256), 0, e->stream, e->ngroups, n, e->gstart,
                            e->cnt, e->cstart);
     }
@@ -856,20 +920,35 @@ void sort_step_t(dpd_engine* e)
 }
 
 template <int MODE>
-int sort_step(dpd_engine* e)
+int sort_step(dpd_engine* e, bool advanced = false)
 {
-    if (multi(e)) sort_step_t<MODE, true>(e);
-    else sort_step_t<MODE, false>(e);
+    if (multi(e)) sort_step_t<MODE, true>(e, advanced);
+    else sort_step_t<MODE, false>(e, advanced);
     HIPCHK(hipGetLastError());
     return DPD_OK;
 }
 
+// whether the pair launch of a step that another follows can carry the step's epilogue
+inline bool epilogue_ok(const dpd_engine* e) { return epilogue_available(e->sched, e->n); }
+
 // Pair forces of this step.  With the hit-list kernel the bond and angle terms run in the
 // same launch (k_pair_hitlist_bonded); `bonded` then tells launch_bonded to add only the sums.
-int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr)
+// `epi` (epilogue_ok): the launch also closes the step for the beads without bonded terms and
+// starts the next one (pair_epilogue); launch_advance_bonded must follow the bonded terms.
+int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr, bool epi = false)
 {
     if (bonded) *bonded = false;
     PairArgs a;
+    {
+        const Buffers S = bufs(e, e->cur);
+        a.vel = S.vel;
+        a.fold = S.fold;
+        a.nkey = e->nkey;
+        a.rank = e->rank;
+        a.cnt = e->cnt;
+        a.gcnt = e->gcnt;
+        a.flags = e->flags;
+    }
     a.n = (int)e->n;
     a.nbox = (int)e->boxes.size();
     a.ngroups = e->ngroups;
@@ -892,14 +971,30 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr)
     if (bonded && nq > 0 && e->pair_kernel == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask && !multi(e)) {
         const BoxDev B0 = e->boxes[0].B;
         const Buffers S = bufs(e, e->cur);
-        if (obs) launch_hitlist_bonded<true>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
-        else launch_hitlist_bonded<false>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
+        if (obs) launch_hitlist_bonded<true, false>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
+        else if (epi) launch_hitlist_bonded<false, true>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
+        else launch_hitlist_bonded<false, false>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
         *bonded = true;
         return hipGetLastError() == hipSuccess ? DPD_OK : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
     }
-    return launch_pair_kernel(e->sched, a, e->boxes[0].B, obs, e->stream) == hipSuccess
+    return launch_pair_kernel(e->sched, a, e->boxes[0].B, obs, e->stream, epi) == hipSuccess
                ? DPD_OK
                : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
+}
+
+// after a pair launch with the epilogue and this step's bonded terms: the bonded beads' sums,
+// UpdateMom, UpdatePos, keys and ranks (k_advance_bonded)
+void launch_advance_bonded(dpd_engine* e)
+{
+    if (e->nbonded == 0) return;
+    const Buffers S = bufs(e, e->cur);
+    const BoxDev B0 = e->boxes[0].B;
+    if (multi(e))
+        hipLaunchKernelGGL((k_advance_bonded<true>), dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, topo(e),
+                           e->slot_of, S, e->frc, e->dboxes, B0, e->cnt, e->gcnt, e->nkey, e->rank, e->flags);
+    else
+        hipLaunchKernelGGL((k_advance_bonded<false>), dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, topo(e),
+                           e->slot_of, S, e->frc, e->dboxes, B0, e->cnt, e->gcnt, e->nkey, e->rank, e->flags);
 }
 
 template <bool UPDATE_V, bool OBS>
@@ -1004,17 +1099,23 @@ int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, int timing)
     int rc;
     launch_zero(e, false);
     int64_t ti = 0;   // timed launches so far (event pair 2 ti, 2 ti + 1)
+    bool advanced = false;   // the previous pair launch carried the step's epilogue
     for (int64_t t = 0; t < nsteps; ++t) {
-        rc = (t == 0) ? sort_step<kBegin>(e) : sort_step<kEndBegin>(e);
+        rc = (t == 0) ? sort_step<kBegin>(e) : sort_step<kEndBegin>(e, advanced);
         if (rc) return rc;
+        const bool last = t == nsteps - 1;
+        const bool epi = !last && epilogue_ok(e);
         const bool timed = timed_step(t, timing);
         if (timed) HIPCHK(hipEventRecord(e->pev[2 * ti], e->stream));
         bool terms_done = false;
-        if ((rc = launch_pair(e, obs && t == nsteps - 1, &terms_done))) return rc;
+        if ((rc = launch_pair(e, obs && last, &terms_done, epi))) return rc;
         if (timed) HIPCHK(hipEventRecord(e->pev[2 * ti++ + 1], e->stream));
-        // the bonded sum of every step but the last is added by the next step's k_advance
-        if (obs && t == nsteps - 1) launch_bonded<true>(e, terms_done);
-        else launch_bonded<false>(e, terms_done, t == nsteps - 1);
+        // the bonded sum of every step but the last is added by the next step's k_advance, or
+        // by k_advance_bonded right after the terms when the pair launch carried the epilogue
+        if (obs && last) launch_bonded<true>(e, terms_done);
+        else launch_bonded<false>(e, terms_done, last);
+        if (epi) launch_advance_bonded(e);
+        advanced = epi;
     }
     if (obs) launch_finish<true, true>(e);
     else launch_finish<true, false>(e);
@@ -1744,13 +1845,29 @@ int dpd_time_pair(dpd_engine* e, int32_t nlaunch, double* ms_per_launch)
     const size_t f4 = sizeof(float4) * (size_t)e->n;
     float4* keep = e->fold[e->cur ^ 1];
     HIPCHK(hipMemcpyAsync(keep, e->frc, f4, hipMemcpyDeviceToDevice, e->stream));
+    // the launch a step that another follows uses: with the epilogue where the schedule has
+    // one, which also updates the velocities in place (kept in the inactive buffer), counts
+    // the counting sort's cells and may raise the flags (restored below)
+    const bool epi = epilogue_ok(e);
+    float4* keepv = e->vel[e->cur ^ 1];
+    int flags0[2] = {0, 0};
+    if (epi) {
+        HIPCHK(hipMemcpyAsync(keepv, e->vel[e->cur], f4, hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(flags0, e->flags, sizeof(flags0), hipMemcpyDeviceToHost, e->stream));
+    }
     bool terms_done = false;
-    if ((rc = launch_pair(e, false, &terms_done))) return rc;   // untimed first launch
+    if ((rc = launch_pair(e, false, &terms_done, epi))) return rc;   // untimed first launch
     HIPCHK(hipEventRecord(e->ev[0], e->stream));
     for (int32_t i = 0; i < nlaunch; ++i)
-        if ((rc = launch_pair(e, false, &terms_done))) return rc;
+        if ((rc = launch_pair(e, false, &terms_done, epi))) return rc;
     HIPCHK(hipEventRecord(e->ev[1], e->stream));
     HIPCHK(hipMemcpyAsync(e->frc, keep, f4, hipMemcpyDeviceToDevice, e->stream));
+    if (epi) {
+        HIPCHK(hipMemcpyAsync(e->vel[e->cur], keepv, f4, hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(e->flags, flags0, sizeof(flags0), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemsetAsync(e->cnt, 0, sizeof(int) * (size_t)e->nkeys, e->stream));
+        HIPCHK(hipMemsetAsync(e->gcnt, 0, sizeof(int) * (size_t)e->ngroups, e->stream));
+    }
     HIPCHK(hipStreamSynchronize(e->stream));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));

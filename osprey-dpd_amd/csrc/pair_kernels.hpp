@@ -35,6 +35,16 @@ struct PairArgs {
     double* acc;           // out (OBS): per box [6..14] virial sum dx_b F_a, [15] pair PE
     const int4* hitblk;    // hit-list workgroups of a batch: (box, first slot, end slot)
     int nhitblk;
+    // the epilogue of a step that another follows (hit-list kernels): UpdateMom of this step,
+    // UpdatePos of the next, the new cell key and counting-sort rank, for every bead without
+    // bonded terms (k_advance_bonded does the others once their bonded forces are known)
+    float4* vel;           // velocity, updated in place
+    const float4* fold;    // force of the previous step
+    int* nkey;             // new cell key per slot
+    int* rank;             // rank inside the new cell
+    int* cnt;              // beads per cell key (counting sort)
+    int* gcnt;             // beads per scan group
+    int* flags;            // [0] non-finite force or velocity, [1] moves of more than one cell
 };
 
 constexpr int kObsVir = 6;   // virial slots in the per-box accumulator
@@ -148,16 +158,61 @@ typedef uint32_t code2_t __attribute__((ext_vector_type(2), aligned(4)));
 // beads per workgroup of THREADS threads with SPLIT lanes per bead (21 per wavefront for 3)
 constexpr int hit_beads(int threads, int split) { return split == 1 ? threads : (threads / 64) * 21; }
 
+// The step's end and the next step's start for slot i, whose force (Fx, Fy, Fz) this launch
+// computed: the arithmetic of k_advance (MODE kEndBegin) for a bead without bonded terms.
+// Every lane of the wavefront calls it (the counting-sort ranks group the wavefront's keys);
+// `act` marks the lanes that hold a finished bead.
+__device__ inline void pair_epilogue(const PairArgs& A, const BoxDev& B, int i, bool act, float Fx, float Fy,
+                                     float Fz)
+{
+    int k = 0;
+    if (act) {
+        const float4 p = A.pos[i];
+        act = !(__float_as_int(p.w) & kBondedBit);
+        if (act) {
+            const float4 fo = A.fold[i];
+            float4 v = A.vel[i];
+            v.x = v.x + B.hdt * (Fx + fo.x);
+            v.y = v.y + B.hdt * (Fy + fo.y);
+            v.z = v.z + B.hdt * (Fz + fo.z);
+            A.vel[i] = v;
+            if (!(fabsf(Fx) + fabsf(Fy) + fabsf(Fz) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
+                atomicAdd(A.flags + 0, 1);
+            int c[3];
+            key_cell(A.key[i] - B.koff, B, c);
+            const float o[3] = {p.x, p.y, p.z};
+            const int img[3] = {0, 0, 0};   // the image change is applied by k_permute
+            const float d[3] = {B.dt * v.x + B.hdt2 * Fx, B.dt * v.y + B.hdt2 * Fy, B.dt * v.z + B.hdt2 * Fz};
+            Moved m;
+            advance(o, c, img, d, B, m);
+            k = B.koff + cell_key(m.c[0], m.c[1], m.c[2], B);
+            A.nkey[i] = k;
+            if (m.jump) atomicAdd(A.flags + 1, 1);
+            if (m.bad) atomicAdd(A.flags + 0, 1);
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    int r, size, leader;
+    wave_groups(k, act, r, size, leader);
+    int base = 0;
+    if (act && lane == leader) base = atomicAdd(A.cnt + k, size);
+    base = __shfl(base, leader, 64);
+    if (act) A.rank[i] = base + r;
+    wave_groups(k >> 6, act, r, size, leader);
+    if (act && lane == leader) atomicAdd(A.gcnt + (k >> 6), size);
+}
+
 // body of the hit-list kernel for workgroup blockIdx.x (< 8 * per_xcd); also called by the
 // kernel that runs the bonded terms in extra workgroups after the pair workgroups.  With
 // several boxes each workgroup takes slots of one box only (A.hitblk[blk] = box, first
 // slot, end slot), so the box, its constants and its table are uniform: scalar loads and
 // the table in LDS, as for one box.
-template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT>
+template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT, bool EPI>
 __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArgs& A, const BoxDev& B0, int nblocks,
                                                                    int per_xcd)
 {
     static_assert(SPLIT == 1 || SPLIT == 3, "one or three lanes per bead");
+    static_assert(!(OBS && EPI), "the observing launch ends a sequence of steps");
     __shared__ uint32_t hits[CAP][THREADS];
     __shared__ float stab[kTableStride];
     const int blk = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
@@ -335,8 +390,12 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         const float ax = __shfl(fx, b), ay = __shfl(fy, b), az = __shfl(fz, b);
         const float bx = __shfl(fx, b + 1), by = __shfl(fy, b + 1), bz = __shfl(fz, b + 1);
         const float cx = __shfl(fx, b + 2), cy = __shfl(fy, b + 2), cz = __shfl(fz, b + 2);
-        if (valid && part == 0) A.frc[i] = make_float4((ax + bx) + cx, (ay + by) + cy, (az + bz) + cz, 0.f);
+        fx = (ax + bx) + cx;
+        fy = (ay + by) + cy;
+        fz = (az + bz) + cz;
+        if (valid && part == 0) A.frc[i] = make_float4(fx, fy, fz, 0.f);
     }
+    if (EPI) pair_epilogue(A, B, i, valid && part == 0, fx, fy, fz);
     if (OBS) {
         accumulate_boxes<10, THREADS / 64>(vir, valid ? box : -1, box, kObsVir, A.acc);
     }
@@ -359,11 +418,11 @@ constexpr int hit_waves_eu(bool obs, int threads, int cap, int split)
     return (obs || split > 1) ? 4 : hit_waves(threads, cap);
 }
 
-template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT>
+template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT, bool EPI>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT))))
 void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
 {
-    hitlist_body<OBS, MULTI, THREADS, CAP, FLUSH, SPLIT>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, MULTI, THREADS, CAP, FLUSH, SPLIT, EPI>(A, B0, nblocks, per_xcd);
 }
 
 // The schedules of the pair launch: hit-list kernels with their parameters, and the
@@ -389,26 +448,33 @@ constexpr int kDefaultSchedule = 0;
 constexpr int kFullShellSchedule = 4;
 
 // hit-list launch of schedule S on a grid of workgroups of its thread count
-template <int S, bool OBS, bool MULTI>
+template <int S, bool OBS, bool MULTI, bool EPI>
 inline void launch_hitlist_s(const PairArgs& a, const BoxDev& B0, int nblocks, hipStream_t s)
 {
     constexpr Schedule q = kSchedule[S];
     const int ph = (nblocks + 7) / 8;
-    hipLaunchKernelGGL((k_pair_hitlist<OBS, MULTI, q.threads, q.cap, q.flush, q.split>), dim3(8 * ph), dim3(q.threads),
-                       0, s, a, B0, nblocks, ph);
+    hipLaunchKernelGGL((k_pair_hitlist<OBS, MULTI, q.threads, q.cap, q.flush, q.split, EPI>), dim3(8 * ph),
+                       dim3(q.threads), 0, s, a, B0, nblocks, ph);
 }
 
-template <bool OBS, bool MULTI>
+template <bool OBS, bool MULTI, bool EPI>
 inline void launch_hitlist(int sched, const PairArgs& a, const BoxDev& B0, int nblocks, hipStream_t s)
 {
     switch (sched) {
-    case 1: launch_hitlist_s<1, OBS, MULTI>(a, B0, nblocks, s); break;
-    case 2: launch_hitlist_s<2, OBS, MULTI>(a, B0, nblocks, s); break;
-    case 3: launch_hitlist_s<3, OBS, MULTI>(a, B0, nblocks, s); break;
-    case 5: launch_hitlist_s<5, OBS, MULTI>(a, B0, nblocks, s); break;
-    case 6: launch_hitlist_s<6, OBS, MULTI>(a, B0, nblocks, s); break;
-    default: launch_hitlist_s<0, OBS, MULTI>(a, B0, nblocks, s); break;
+    case 1: launch_hitlist_s<1, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
+    case 2: launch_hitlist_s<2, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
+    case 3: launch_hitlist_s<3, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
+    case 5: launch_hitlist_s<5, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
+    case 6: launch_hitlist_s<6, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
+    default: launch_hitlist_s<0, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     }
+}
+
+// whether schedule `sched` can carry the step's epilogue (hit-list kernels only)
+inline bool epilogue_available(int sched, int64_t n)
+{
+    return sched >= 0 && sched < kSchedules && kSchedule[sched].kernel == DPD_PAIR_HITLIST &&
+           n <= (int64_t)kHitSlotMask;
 }
 
 // The automatic choice: the hit-list kernel for every engine whose slots fit its 26-bit list
@@ -422,7 +488,9 @@ inline bool pair_kernel_available(int which) { return which == DPD_PAIR_FULLSHEL
 
 // The pair launch of schedule `sched`.  A batch's hit-list workgroups take the slots of one
 // box each: hitblk (nhitblk entries) must be the table built for the schedule's thread count.
-inline hipError_t launch_pair_kernel(int sched, const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s)
+// `epi`: the launch also carries the step's epilogue (epilogue_available(sched) must hold).
+inline hipError_t launch_pair_kernel(int sched, const PairArgs& a, const BoxDev& B0, bool obs, hipStream_t s,
+                                     bool epi = false)
 {
     const bool multi = a.nbox > 1;
     if (sched < 0 || sched >= kSchedules) sched = kDefaultSchedule;
@@ -430,14 +498,18 @@ inline hipError_t launch_pair_kernel(int sched, const PairArgs& a, const BoxDev&
         const int nb = kSchedule[sched].beads();
         const int gh = multi ? a.nhitblk : (a.n + nb - 1) / nb;
         if (obs) {
-            if (multi) launch_hitlist<true, true>(sched, a, B0, gh, s);
-            else launch_hitlist<true, false>(sched, a, B0, gh, s);
+            if (multi) launch_hitlist<true, true, false>(sched, a, B0, gh, s);
+            else launch_hitlist<true, false, false>(sched, a, B0, gh, s);
+        } else if (epi) {
+            if (multi) launch_hitlist<false, true, true>(sched, a, B0, gh, s);
+            else launch_hitlist<false, false, true>(sched, a, B0, gh, s);
         } else {
-            if (multi) launch_hitlist<false, true>(sched, a, B0, gh, s);
-            else launch_hitlist<false, false>(sched, a, B0, gh, s);
+            if (multi) launch_hitlist<false, true, false>(sched, a, B0, gh, s);
+            else launch_hitlist<false, false, false>(sched, a, B0, gh, s);
         }
         return hipGetLastError();
     }
+    if (epi) return hipErrorInvalidValue;   // the full-shell kernel carries no epilogue
     // the full-shell kernel (also for hit lists beyond 2^26 beads)
     const int g = (a.n + 255) / 256;
     const int per_xcd = (g + 7) / 8;
