@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
 // workgroups come first, the bond and angle workgroups after them, so the bonded work fills
 // the CUs the pair kernel's last workgroups leave idle (the two write disjoint buffers).
 // (one box; held to 80 VGPRs like the hit-list kernel, six wavefronts per SIMD)
-template <bool OBS, int THREADS, int CAP, int FLUSH, int SPLIT, bool EPI>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT))))
+template <bool OBS, int THREADS, int CAP, int FLUSH, int SPLIT, int PASS, bool EPI>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT, PASS))))
 void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buffers S,
                            const int* __restrict__ slot_of, Topo T)
 {
@@ -195,7 +195,7 @@ void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buff
         bonded_body<OBS, false, THREADS / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
         return;
     }
-    hitlist_body<OBS, false, THREADS, CAP, FLUSH, SPLIT, EPI>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, false, THREADS, CAP, FLUSH, SPLIT, PASS, EPI>(A, B0, nblocks, per_xcd);
 }
 
 // the fused launch of hit-list schedule S with `nq` bonds and angles
@@ -207,8 +207,8 @@ void launch_hitlist_bonded_s(const PairArgs& a, const BoxDev& B0, int nq, const 
     const int gh = (a.n + q.beads() - 1) / q.beads();
     const int ph = (gh + 7) / 8;
     const dim3 gr(8 * ph + (nq + q.threads - 1) / q.threads), bl(q.threads);
-    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush, q.split, EPI>), gr, bl, 0, st, a, B0, gh,
-                       ph, Sb, slot_of, T);
+    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush, q.split, q.pass, EPI>), gr, bl, 0, st,
+                       a, B0, gh, ph, Sb, slot_of, T);
 }
 
 template <bool OBS, bool EPI>
@@ -221,6 +221,7 @@ void launch_hitlist_bonded(int sched, const PairArgs& a, const BoxDev& B0, int n
     case 3: launch_hitlist_bonded_s<3, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     case 5: launch_hitlist_bonded_s<5, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     case 6: launch_hitlist_bonded_s<6, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 7: launch_hitlist_bonded_s<7, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     default: launch_hitlist_bonded_s<0, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     }
 }
@@ -259,61 +260,6 @@ __global__ __launch_bounds__(256) void k_bonded_sum(Topo T, const int* __restric
     float4 f = frc[s];
     f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
     frc[s] = f;
-}
-
-// The step's end and the next step's start for the beads with bonded terms, after a pair
-// launch that carried the epilogue for the others: each bonded bead's bond and angle forces
-// added to its pair force in place, then the arithmetic of k_advance (MODE kEndBegin).  One
-// thread per bead that has bonds.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_advance_bonded(Topo T, const int* __restrict__ slot_of, Buffers S,
-                                                        float4* __restrict__ frc, const BoxDev* boxes, BoxDev B0,
-                                                        int* __restrict__ cnt, int* __restrict__ gcnt,
-                                                        int* __restrict__ nkey, int* __restrict__ rank,
-                                                        int* __restrict__ flags)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = t < T.nbonded;
-    int k = 0, s = 0;
-    if (active) {
-        const int q = T.bonded[t];
-        s = slot_of[q];
-        const float4 p = S.pos[s];
-        const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
-        float4 v = S.vel[s];
-        float4 f = frc[s];
-        double fb[3];
-        bonded_gather(q, T, fb);
-        f.x += (float)fb[0]; f.y += (float)fb[1]; f.z += (float)fb[2];
-        frc[s] = f;
-        const float4 fo = S.fold[s];
-        v.x = v.x + B.hdt * (f.x + fo.x);
-        v.y = v.y + B.hdt * (f.y + fo.y);
-        v.z = v.z + B.hdt * (f.z + fo.z);
-        S.vel[s] = v;
-        if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
-            atomicAdd(flags + 0, 1);
-        int c[3];
-        key_cell(S.key[s] - B.koff, B, c);
-        const float o[3] = {p.x, p.y, p.z};
-        const int img[3] = {0, 0, 0};
-        const float d[3] = {B.dt * v.x + B.hdt2 * f.x, B.dt * v.y + B.hdt2 * f.y, B.dt * v.z + B.hdt2 * f.z};
-        Moved m;
-        advance(o, c, img, d, B, m);
-        k = B.koff + cell_key(m.c[0], m.c[1], m.c[2], B);
-        nkey[s] = k;
-        if (m.jump) atomicAdd(flags + 1, 1);
-        if (m.bad) atomicAdd(flags + 0, 1);
-    }
-    const int lane = threadIdx.x & 63;
-    int r, size, leader;
-    wave_groups(k, active, r, size, leader);
-    int base = 0;
-    if (active && lane == leader) base = atomicAdd(cnt + k, size);
-    base = __shfl(base, leader, 64);
-    if (active) rank[s] = base + r;
-    wave_groups(k >> 6, active, r, size, leader);
-    if (active && lane == leader) atomicAdd(gcnt + (k >> 6), size);
 }
 
 // Close the previous step (kEndBegin) and start the next one: new cell key and rank.
@@ -725,7 +671,7 @@ struct dpd_engine {
     int* bead[2] = {};
     int4* img = nullptr;   // periodic image counts by bead index
     int* key[2] = {};
-    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 3 slots past the end
+    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 15 slots past the end
     float4 *frc = nullptr, *vp = nullptr;
     int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
     int2* place = nullptr;    // counting-sort placement: (bead index, source slot) per new slot
@@ -891,9 +837,8 @@ inline int grid(int64_t n, int b) { return (int)((n + b - 1) / b); }
 inline bool multi(dpd_engine* e) { return e->boxes.size() > 1; }
 
 // counting sort of buffer `cur` into buffer `cur^1`; MODE decides the move.  `advanced`: the
-// previous pair launch's epilogue (and k_advance_bonded) already closed the step and computed
-// the new keys and ranks (MODE kEndBegin only), so k_advance is left out and the scan advances
-// the simulation time.
+// previous pair launch's epilogue already closed the step and computed the new keys and ranks
+// (MODE kEndBegin only), so k_advance is left out and the scan advances the simulation time.
 template <int MODE, bool MULTI>
 void sort_step_t(dpd_engine* e, bool advanced)
 {
@@ -928,13 +873,16 @@ int sort_step(dpd_engine* e, bool advanced = false)
     return DPD_OK;
 }
 
-// whether the pair launch of a step that another follows can carry the step's epilogue
-inline bool epilogue_ok(const dpd_engine* e) { return epilogue_available(e->sched, e->n); }
+// Whether the pair launch of a step that another follows carries the step's epilogue: for
+// engines without bonded terms.  (With them, the bonded beads would need their own pass once
+// the bond and angle forces are known: measured on the condensate box, the epilogue added
+// 14 us to the pair launch and that pass took 20 us, against the 25 us of k_advance it
+// replaced.)
+inline bool epilogue_ok(const dpd_engine* e) { return e->nbonded == 0 && epilogue_available(e->sched, e->n); }
 
 // Pair forces of this step.  With the hit-list kernel the bond and angle terms run in the
 // same launch (k_pair_hitlist_bonded); `bonded` then tells launch_bonded to add only the sums.
-// `epi` (epilogue_ok): the launch also closes the step for the beads without bonded terms and
-// starts the next one (pair_epilogue); launch_advance_bonded must follow the bonded terms.
+// `epi` (epilogue_ok): the launch also closes the step and starts the next one (pair_epilogue).
 int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr, bool epi = false)
 {
     if (bonded) *bonded = false;
@@ -971,8 +919,8 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr, bool epi = fals
     if (bonded && nq > 0 && e->pair_kernel == DPD_PAIR_HITLIST && a.n <= (int)kHitSlotMask && !multi(e)) {
         const BoxDev B0 = e->boxes[0].B;
         const Buffers S = bufs(e, e->cur);
+        // (never with the epilogue: epilogue_ok excludes engines with bonded terms)
         if (obs) launch_hitlist_bonded<true, false>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
-        else if (epi) launch_hitlist_bonded<false, true>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
         else launch_hitlist_bonded<false, false>(e->sched, a, B0, nq, S, e->slot_of, topo(e), e->stream);
         *bonded = true;
         return hipGetLastError() == hipSuccess ? DPD_OK : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
@@ -980,21 +928,6 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr, bool epi = fals
     return launch_pair_kernel(e->sched, a, e->boxes[0].B, obs, e->stream, epi) == hipSuccess
                ? DPD_OK
                : fail(e, DPD_ERR_HIP, "pair kernel launch failed");
-}
-
-// after a pair launch with the epilogue and this step's bonded terms: the bonded beads' sums,
-// UpdateMom, UpdatePos, keys and ranks (k_advance_bonded)
-void launch_advance_bonded(dpd_engine* e)
-{
-    if (e->nbonded == 0) return;
-    const Buffers S = bufs(e, e->cur);
-    const BoxDev B0 = e->boxes[0].B;
-    if (multi(e))
-        hipLaunchKernelGGL((k_advance_bonded<true>), dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, topo(e),
-                           e->slot_of, S, e->frc, e->dboxes, B0, e->cnt, e->gcnt, e->nkey, e->rank, e->flags);
-    else
-        hipLaunchKernelGGL((k_advance_bonded<false>), dim3(grid(e->nbonded, 256)), dim3(256), 0, e->stream, topo(e),
-                           e->slot_of, S, e->frc, e->dboxes, B0, e->cnt, e->gcnt, e->nkey, e->rank, e->flags);
 }
 
 template <bool UPDATE_V, bool OBS>
@@ -1110,11 +1043,9 @@ int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, int timing)
         bool terms_done = false;
         if ((rc = launch_pair(e, obs && last, &terms_done, epi))) return rc;
         if (timed) HIPCHK(hipEventRecord(e->pev[2 * ti++ + 1], e->stream));
-        // the bonded sum of every step but the last is added by the next step's k_advance, or
-        // by k_advance_bonded right after the terms when the pair launch carried the epilogue
+        // the bonded sum of every step but the last is added by the next step's k_advance
         if (obs && last) launch_bonded<true>(e, terms_done);
         else launch_bonded<false>(e, terms_done, last);
-        if (epi) launch_advance_bonded(e);
         advanced = epi;
     }
     if (obs) launch_finish<true, true>(e);
