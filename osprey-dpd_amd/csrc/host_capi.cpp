@@ -56,113 +56,6 @@ double box_len(const InputData& d, int a)
     return a == 0 ? d.nx * d.wx : (a == 1 ? d.ny * d.wy : d.nz * d.wz);
 }
 
-// Maxwell velocities at kT: Gaussian components, zero total momentum, exact kT
-// (the reference rescales to the target temperature too, src/RandomBuilder.cpp:315-360)
-void maxwell(std::mt19937_64& rng, double kT, size_t n, double* v)
-{
-    std::normal_distribution<double> nd(0.0, std::sqrt(kT > 0 ? kT : 1.0));
-    double m[3] = {0, 0, 0};
-    for (size_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) { v[3 * i + d] = nd(rng); m[d] += v[3 * i + d]; }
-    double s2 = 0.0;
-    for (size_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) { v[3 * i + d] -= m[d] / (double)n; s2 += v[3 * i + d] * v[3 * i + d]; }
-    const double scale = (kT > 0 && s2 > 0) ? std::sqrt(3.0 * (double)n * kT / s2) : 0.0;
-    for (size_t i = 0; i < 3 * n; ++i) v[i] *= scale;
-}
-
-void wrap(const InputData& d, size_t n, const double* u, double* x)
-{
-    for (size_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) {
-            const double L = box_len(d, a);
-            double y = u[3 * i + a] - L * std::floor(u[3 * i + a] / L);
-            if (y >= L) y = 0.0;
-            x[3 * i + a] = y;
-        }
-}
-
-// lamella: the named polymer in two leaflets about the mid-plane normal to `Normal`
-// (src/Builder.cpp:842, isBilayer::Assemble), heads outwards, each bead placed by its
-// bond depth from the head; other beads uniform outside the slab.
-bool build_lamella(const dpd_input* in, std::mt19937_64& rng, double* u, std::string& err)
-{
-    const InputData& d = in->d;
-    const System& s = in->s;
-    std::vector<std::string> pol = d.state_values("Polymer", 1);
-    std::vector<std::string> nrm = d.state_values("Normal", 3);
-    std::vector<std::string> cen = d.state_values("Centre", 1);
-    std::vector<std::string> thk = d.state_values("Thickness", 1);
-    std::vector<std::string> upf = d.state_values("UpperFraction", 1);
-    if (pol.empty() || nrm.size() != 3 || cen.empty() || thk.empty()) {
-        err = "lamella state needs Polymer, Normal, Centre and Thickness";
-        return false;
-    }
-    int axis = -1;
-    for (int a = 0; a < 3; ++a)
-        if (std::atof(nrm[a].c_str()) != 0.0) axis = a;
-    int ptype = -1;
-    for (size_t p = 0; p < d.polymers.size(); ++p)
-        if (d.polymers[p].name == pol[0]) ptype = (int)p;
-    if (axis < 0 || ptype < 0) {
-        err = "lamella polymer or normal not found";
-        return false;
-    }
-    const double L = box_len(d, axis);
-    const double zc = std::atof(cen[0].c_str()) * L;
-    const double half = 0.5 * std::atof(thk[0].c_str());
-    const double upper = upf.empty() ? 0.5 : std::atof(upf[0].c_str());
-    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
-    std::uniform_real_distribution<double> U(0.0, 1.0);
-    // depth of each bead of the polymer template (bonds from bead 0)
-    const auto& tmpl = s.types[ptype];
-    if (tmpl.bead_type.empty()) {
-        err = "lamella polymer has no beads";
-        return false;
-    }
-    std::vector<int> depth(tmpl.bead_type.size(), 0);
-    for (size_t b = 0; b < tmpl.bond_head.size(); ++b) depth[tmpl.bond_tail[b]] = depth[tmpl.bond_head[b]] + 1;
-    const int maxd = std::max(1, *std::max_element(depth.begin(), depth.end()));
-    const int64_t nlip = s.polymers_per_type[ptype];
-    const int64_t nup = (int64_t)std::llround(upper * (double)nlip);
-    // lipids on a jittered square grid per leaflet
-    auto grid_side = [](int64_t m) { return std::max<int64_t>(1, (int64_t)std::ceil(std::sqrt((double)m))); };
-    int64_t lip = 0;
-    for (size_t i = 0; i < s.id.size();) {
-        const int pt = s.poly_type[i];
-        size_t j = i;
-        while (j < s.id.size() && s.poly_id[j] == s.poly_id[i]) ++j;
-        if (pt == ptype) {
-            const bool up = lip < nup;
-            const int64_t k = up ? lip : lip - nup;
-            const int64_t m = up ? nup : nlip - nup;
-            const int64_t side = grid_side(m);
-            const double p1 = ((double)(k % side) + 0.5 + 0.3 * (U(rng) - 0.5)) * box_len(d, a1) / side;
-            const double p2 = ((double)(k / side) + 0.5 + 0.3 * (U(rng) - 0.5)) * box_len(d, a2) / side;
-            for (size_t q = i; q < j; ++q) {
-                const int dep = depth[q - i];
-                const double off = half * (1.0 - (double)dep / (double)maxd) * 0.95;
-                u[3 * q + axis] = up ? zc + off + 0.02 : zc - off - 0.02;
-                u[3 * q + a1] = p1 + 0.25 * (U(rng) - 0.5);
-                u[3 * q + a2] = p2 + 0.25 * (U(rng) - 0.5);
-            }
-            ++lip;
-        } else {
-            for (size_t q = i; q < j; ++q) {
-                // outside the slab: draw the normal coordinate from the complement
-                const double out = L - 2.0 * half;
-                double z = U(rng) * std::max(out, 1e-3);
-                z = zc + half + z;
-                u[3 * q + axis] = z;
-                u[3 * q + a1] = U(rng) * box_len(d, a1);
-                u[3 * q + a2] = U(rng) * box_len(d, a2);
-            }
-        }
-        i = j;
-    }
-    return true;
-}
-
 // The reference's uniform stream for building states: its 64-bit LCG (CCNTCell::Randf,
 // src/CNTCell.cpp:2454-2465) started from |RNGSeed| (CCNTCell::SetRNGSeed, :134-137)
 struct RefUniform {
@@ -246,6 +139,270 @@ void build_reference_random(const dpd_input* in, uint64_t seed, double* x, doubl
     vtotal = std::sqrt(vtotal / (double)(3 * n));
     const double sk = std::sqrt(d.kT);
     for (int64_t i = 0; i < 3 * n; ++i) v[i] = sk * v[i] / vtotal;
+}
+
+// CLamellaBuilder::Assemble (src/LamellaBuilder.cpp:126-1003) without walls or polymerised
+// lipids, with the lamella set-up of CInitialStateLamella::ValidateData
+// (src/InitialStateLamella.cpp:672-769):
+// * the lipids (the named polymer type, in creation order) split into an upper monolayer of
+//   (long)(UpperFraction n) and a lower one of the rest, each on a hexagonal lattice of
+//   spacing 0.999 sqrt(1.1547 A / n_m) started 0.001 of a lattice vector from the origin;
+//   rows of the lattice advance when a site passes the in-plane length plus the row's shift,
+//   and a site past the length alone wraps back by it;
+// * per lipid the tail (last bead) at centre +- 0.2, the head (first bead) at
+//   centre +- Thickness/2, the other beads in order on the line between at spacing
+//   Thickness / (2 (size - 1)) (Linearise 1), or (Linearise 0) at lattice site + 0.5 e1 U,
+//   + 0.5 e2y U and centre +- Thickness/2 U along the normal, pulled back by half a lattice
+//   vector past the box;
+// * every other polymer (CBuilder::isRandom, src/Builder.cpp:293-490) in one of the two free
+//   slabs, chosen with probability proportional to its width, beads uniform in it along the
+//   normal and in 0.95 of the box across it, then the bond fix-up of the random builder;
+// * unwrapped = wrapped; 2-d Maxwell velocities in the plane (CBuilder::AssignMaxwellVelocities,
+//   src/Builder.cpp:4207-4383): per bead a speed from the Maxwell table and an angle, the
+//   centre-of-mass velocity removed and the in-plane kinetic energy scaled to kT per
+//   component.
+// The uniform stream is the reference's LCG from |RNGSeed|, as for the random builder.
+bool build_reference_lamella(const dpd_input* in, uint64_t seed, double* x, double* v, std::string& err)
+{
+    const InputData& d = in->d;
+    const System& s = in->s;
+    const int64_t n = (int64_t)s.id.size();
+    std::vector<std::string> pol = d.state_values("Polymer", 1);
+    std::vector<std::string> nrm = d.state_values("Normal", 3);
+    std::vector<std::string> cen = d.state_values("Centre", 1);
+    std::vector<std::string> thk = d.state_values("Thickness", 1);
+    std::vector<std::string> lin = d.state_values("Linearise", 1);
+    std::vector<std::string> upf = d.state_values("UpperFraction", 1);
+    std::vector<std::string> plm = d.state_values("Polymerise", 1);
+    if (pol.empty() || nrm.size() != 3 || cen.empty() || thk.empty() || lin.empty() || upf.empty()) {
+        err = "lamella state needs Polymer, Normal, Centre, Thickness, Linearise and UpperFraction";
+        return false;
+    }
+    if (!plm.empty() && std::atoi(plm[0].c_str()) != 0) {
+        err = "polymerised lamellae are not built";
+        return false;
+    }
+    const int X = std::atoi(nrm[0].c_str()), Y = std::atoi(nrm[1].c_str()), Z = std::atoi(nrm[2].c_str());
+    if (X + Y + Z != 1 || X < 0 || Y < 0 || Z < 0) {
+        err = "lamella normal must be a unit Cartesian axis";
+        return false;
+    }
+    int ptype = -1;
+    for (size_t p = 0; p < d.polymers.size(); ++p)
+        if (d.polymers[p].name == pol[0]) ptype = (int)p;
+    if (ptype < 0) {
+        err = "lamella polymer not found";
+        return false;
+    }
+    const double L[3] = {box_len(d, 0), box_len(d, 1), box_len(d, 2)};
+    const int na = X ? 0 : (Y ? 1 : 2);                // normal axis
+    const double l1 = X ? L[1] : L[0];                 // in-plane lengths
+    const double l2 = Z ? L[1] : L[2];
+    const double thickness = std::atof(thk[0].c_str());
+    const double centre = std::atof(cen[0].c_str()) * L[na];
+    if (centre + 0.5 * thickness > L[na] || centre - 0.5 * thickness < 0.0) {
+        err = "lamella outside the box";
+        return false;
+    }
+    const double area = l1 * l2;
+    double lowerHead = centre - 0.5 * thickness, upperHead = centre + 0.5 * thickness;
+    const bool linearise = std::atoi(lin[0].c_str()) != 0;
+    const double upperFraction = std::atof(upf[0].c_str());
+
+    // polymers in creation order: first bead and bead count
+    std::vector<int64_t> pstart, plen;
+    std::vector<int> ptyp;
+    for (int64_t i = 0; i < n;) {
+        int64_t j = i;
+        while (j < n && s.poly_id[j] == s.poly_id[i]) ++j;
+        pstart.push_back(i);
+        plen.push_back(j - i);
+        ptyp.push_back(s.poly_type[i]);
+        i = j;
+    }
+    std::vector<size_t> lipids;
+    for (size_t p = 0; p < pstart.size(); ++p)
+        if (ptyp[p] == ptype) lipids.push_back(p);
+    const int64_t total = (int64_t)lipids.size();
+    if (total == 0) {
+        err = "no lamella polymers";
+        return false;
+    }
+    const int64_t size = plen[lipids[0]];
+    if (size < 2) {
+        err = "lamella polymers need a head and a tail";
+        return false;
+    }
+
+    int64_t mtotal[2], first[2] = {0, 0}, lattice[2];
+    double sign[2] = {1.0, -1.0}, mhead[2];
+    mtotal[0] = (int64_t)(upperFraction * (double)total);
+    mtotal[1] = total - mtotal[0];
+    if (mtotal[0] == 0) {
+        lattice[0] = 0; lattice[1] = mtotal[1];
+        mhead[0] = 0.0; mhead[1] = lowerHead;
+        upperHead = centre;
+    } else if (mtotal[1] == 0) {
+        lattice[0] = mtotal[0]; lattice[1] = 0;
+        mhead[0] = upperHead; mhead[1] = 0.0;
+        lowerHead = centre;
+    } else {
+        first[1] = mtotal[0];
+        lattice[0] = mtotal[0]; lattice[1] = mtotal[1];
+        mhead[0] = upperHead; mhead[1] = lowerHead;
+    }
+    double hs[2], e1x[2], e2x[2], e2y[2];
+    for (int m = 0; m < 2; ++m) {
+        hs[m] = lattice[m] > 0 ? 0.999 * std::sqrt(1.1547 * area / (double)lattice[m]) : 0.0;
+        e1x[m] = hs[m];
+        e2x[m] = 0.5 * hs[m];
+        e2y[m] = 0.866025 * hs[m];
+    }
+    // head first and tail last in the bead order: every bead counts
+    const double caxis = 0.5 * thickness / (double)(size - 1);
+
+    RefUniform U{seed};
+    // in-plane (p0, p1) and normal coordinate to x, y, z
+    auto put = [&](int64_t b, double p0, double p1, double pn) {
+        double* q = x + 3 * b;
+        if (X) { q[0] = pn; q[1] = p0; q[2] = p1; }
+        else if (Y) { q[0] = p1; q[1] = pn; q[2] = p0; }
+        else { q[0] = p0; q[1] = p1; q[2] = pn; }
+    };
+    for (int m = 0; m < 2; ++m) {
+        double hexp[2] = {0.001 * e1x[m], 0.001 * e2y[m]};
+        double xp[2] = {hexp[0], hexp[1]};
+        int64_t rowCounter = 0;
+        for (int64_t iv = first[m]; iv < first[m] + lattice[m]; ++iv) {
+            const size_t p = lipids[(size_t)iv];
+            const int64_t b0 = pstart[p], b1 = b0 + plen[p];
+            const int64_t head = b0, tail = b1 - 1;
+            put(tail, xp[0], xp[1], centre + sign[m] * 0.2);
+            put(head, xp[0], xp[1], mhead[m]);
+            double beadCounter = 1.0;
+            for (int64_t b = b0; b < b1; ++b) {
+                if (b == head || b == tail) continue;
+                if (linearise) {
+                    put(b, xp[0], xp[1], mhead[m] - sign[m] * beadCounter * caxis);
+                    beadCounter += 1.0;
+                } else {
+                    // the draws in the reference's order of the three coordinates
+                    double q[3];
+                    if (X) {
+                        q[0] = centre + 0.5 * sign[m] * thickness * U.next();
+                        q[1] = xp[0] + 0.5 * e1x[m] * U.next();
+                        q[2] = xp[1] + 0.5 * e2y[m] * U.next();
+                        if (q[1] > L[1]) q[1] -= 0.5 * e1x[m];
+                        if (q[2] > L[2]) q[2] -= 0.5 * e2y[m];
+                    } else if (Y) {
+                        q[0] = xp[1] + 0.5 * e2y[m] * U.next();
+                        q[1] = centre + 0.5 * sign[m] * thickness * U.next();
+                        q[2] = xp[0] + 0.5 * e1x[m] * U.next();
+                        if (q[0] > L[0]) q[0] -= 0.5 * e2y[m];
+                        if (q[2] > L[2]) q[2] -= 0.5 * e1x[m];
+                    } else {
+                        q[0] = xp[0] + 0.5 * e1x[m] * U.next();
+                        q[1] = xp[1] + 0.5 * e2y[m] * U.next();
+                        q[2] = centre + 0.5 * sign[m] * thickness * U.next();
+                        if (q[0] > L[0]) q[0] -= 0.5 * e1x[m];
+                        if (q[1] > L[1]) q[1] -= 0.5 * e2y[m];
+                    }
+                    for (int a = 0; a < 3; ++a) x[3 * b + a] = q[a];
+                }
+            }
+            hexp[0] += e1x[m];
+            if (hexp[0] > l1 + (double)rowCounter * e2x[m]) {
+                ++rowCounter;
+                hexp[0] = (double)rowCounter * e2x[m];
+                hexp[1] += e2y[m];
+                xp[0] = hexp[0];
+                xp[1] = hexp[1];
+                if (hexp[1] >= l2) {
+                    err = "lamella lattice outside SimBox";
+                    return false;
+                }
+            } else if (hexp[0] > l1) {
+                xp[0] = hexp[0] - l1;
+            } else {
+                xp[0] = hexp[0];
+                xp[1] = hexp[1];
+            }
+        }
+    }
+
+    // the other polymers in the two free slabs (no walls: [0, lowerHead], [upperHead, L])
+    const double ex[6] = {0.0, 0.0, lowerHead, upperHead, L[na], L[na]};
+    const double freeLength = (ex[2] - ex[1]) + (ex[4] - ex[3]);
+    const double cum[2] = {(ex[2] - ex[1]) / freeLength, (ex[2] - ex[1]) / freeLength + (ex[4] - ex[3]) / freeLength};
+    for (size_t p = 0; p < pstart.size(); ++p) {
+        if (ptyp[p] == ptype) continue;
+        const double r = U.next();
+        int index = 0;
+        bool unassigned = true;
+        for (int k = 0; k < 2; ++k)
+            if (unassigned && r <= cum[k]) { unassigned = false; index = k; }
+        const double width = ex[2 * index + 2] - ex[2 * index + 1];
+        const double base = ex[2 * index + 1];
+        for (int64_t b = pstart[p]; b < pstart[p] + plen[p]; ++b) {
+            double q[3];
+            if (X) {
+                q[0] = base + width * U.next();
+                q[1] = 0.95 * L[1] * U.next();
+                q[2] = 0.95 * L[2] * U.next();
+            } else if (Y) {
+                q[0] = 0.95 * L[0] * U.next();
+                q[1] = base + width * U.next();
+                q[2] = 0.95 * L[2] * U.next();
+            } else {
+                q[0] = 0.95 * L[0] * U.next();
+                q[1] = 0.95 * L[1] * U.next();
+                q[2] = base + width * U.next();
+            }
+            for (int a = 0; a < 3; ++a) x[3 * b + a] = q[a];
+        }
+    }
+    // the bond fix-up of the free polymers (their bonds, in polymer order)
+    for (size_t b = 0; b < s.bond_head.size(); ++b) {
+        const int64_t h = s.bond_head[b], t = s.bond_tail[b];
+        if (s.poly_type[h] == ptype) continue;
+        double rd[3];
+        for (int a = 0; a < 3; ++a) rd[a] = x[3 * t + a] - x[3 * h + a];
+        const double rmag = std::sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
+        for (int a = 0; a < 3; ++a)
+            if (std::fabs(rd[a]) > 1.0) x[3 * t + a] = x[3 * h + a] + rd[a] / rmag;
+    }
+
+    // 2-d Maxwell velocities in the plane of the lamella
+    const std::vector<double> table = maxwell_table(n);
+    const double two_pi = 2.0 * 3.141592653589793238;
+    std::vector<double> vx((size_t)n), vy((size_t)n);
+    double vcm[2] = {0.0, 0.0};
+    for (int64_t i = 0; i < n; ++i) {
+        const size_t index = (size_t)((double)table.size() * U.next());
+        const double vmag = table[index];
+        const double vphi = two_pi * U.next();
+        vx[(size_t)i] = vmag * std::cos(vphi);
+        vy[(size_t)i] = vmag * std::sin(vphi);
+        vcm[0] += vx[(size_t)i];
+        vcm[1] += vy[(size_t)i];
+    }
+    vcm[0] /= (double)n;
+    vcm[1] /= (double)n;
+    double vtotal = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        vx[(size_t)i] -= vcm[0];
+        vy[(size_t)i] -= vcm[1];
+        vtotal += vx[(size_t)i] * vx[(size_t)i] + vy[(size_t)i] * vy[(size_t)i];
+    }
+    vtotal = std::sqrt(vtotal / (2.0 * (double)n * d.kT));
+    for (int64_t i = 0; i < n; ++i) {
+        const double a = vx[(size_t)i] / vtotal, c = vy[(size_t)i] / vtotal;
+        double* q = v + 3 * i;
+        if (X) { q[0] = 0.0; q[1] = c; q[2] = a; }
+        else if (Y) { q[0] = a; q[1] = 0.0; q[2] = c; }
+        else { q[0] = a; q[1] = c; q[2] = 0.0; }
+    }
+    return true;
 }
 
 std::string fmt6(double v)
@@ -442,13 +599,9 @@ int dpd_input_build_state(const dpd_input* in, uint64_t seed, double* x, double*
         if (unpbc) std::copy(x, x + 3 * n, unpbc);
         return DPD_OK;
     }
-    std::mt19937_64 rng(seed);
-    std::vector<double> u(3 * n);
     std::string err;
-    if (!build_lamella(in, rng, u.data(), err)) return DPD_ERR_UNSUPPORTED;
-    wrap(in->d, n, u.data(), x);
-    if (unpbc) std::copy(u.begin(), u.end(), unpbc);
-    maxwell(rng, in->d.kT, n, v);
+    if (!build_reference_lamella(in, seed, x, v, err)) return DPD_ERR_UNSUPPORTED;
+    if (unpbc) std::copy(x, x + 3 * n, unpbc);
     return DPD_OK;
 }
 

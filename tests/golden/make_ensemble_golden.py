@@ -11,9 +11,10 @@ Cases:
             dt = 0.02, lambda = 0.5, 5000 steps from a random start: kT and P every 10
             steps (dmpchs), snapshots every 250 steps for g(r).
   bilayer16 H3(T4)2 lipids (Bond k 128 l0 0.5, BondPair k 20 phi0 0) in water, 16^3
-            cells, a_HT 35, a_TW 75, a_HW 35, others 25, started as a lamella
-            (State lamella), 4000 steps: kT, P, density profiles along the normal and
-            the mesh area per lipid from snapshots every 250 steps.
+            cells, a_HT 35, a_TW 75, a_HW 35, others 25, started as a lamella by the
+            reference's own builder (State lamella; the engine's builder reproduces the
+            same states), 4000 steps: kT, P, density profiles along the normal and the
+            mesh area per lipid from snapshots every 250 steps.
   binary20  water/oil 50/50 at random (the binary demixing configuration on a 20^3 box,
             24,000 beads), a_WW = a_OO = 25, a_WO = 80, 4000 steps from a random start:
             kT and P, and the W-W and W-O g(r) of snapshots every 250 steps after step 2000
@@ -25,12 +26,9 @@ Cases:
             restart, as for the bilayer): kT, P, the sticker-sticker g(r) and the mean bond
             length from snapshots every 250 steps after step 2000.
 Each case runs NRUNS times from independent starting states (RNGSeed differs per run).
-The bilayer runs start from this build's lamella states (dpd_amd.host.Input.build_state
-with the run's seed), handed to the reference as restart snapshots (State restart), so
-that the engine and the reference are compared from identical structures; NRUNS more
-bilayer runs use the reference's own lamella builder, and their kT and P are kept as
-"reference_builder_runs" to record how much the starting structure alone moves the
-4000-step averages.
+The condensate runs start from this build's random states handed to the reference as
+restart snapshots (State restart); the water, binary and bilayer runs from the reference's
+own builders, whose states this build reproduces from the same seeds.
 Per run, kT and P are block-averaged over the second part of the run (water: after step
 1000, bilayer: after step 2000); the fixtures keep every run's mean so that comparisons
 use the run-to-run spread, which for a bilayer relaxing over a few thousand steps is
@@ -369,12 +367,9 @@ def own_start(work, run_id, text, seed):
         fh.write(text)
 
 
-def bilayer_run(work, r, own=True):
-    rid = f"b{r}" if own else f"rb{r}"
+def bilayer_run(work, r):
+    rid = f"b{r}"
     text = BILAYER.format(seed=BILAYER_SEEDS[r])
-    if own:
-        own_start(work, f"own{r}", text, BILAYER_SEEDS[r])
-        text = restart_text(text, f"own{r}")
     run(work, rid, text)
     h = history(os.path.join(work, "dmpchs." + rid))
     h = h[h[:, 0] > 2000]
@@ -401,17 +396,15 @@ def water(runs):
             "steps": 5000, "discard": 1000}, WATER.format(seed=WATER_SEEDS[0])
 
 
-def bilayer(runs, refb):
+def bilayer(runs):
     p, pse = pooled([p for r in runs for p in r["prof"]])
     apl = [a for r in runs for a in r["apl"]]
     return {"runs": [{"kT": r["kT"], "P": r["P"], "apl": float(np.mean(r["apl"]))} for r in runs],
-            "seeds": BILAYER_SEEDS, "start": "dpd_amd lamella builder, seed per run", "z": runs[0]["z"].tolist(),
-            "profile": p, "profile_se": pse,
+            "seeds": BILAYER_SEEDS, "start": "the reference's lamella builder (State lamella), seed per run",
+            "z": runs[0]["z"].tolist(), "profile": p, "profile_se": pse,
             "apl": [float(np.mean(apl)), float(np.std(apl, ddof=1) / np.sqrt(len(apl)))],
             "nlipids": runs[0]["nlip"], "nsnap": len(apl), "types": ["H", "T", "W"], "steps": 4000,
-            "discard": 2000,
-            "reference_builder_runs": [{"kT": r["kT"], "P": r["P"], "apl": float(np.mean(r["apl"]))} for r in refb]}, \
-        BILAYER.format(seed=BILAYER_SEEDS[0])
+            "discard": 2000}, BILAYER.format(seed=BILAYER_SEEDS[0])
 
 
 def main():
@@ -426,8 +419,7 @@ def main():
             if "water20" in wanted:
                 jobs["water20"] = ([pool.submit(water_run, work, r) for r in range(NRUNS)],)
             if "bilayer16" in wanted:
-                jobs["bilayer16"] = ([pool.submit(bilayer_run, work, r) for r in range(NRUNS)],
-                                     [pool.submit(bilayer_run, work, r, False) for r in range(NRUNS)])
+                jobs["bilayer16"] = ([pool.submit(bilayer_run, work, r) for r in range(NRUNS)],)
             if "binary20" in wanted:
                 jobs["binary20"] = ([pool.submit(binary_run, work, r) for r in range(NRUNS)],)
             if "condensate20" in wanted:

@@ -359,6 +359,69 @@ def make_io(work):
         fh.write(dmpci("lipidphi8", "restart RunId lipidphi8 StateId 0", 10, cmds))
 
 
+LAMELLA_BODY = """Bead  H
+      0.5
+      25
+      4.5
+
+Bead  T
+      0.5
+      35  25
+      4.5 4.5
+
+Bead  W
+      0.5
+      35  75  25
+      4.5 4.5 4.5
+
+Bond    H H  128  0.5
+Bond    H T  128  0.5
+Bond    T T  128  0.5
+
+BondPair  H T T  20.0  0.0
+BondPair  T T T  20.0  0.0
+
+Polymer	Water    {water}   " (W) "
+Polymer	Lipid    {lipid}   " (H H (* (T T T T)) H T T T T) "
+
+"""
+
+# tag -> (box, normal, centre, thickness, linearise, upper fraction, lipid number fraction, seed)
+LAMELLA = {
+    "z": ((10, 10, 10), (0, 0, 1), 0.5, 5.0, 1, 0.5, 0.05, -2468),
+    "x": ((8, 10, 10), (1, 0, 0), 0.45, 4.0, 0, 0.4, 0.05, -1357),
+    "y": ((10, 8, 10), (0, 1, 0), 0.55, 5.0, 1, 0.6, 0.05, -9753),
+}
+
+
+def lamella_dmpci(tag, time, extra=""):
+    box, normal, centre, thk, lin, upper, frac, seed = LAMELLA[tag]
+    state = ("lamella\n        Polymer\t\t\tLipid\n        Normal\t\t\t{} {} {}\n        Centre\t\t\t{}\n"
+             "        Thickness\t\t{}\n        Linearise\t\t{}\n        UpperFraction\t{}\n"
+             "        Polymerise\t\t0").format(*normal, centre, thk, lin, upper)
+    txt = HEADER.format(title="lamella " + tag) + "State\t" + state + "\n\n"
+    txt += LAMELLA_BODY.format(water=f"{1 - frac:.5f}", lipid=f"{frac:.5f}")
+    txt += TAIL.format(nx=box[0], ny=box[1], nz=box[2], seed=seed, dt=0.02, time=time)
+    return txt + extra
+
+
+def make_lamella(work):
+    """The reference's lamella starting states (CLamellaBuilder::Assemble,
+    src/LamellaBuilder.cpp:126-1003: the hexagonal lattice of lipids in two monolayers,
+    isRandom for the solvent, 2-d Maxwell velocities): one step with the forces toggled off
+    moves every bead by dt v and leaves v alone, so the time-1 snapshot carries the built
+    state.  Normals along z, x and y; straight and randomly placed lipid beads; symmetric
+    and asymmetric monolayers."""
+    out = os.path.join(HERE, "io")
+    os.makedirs(out, exist_ok=True)
+    for tag in LAMELLA:
+        text = lamella_dmpci(tag, 1)
+        with open(os.path.join(out, f"lamella_{tag}.dmpci"), "w") as fh:
+            fh.write(text)
+        run(work, "lam" + tag, text + "Command ToggleDPDBeadForces 1\n")
+        write_gz(os.path.join(out, f"lamella_{tag}.dat.gz"), bead_lines(os.path.join(work, f"dmpcrs.lam{tag}.con.1.dat")))
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
@@ -368,6 +431,10 @@ def main():
         if args[:1] == ["io"]:
             make_io(work)
             print("wrote io fixtures")
+            return
+        if args[:1] == ["lamella"]:
+            make_lamella(work)
+            print("wrote lamella fixtures")
             return
         if args[:1] == ["extras"]:
             # only the energy and command fixtures (the snapshots stay as committed)

@@ -1,9 +1,10 @@
 """Statistical parity of thermostatted runs: the GPU engine (counter-based pair random
 stream) against several independent runs of the reference (serial LCG stream) on the
 same control files.  The GPU side runs 8 replicas from independent starting states in
-one batched engine.  Bilayer runs on both sides start from this build's lamella states
-(the reference reads them as restart snapshots), since the starting structure alone
-moves 4000-step bilayer averages by ~0.2% in pressure.  Criteria: kT within 0.5% and
+one batched engine.  The bilayer runs of the reference start from its own lamella builder
+and the GPU replicas from this build's restatement of it, which gives the same states for
+the same seeds (the starting structure alone moves 4000-step bilayer averages by ~0.2% in
+pressure, so the starts must agree).  Criteria: kT within 0.5% and
 pressure within 1% of the reference ensemble mean, and both consistent with it within 4
 standard errors of the run-to-run spread; g(r) and bilayer density profiles within their
 combined error bars."""
@@ -75,8 +76,9 @@ def test_water_ensemble():
 
 def test_bilayer_ensemble():
     ref = json.load(open(os.path.join(ENS, "bilayer16.json")))
-    # the reference runs started from this build's lamella states with these seeds: the
-    # first replicas start from the same states, the rest from further seeds
+    # the reference runs started from its lamella builder with these seeds, which this
+    # build's builder reproduces: the first replicas start from the same states, the rest
+    # from further seeds
     seeds = list(ref["seeds"]) + [9090 + 104729 * r for r in range(REPLICAS - len(ref["seeds"]))]
     runs, snaps, types, L = _run("bilayer16", ref["discard"], 2250, 250, seeds)
     _close("kT", [A.block_average(r[:, 0])[0] for r in runs], [r["kT"][0] for r in ref["runs"]], 0.005)

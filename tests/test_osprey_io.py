@@ -85,3 +85,32 @@ def test_random_state_is_the_references(case):
     # kT exactly, no drift, as the reference normalises
     assert abs((v ** 2).sum() / (3 * len(v)) - inp.counts["kT"]) < 1e-12
     assert np.abs(v.sum(axis=0)).max() < 1e-9
+
+
+@pytest.mark.parametrize("tag", ["z", "x", "y"])
+def test_lamella_state_is_the_references(tag):
+    """State lamella builds the reference's own starting state (CLamellaBuilder::Assemble,
+    src/LamellaBuilder.cpp:126-1003): lipids on the hexagonal lattices of the two monolayers,
+    heads and tails on the lamella's planes and the beads between on the line (Linearise 1) or
+    drawn about it (Linearise 0), the solvent in the free slabs (CBuilder::isRandom), 2-d
+    Maxwell velocities in the plane; normals along z, x and y, monolayer fractions 0.5, 0.4,
+    0.6.  The reference's time-1 snapshot with the forces toggled off fixes x and v to its 6
+    digits (as for the random builder)."""
+    text = open(os.path.join(IO, f"lamella_{tag}.dmpci")).read()
+    inp = H.Input(text=text)
+    x, u, v = inp.build_state(seed=abs(inp.counts["seed"]))
+    ref = np.loadtxt(os.path.join(IO, f"lamella_{tag}.dat.gz"))
+    L = np.array(inp.cells, float)
+    assert len(ref) == len(x)
+    rv, ru = ref[:, 10:13], ref[:, 7:10]
+    dt = inp.counts["dt"]
+    tol = lambda r: 5.01e-6 * np.abs(r) + 1e-12
+    assert np.all(np.abs(v - rv) <= tol(rv))
+    assert np.all(np.abs(u + dt * v - ru) <= tol(ru) + 1e-9)
+    assert np.array_equal(x, u)
+    # the velocities lie in the plane, with kT per in-plane component and no drift
+    normal = {"z": 2, "x": 0, "y": 1}[tag]
+    assert np.all(v[:, normal] == 0.0)
+    assert abs((v ** 2).sum() / (2 * len(v)) - inp.counts["kT"]) < 1e-12
+    assert np.abs(v.sum(axis=0)).max() < 1e-9
+    assert (x >= 0).all() and (x < L).all()
