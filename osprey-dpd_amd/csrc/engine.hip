@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void k_bonded(Buffers S, const int* __restrict
 // workgroups come first, the bond and angle workgroups after them, so the bonded work fills
 // the CUs the pair kernel's last workgroups leave idle (the two write disjoint buffers).
 // (one box; held to 80 VGPRs like the hit-list kernel, six wavefronts per SIMD)
-template <bool OBS, int THREADS, int CAP, int FLUSH, int SPLIT, int PASS, bool EPI>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT, PASS))))
+template <bool OBS, int THREADS, int CAP, int FLUSH, int SPLIT, bool EPI>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT))))
 void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buffers S,
                            const int* __restrict__ slot_of, Topo T)
 {
@@ -195,7 +195,7 @@ void k_pair_hitlist_bonded(PairArgs A, BoxDev B0, int nblocks, int per_xcd, Buff
         bonded_body<OBS, false, THREADS / 64>(S, slot_of, T, A.boxes, B0, A.acc, q0 + (int)threadIdx.x, q0);
         return;
     }
-    hitlist_body<OBS, false, THREADS, CAP, FLUSH, SPLIT, PASS, EPI>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, false, THREADS, CAP, FLUSH, SPLIT, EPI>(A, B0, nblocks, per_xcd);
 }
 
 // the fused launch of hit-list schedule S with `nq` bonds and angles
@@ -207,7 +207,7 @@ void launch_hitlist_bonded_s(const PairArgs& a, const BoxDev& B0, int nq, const 
     const int gh = (a.n + q.beads() - 1) / q.beads();
     const int ph = (gh + 7) / 8;
     const dim3 gr(8 * ph + (nq + q.threads - 1) / q.threads), bl(q.threads);
-    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush, q.split, q.pass, EPI>), gr, bl, 0, st,
+    hipLaunchKernelGGL((k_pair_hitlist_bonded<OBS, q.threads, q.cap, q.flush, q.split, EPI>), gr, bl, 0, st,
                        a, B0, gh, ph, Sb, slot_of, T);
 }
 
@@ -220,8 +220,6 @@ void launch_hitlist_bonded(int sched, const PairArgs& a, const BoxDev& B0, int n
     case 2: launch_hitlist_bonded_s<2, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     case 3: launch_hitlist_bonded_s<3, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     case 5: launch_hitlist_bonded_s<5, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
-    case 6: launch_hitlist_bonded_s<6, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
-    case 7: launch_hitlist_bonded_s<7, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     default: launch_hitlist_bonded_s<0, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     }
 }
@@ -671,7 +669,7 @@ struct dpd_engine {
     int* bead[2] = {};
     int4* img = nullptr;   // periodic image counts by bead index
     int* key[2] = {};
-    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 15 slots past the end
+    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 3 slots past the end
     float4 *frc = nullptr, *vp = nullptr;
     int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
     int2* place = nullptr;    // counting-sort placement: (bead index, source slot) per new slot

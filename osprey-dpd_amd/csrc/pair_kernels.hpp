@@ -145,9 +145,9 @@ constexpr int kHitThreads = 512;   // the default schedule's
 constexpr int kHitCap = 24;
 constexpr int kHitSlotBits = 26;
 constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
-// slots of padding after the last scan code: the scan reads candidates PASS at a time, one
-// pass ahead (2 PASS - 1 past the end at most, PASS <= 8)
-constexpr int kCodePad = 16;
+// slots of padding after the last scan code: the scan reads candidates two at a time,
+// one pass ahead (three past the end at most)
+constexpr int kCodePad = 8;
 // 1e-18 < dr2 < 1 as one unsigned compare of the bits: dr2 >= 0, NaN fails
 constexpr uint32_t kPairLo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
 __device__ inline bool pair_in_range(float dr2) { return __float_as_uint(dr2) - kPairLo < 0x3F800000u - kPairLo; }
@@ -208,12 +208,11 @@ __device__ inline void pair_epilogue(const PairArgs& A, const BoxDev& B, int i, 
 // several boxes each workgroup takes slots of one box only (A.hitblk[blk] = box, first
 // slot, end slot), so the box, its constants and its table are uniform: scalar loads and
 // the table in LDS, as for one box.
-template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT, int PASS, bool EPI>
+template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT, bool EPI>
 __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArgs& A, const BoxDev& B0, int nblocks,
                                                                    int per_xcd)
 {
     static_assert(SPLIT == 1 || SPLIT == 3, "one or three lanes per bead");
-    static_assert((PASS == 2 || PASS == 8) && CAP >= PASS, "candidates per pass");
     static_assert(!(OBS && EPI), "the observing launch ends a sequence of steps");
     __shared__ uint32_t hits[CAP][THREADS];
     __shared__ float stab[kTableStride];
@@ -365,38 +364,16 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                     // the padding after the last slot); none of those is listed.  Slots are
                     // below 2^26, so byte offsets fit 32 bits and the loads address from the
                     // array base.
-                    if (PASS == 2) {
-                        code2_t c2 = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j0 << 2));
-                        for (int j = j0; j < j1; j += 2) {
-                            const code2_t cn = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j << 2) + 8);
-                            const uint32_t jc = (uint32_t)j | code;
-                            // room for the two: a list past CAP - 2 entries is evaluated
-                            // first (overflow inside a segment: this lane alone)
-                            if (wp > laddr + 4 * (CAP - 2) * THREADS) drain();
-                            test(jc, c2.x, fxbits, qx, qy, qz, true);
-                            test(jc + 1, c2.y, fxbits, qx, qy, qz, j + 1 < j1);
-                            c2 = cn;
-                        }
-                    } else {
-                        // eight candidates per pass from four 8-byte loads, the next eight in
-                        // flight meanwhile: fewer dependent load rounds per row, for boxes too
-                        // small to hide a round's latency behind other wavefronts
-                        auto ld = [&](int j) { return *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j << 2)); };
-                        code2_t a0 = ld(j0), a1 = ld(j0 + 2), a2 = ld(j0 + 4), a3 = ld(j0 + 6);
-                        for (int j = j0; j < j1; j += 8) {
-                            const code2_t b0 = ld(j + 8), b1 = ld(j + 10), b2 = ld(j + 12), b3 = ld(j + 14);
-                            const uint32_t jc = (uint32_t)j | code;
-                            if (wp > laddr + 4 * (CAP - 8) * THREADS) drain();
-                            test(jc, a0.x, fxbits, qx, qy, qz, true);
-                            test(jc + 1, a0.y, fxbits, qx, qy, qz, j + 1 < j1);
-                            test(jc + 2, a1.x, fxbits, qx, qy, qz, j + 2 < j1);
-                            test(jc + 3, a1.y, fxbits, qx, qy, qz, j + 3 < j1);
-                            test(jc + 4, a2.x, fxbits, qx, qy, qz, j + 4 < j1);
-                            test(jc + 5, a2.y, fxbits, qx, qy, qz, j + 5 < j1);
-                            test(jc + 6, a3.x, fxbits, qx, qy, qz, j + 6 < j1);
-                            test(jc + 7, a3.y, fxbits, qx, qy, qz, j + 7 < j1);
-                            a0 = b0; a1 = b1; a2 = b2; a3 = b3;
-                        }
+                    code2_t c2 = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j0 << 2));
+                    for (int j = j0; j < j1; j += 2) {
+                        const code2_t cn = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j << 2) + 8);
+                        const uint32_t jc = (uint32_t)j | code;
+                        // room for the two: a list past CAP - 2 entries is evaluated
+                        // first (overflow inside a segment: this lane alone)
+                        if (wp > laddr + 4 * (CAP - 2) * THREADS) drain();
+                        test(jc, c2.x, fxbits, qx, qy, qz, true);
+                        test(jc + 1, c2.y, fxbits, qx, qy, qz, j + 1 < j1);
+                        c2 = cn;
                     }
                     if (FLUSH < CAP && __ballot(wp >= laddr + 4 * FLUSH * THREADS) != 0) drain();
                 }
@@ -435,18 +412,18 @@ constexpr int hit_waves(int threads, int cap)
     const int w = (160 * 1024 / lds) * (threads / 64) / 4;
     return w < 6 ? w : 6;
 }
-// the occupancy a kernel is held to: the split and wide-pass schedules run small boxes, which
-// never fill the SIMDs, and get the registers of four wavefronts (no spills)
-constexpr int hit_waves_eu(bool obs, int threads, int cap, int split, int pass)
+// the occupancy a kernel is held to: the split schedules run small boxes, which never fill
+// the SIMDs, and get the registers of four wavefronts (no spills)
+constexpr int hit_waves_eu(bool obs, int threads, int cap, int split)
 {
-    return (obs || split > 1 || pass > 2) ? 4 : hit_waves(threads, cap);
+    return (obs || split > 1) ? 4 : hit_waves(threads, cap);
 }
 
-template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT, int PASS, bool EPI>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT, PASS))))
+template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT, bool EPI>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(hit_waves_eu(OBS, THREADS, CAP, SPLIT))))
 void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
 {
-    hitlist_body<OBS, MULTI, THREADS, CAP, FLUSH, SPLIT, PASS, EPI>(A, B0, nblocks, per_xcd);
+    hitlist_body<OBS, MULTI, THREADS, CAP, FLUSH, SPLIT, EPI>(A, B0, nblocks, per_xcd);
 }
 
 // The schedules of the pair launch: hit-list kernels with their parameters, and the
@@ -455,20 +432,17 @@ struct Schedule {
     int kernel;    // DPD_PAIR_HITLIST or DPD_PAIR_FULLSHELL
     int threads, cap, flush;
     int split;     // lanes per bead (hit list): 1, or 3 (one z layer of neighbour rows each)
-    int pass;      // scan codes tested per pass of the scan: 2, or 8 (more loads in flight)
     const char* name;
     constexpr int beads() const { return hit_beads(threads, split); }   // per workgroup
 };
-constexpr int kSchedules = 8;
+constexpr int kSchedules = 6;
 constexpr Schedule kSchedule[kSchedules] = {
-    {DPD_PAIR_HITLIST, 512, 24, 24, 1, 2, "hitlist-t512-c24"},
-    {DPD_PAIR_HITLIST, 512, 24, 16, 1, 2, "hitlist-t512-c24-f16"},
-    {DPD_PAIR_HITLIST, 256, 24, 24, 1, 2, "hitlist-t256-c24"},
-    {DPD_PAIR_HITLIST, 256, 32, 32, 1, 2, "hitlist-t256-c32"},
-    {DPD_PAIR_FULLSHELL, 256, 0, 0, 1, 2, "fullshell-t256"},
-    {DPD_PAIR_HITLIST, 256, 12, 12, 3, 2, "hitlist-split3-t256-c12"},
-    {DPD_PAIR_HITLIST, 256, 16, 16, 3, 8, "hitlist-split3-t256-c16-p8"},
-    {DPD_PAIR_HITLIST, 256, 32, 32, 1, 8, "hitlist-t256-c32-p8"},
+    {DPD_PAIR_HITLIST, 512, 24, 24, 1, "hitlist-t512-c24"},
+    {DPD_PAIR_HITLIST, 512, 24, 16, 1, "hitlist-t512-c24-f16"},
+    {DPD_PAIR_HITLIST, 256, 24, 24, 1, "hitlist-t256-c24"},
+    {DPD_PAIR_HITLIST, 256, 32, 32, 1, "hitlist-t256-c32"},
+    {DPD_PAIR_FULLSHELL, 256, 0, 0, 1, "fullshell-t256"},
+    {DPD_PAIR_HITLIST, 256, 12, 12, 3, "hitlist-split3-t256-c12"},
 };
 constexpr int kDefaultSchedule = 0;
 constexpr int kFullShellSchedule = 4;
@@ -479,7 +453,7 @@ inline void launch_hitlist_s(const PairArgs& a, const BoxDev& B0, int nblocks, h
 {
     constexpr Schedule q = kSchedule[S];
     const int ph = (nblocks + 7) / 8;
-    hipLaunchKernelGGL((k_pair_hitlist<OBS, MULTI, q.threads, q.cap, q.flush, q.split, q.pass, EPI>), dim3(8 * ph),
+    hipLaunchKernelGGL((k_pair_hitlist<OBS, MULTI, q.threads, q.cap, q.flush, q.split, EPI>), dim3(8 * ph),
                        dim3(q.threads), 0, s, a, B0, nblocks, ph);
 }
 
@@ -491,8 +465,6 @@ inline void launch_hitlist(int sched, const PairArgs& a, const BoxDev& B0, int n
     case 2: launch_hitlist_s<2, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     case 3: launch_hitlist_s<3, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     case 5: launch_hitlist_s<5, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
-    case 6: launch_hitlist_s<6, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
-    case 7: launch_hitlist_s<7, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     default: launch_hitlist_s<0, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     }
 }
