@@ -9,7 +9,7 @@
 #include <fstream>
 #include <iomanip>
 #include <memory>
-#include <random>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -68,20 +68,27 @@ struct RefUniform {
 };
 
 // CBuilder::MaxwellDist (src/Builder.cpp:104-129): a table of max(n, 10^7) speeds, each
-// from the factor the previous one left; the last entry repeats the one before it
-std::vector<double> maxwell_table(int64_t n)
+// from the factor the previous one left; the last entry repeats the one before it.  The
+// table depends on its size alone, so the last one built is kept (10^7 entries take ~0.5 s,
+// and every state of up to 10^7 beads uses the same table).
+std::shared_ptr<const std::vector<double>> maxwell_table(int64_t n)
 {
+    static std::mutex mu;
+    static std::shared_ptr<const std::vector<double>> cache;
     const int64_t m = std::max<int64_t>(n, 10000000);
-    std::vector<double> t((size_t)m, 0.0);
+    std::lock_guard<std::mutex> lock(mu);
+    if (cache && (int64_t)cache->size() == m) return cache;
+    auto t = std::make_shared<std::vector<double>>((size_t)m, 0.0);
     const double dm = (double)m;
     double factor = 1.0;
     for (int64_t j = 0; j < m - 1; ++j) {
         const double vel = std::sqrt(2.0 * std::log(dm * factor / (dm - factor)));
         factor = std::exp(0.5 * vel * vel);
-        t[(size_t)j] = vel;
+        (*t)[(size_t)j] = vel;
     }
-    t[(size_t)(m - 1)] = t[(size_t)(m - 2)];
-    return t;
+    (*t)[(size_t)(m - 1)] = (*t)[(size_t)(m - 2)];
+    cache = t;
+    return cache;
 }
 
 // CRandomBuilder::Assemble (src/RandomBuilder.cpp:71-425) without walls: per bead, in
@@ -97,7 +104,8 @@ void build_reference_random(const dpd_input* in, uint64_t seed, double* x, doubl
     const System& s = in->s;
     const int64_t n = (int64_t)s.id.size();
     RefUniform U{seed};
-    const std::vector<double> table = maxwell_table(n);
+    const auto tab = maxwell_table(n);
+    const std::vector<double>& table = *tab;
     const double L[3] = {box_len(d, 0), box_len(d, 1), box_len(d, 2)};
     double vcm[3] = {0.0, 0.0, 0.0};
     const double two_pi = 2.0 * 3.141592653589793238;
@@ -373,7 +381,8 @@ bool build_reference_lamella(const dpd_input* in, uint64_t seed, double* x, doub
     }
 
     // 2-d Maxwell velocities in the plane of the lamella
-    const std::vector<double> table = maxwell_table(n);
+    const auto tab = maxwell_table(n);
+    const std::vector<double>& table = *tab;
     const double two_pi = 2.0 * 3.141592653589793238;
     std::vector<double> vx((size_t)n), vy((size_t)n);
     double vcm[2] = {0.0, 0.0};
