@@ -362,9 +362,14 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the water, binary and bilayer boxes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run one 256 MiB device-to-device copy (calibrates the PMC byte counters)")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="only the water, binary and bilayer boxes (other_configs), one JSON line")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.configs_only:
+        print(json.dumps({"other_configs": bench_configs()}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (multi-GPU "
