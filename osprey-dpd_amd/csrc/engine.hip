@@ -220,6 +220,7 @@ void launch_hitlist_bonded(int sched, const PairArgs& a, const BoxDev& B0, int n
     case 2: launch_hitlist_bonded_s<2, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     case 3: launch_hitlist_bonded_s<3, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     case 5: launch_hitlist_bonded_s<5, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
+    case 6: launch_hitlist_bonded_s<6, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     default: launch_hitlist_bonded_s<0, OBS, EPI>(a, B0, nq, Sb, slot_of, T, st); break;
     }
 }
@@ -669,7 +670,7 @@ struct dpd_engine {
     int* bead[2] = {};
     int4* img = nullptr;   // periodic image counts by bead index
     int* key[2] = {};
-    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 3 slots past the end
+    uint32_t* code[2] = {};   // n + kCodePad: the hit-list scan reads up to 7 slots past the end
     float4 *frc = nullptr, *vp = nullptr;
     int *slot_of = nullptr, *nkey = nullptr, *rank = nullptr;
     int2* place = nullptr;    // counting-sort placement: (bead index, source slot) per new slot

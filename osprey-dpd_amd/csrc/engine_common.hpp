@@ -141,8 +141,8 @@ __device__ inline float pair_scalar(float dr2, float dxdv, float a, float sg, fl
 // the z offset.  The hit-list kernel tests candidates on these 4 bytes instead of the
 // 16-byte position: bytes 0-1 read as one 16-bit number less 256 times a row segment's
 // first cell index give the x position in that segment in units of w/256, so a row of
-// three cells is scanned as one range.  The centre of code q, (q + 1/2) w / 256, is within
-// w / 512 of the offset on each axis.
+// three cells is scanned as one range.  q is the floor of the offset in units of w/256, so
+// the offset lies in [q, q + 1) units.
 __device__ inline uint32_t scan_code(const float o[3], int cx, const BoxDev& B)
 {
     uint32_t q[3];

@@ -145,8 +145,8 @@ constexpr int kHitThreads = 512;   // the default schedule's
 constexpr int kHitCap = 24;
 constexpr int kHitSlotBits = 26;
 constexpr uint32_t kHitSlotMask = (1u << kHitSlotBits) - 1u;
-// slots of padding after the last scan code: the scan reads candidates two at a time,
-// one pass ahead (three past the end at most)
+// slots of padding after the last scan code: the scan reads candidates four (or two) at a
+// time, one pass ahead (seven past the end at most)
 constexpr int kCodePad = 8;
 // 1e-18 < dr2 < 1 as one unsigned compare of the bits: dr2 >= 0, NaN fails
 constexpr uint32_t kPairLo = 0x219392EFu + 1u;   // bits of 1e-18f, plus one
@@ -155,8 +155,28 @@ __device__ inline bool pair_in_range(float dr2) { return __float_as_uint(dr2) - 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 // two scan codes from one 4-byte-aligned 8-byte load
 typedef uint32_t code2_t __attribute__((ext_vector_type(2), aligned(4)));
-// beads per workgroup of THREADS threads with SPLIT lanes per bead (21 per wavefront for 3)
-constexpr int hit_beads(int threads, int split) { return split == 1 ? threads : (threads / 64) * 21; }
+// four scan codes from one 4-byte-aligned 16-byte load
+typedef uint32_t code4_t __attribute__((ext_vector_type(4), aligned(4)));
+// two 16-bit integers in one register (v_pk_sub_i16, v_dot2_i32_i16)
+typedef short short2_t __attribute__((ext_vector_type(2)));
+
+// The candidate test of cubic cells in integer code units (u = w / 256): dx and dy as one
+// packed pair, dz apart, |d|^2 by one two-term dot product plus dz^2.  QXY holds the bead's
+// x (plus the segment's first cell index times 256, mod 2^16) and y in code units, Qz its z:
+// floors of its position seen from the row, as the codes are floors of the candidates'.  Each
+// axis is then within one unit of the true difference, so every pair inside the cut-off
+// passes |d| < (1 + m) / u with m = 1.01 |w| / 256 + 1e-5; the evaluation drops the rest.
+__device__ inline int code_dist2(uint32_t cj, uint32_t QXY, int Qz)
+{
+    // bytes 0, 1 (x16 of the row segment, before the first cell's offset) and 2 (y) of the code
+    const uint32_t p = __builtin_amdgcn_perm(0u, cj, 0x0c020100u);
+    const short2_t d = __builtin_bit_cast(short2_t, QXY) - __builtin_bit_cast(short2_t, p);
+    const int dz = Qz - (int)(cj >> 24);
+    return __builtin_amdgcn_sdot2(d, d, __mul24(dz, dz), false);
+}
+// beads per workgroup of THREADS threads with SPLIT lanes per bead (21 per wavefront for 3,
+// 7 for 9; lane 63 idle)
+constexpr int hit_beads(int threads, int split) { return split == 1 ? threads : (threads / 64) * (63 / split); }
 
 // The step's end and the next step's start for slot i, whose force (Fx, Fy, Fz) this launch
 // computed: the arithmetic of k_advance (MODE kEndBegin).  Used for engines without bonded
@@ -212,7 +232,10 @@ template <bool OBS, bool MULTI, int THREADS, int CAP, int FLUSH, int SPLIT, bool
 __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArgs& A, const BoxDev& B0, int nblocks,
                                                                    int per_xcd)
 {
-    static_assert(SPLIT == 1 || SPLIT == 3, "one or three lanes per bead");
+    static_assert(SPLIT == 1 || SPLIT == 3 || SPLIT == 9, "one, three or nine lanes per bead");
+    // candidates per pass of the scan (one 16- or 8-byte load of scan codes)
+    constexpr int PASS = SPLIT == 1 ? 4 : 2;
+    static_assert(CAP > PASS, "room for a pass");
     static_assert(!(OBS && EPI), "the observing launch ends a sequence of steps");
     __shared__ uint32_t hits[CAP][THREADS];
     __shared__ float stab[kTableStride];
@@ -236,11 +259,12 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
     __syncthreads();
     const int t = threadIdx.x;
     // SPLIT 3: lanes 3g, 3g+1, 3g+2 of a wavefront share bead g (21 per wavefront, lane 63
-    // idle) and scan the neighbour rows of z layers -1, 0, +1 each
+    // idle) and scan the neighbour rows of z layers -1, 0, +1 each; SPLIT 9: lanes 9g .. 9g+8
+    // share bead g (7 per wavefront) and scan one neighbour row each
     const int lane = t & 63;
-    const int grp = SPLIT == 1 ? 0 : lane / 3;
-    const int part = SPLIT == 1 ? 0 : lane - 3 * grp;
-    const int i = SPLIT == 1 ? i0 + t : i0 + (t >> 6) * 21 + grp;
+    const int grp = SPLIT == 1 ? 0 : lane / SPLIT;
+    const int part = SPLIT == 1 ? 0 : lane - SPLIT * grp;
+    const int i = SPLIT == 1 ? i0 + t : i0 + (t >> 6) * (63 / SPLIT) + grp;
     const bool valid = i < i1 && (SPLIT == 1 || lane < 63);
     double vir[10];
     if (OBS)
@@ -306,34 +330,25 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
             }
             wp = laddr;
         };
-        // candidate test on the scan codes.  A decoded offset is within w/512 of the true
-        // one on each axis, so the distance is within m of the true distance and every pair
-        // inside the cut-off passes |dr| < 1 + m; the evaluation drops the rest (the bead
-        // itself among them).  Test in code units: offset = (q + 1/2) u, u = w / 256.
-        const float ux = B.w[0] * (1.0f / 256.0f), uy = B.w[1] * (1.0f / 256.0f), uz = B.w[2] * (1.0f / 256.0f);
-        const float m = sqrtf(B.w[0] * B.w[0] + B.w[1] * B.w[1] + B.w[2] * B.w[2]) * (1.01f / 512.0f) + 1e-5f;
-        const uint32_t thr = __float_as_uint((1.0f + m) * (1.0f + m));
-        // one candidate of a row segment whose first cell has x index fx (fxbits = (fx mod
-        // 256) << 8): code bytes 0-1 less fxbits are r * 256 + q_x, r the candidate's cell
-        // counted from the segment's first one, so its x in the segment is that times ux
-        auto test = [&](uint32_t jc, uint32_t cj, uint32_t fxbits, float qx, float qy, float qz, bool in_seg) {
-            const uint32_t x16 = (cj - fxbits) & 0xFFFFu;
-            const float ddx = qx - (float)x16 * ux;
-            const float ddy = qy - (float)((cj >> 16) & 0xFFu) * uy;
-            const float ddz = qz - (float)(cj >> 24) * uz;
-            const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if ((__float_as_uint(dr2) < thr) & in_seg) {
-                // the candidate's neighbour-cell code: the segment's base plus r in the x field
-                *(lds_u32*)(size_t)wp = jc + ((x16 >> 8) << kHitSlotBits);
-                wp += 4 * THREADS;
-            }
-        };
+        // candidate test on the scan codes in integer code units (code_dist2): each axis of
+        // the tested difference is within one unit u_a = w_a / 256 of the true one, so the
+        // distance is within m = 1.01 |w| / 256 + 1e-5 of it and, with the sum of squares taken
+        // in units of the smallest u_a, every pair inside the cut-off passes
+        // sum d_a^2 < ((1 + m) / u_min)^2 (exact for cubic cells, a wider superset for
+        // others); the evaluation drops the rest (the bead itself among them).
+        const float inv_ux = 256.0f / B.w[0], inv_uy = 256.0f / B.w[1], inv_uz = 256.0f / B.w[2];
+        const float m = sqrtf(B.w[0] * B.w[0] + B.w[1] * B.w[1] + B.w[2] * B.w[2]) * (1.01f / 256.0f) + 1e-5f;
+        const float tu = (1.0f + m) * fmaxf(inv_ux, fmaxf(inv_uy, inv_uz));
+        const uint32_t thr = (uint32_t)(tu * tu) + 1u;
         const char* cbase = reinterpret_cast<const char*>(A.code);
-        const int dz0 = SPLIT == 1 ? -1 : part - 1, dz1 = SPLIT == 1 ? 1 : part - 1;
+        // the neighbour rows of this lane: all nine, one z layer (SPLIT 3) or one row (SPLIT 9)
+        const int dz0 = SPLIT == 1 ? -1 : (SPLIT == 3 ? part - 1 : part / 3 - 1);
+        const int dz1 = SPLIT == 1 ? 1 : dz0;
+        const int dy0 = SPLIT == 9 ? part % 3 - 1 : -1, dy1 = SPLIT == 9 ? dy0 : 1;
         for (int dz = dz0; dz <= dz1; ++dz) {
             int cz = c[2] + dz;
             cz += (cz < 0) ? B.n[2] : ((cz >= B.n[2]) ? -B.n[2] : 0);
-            for (int dy = -1; dy <= 1; ++dy) {
+            for (int dy = dy0; dy <= dy1; ++dy) {
                 int cy = c[1] + dy;
                 cy += (cy < 0) ? B.n[1] : ((cy >= B.n[1]) ? -B.n[1] : 0);
                 // the row's cells are consecutive keys: cells cx-1 .. cx+1 are one range, or two
@@ -343,10 +358,10 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                 // walk (x innermost).
                 const int krow = B.koff + B.n[0] * (cy + B.n[1] * cz);
                 const uint32_t qrow = (uint32_t)(((dy + 1) << 2) | ((dz + 1) << 4));
-                // the bead's y and z seen from this row (the shifts of the evaluation), less
-                // half a code unit
-                const float qy = (pi.y - (float)dy * B.w[1]) - 0.5f * uy;
-                const float qz = (pi.z - (float)dz * B.w[2]) - 0.5f * uz;
+                // the bead's y and z seen from this row (the shifts of the evaluation), in code
+                // units, floored as the codes are
+                const int Qy = (int)floorf((pi.y - (float)dy * B.w[1]) * inv_uy);
+                const int Qz = (int)floorf((pi.z - (float)dz * B.w[2]) * inv_uz);
                 const int cx = c[0], nx = B.n[0];
                 for (int sgm = 0; sgm < 2; ++sgm) {
                     int fx, lx, fu;
@@ -356,24 +371,54 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                     const int j0 = A.cstart[krow + fx], j1 = (lx >= fx) ? A.cstart[krow + lx + 1] : j0;
                     const uint32_t fxbits = (uint32_t)(fx & 0xFF) << 8;
                     const uint32_t code = ((uint32_t)(fu + 1) | qrow) << kHitSlotBits;
-                    const float qx = (pi.x - (float)fu * B.w[0]) - 0.5f * ux;
-                    // two consecutive candidates per pass from one 8-byte load, the next
-                    // pass's two loaded before these are tested.  (A wavefront runs each
-                    // segment to its lanes' longest count rounded up to the pass width.)  The
-                    // loads run up to three codes past the segment (the next cells' beads, or
-                    // the padding after the last slot); none of those is listed.  Slots are
-                    // below 2^26, so byte offsets fit 32 bits and the loads address from the
-                    // array base.
-                    code2_t c2 = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j0 << 2));
-                    for (int j = j0; j < j1; j += 2) {
-                        const code2_t cn = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j << 2) + 8);
-                        const uint32_t jc = (uint32_t)j | code;
-                        // room for the two: a list past CAP - 2 entries is evaluated
-                        // first (overflow inside a segment: this lane alone)
-                        if (wp > laddr + 4 * (CAP - 2) * THREADS) drain();
-                        test(jc, c2.x, fxbits, qx, qy, qz, true);
-                        test(jc + 1, c2.y, fxbits, qx, qy, qz, j + 1 < j1);
-                        c2 = cn;
+                    // the bead's x in the segment (code units) plus fxbits, mod 2^16: code bytes
+                    // 0-1 less fxbits are r * 256 + q_x, r the candidate's cell counted from the
+                    // segment's first one, so the packed difference is x less that
+                    const int Qx = (int)floorf((pi.x - (float)fu * B.w[0]) * inv_ux) + (int)fxbits;
+                    const uint32_t QXY = ((uint32_t)Qx & 0xFFFFu) | ((uint32_t)Qy << 16);
+                    // list entry of slot j: (j | code) + (r << 26), the neighbour-cell code of
+                    // the candidate's cell; r = (code byte 1 - fx) mod 64 = ((bits 8-13 of the
+                    // code) - fx) mod 64, and e0 - j0 holds the parts that do not depend on the
+                    // candidate
+                    const uint32_t e0 = ((uint32_t)j0 | code) - ((uint32_t)(fx & 0xFF) << kHitSlotBits);
+                    // PASS consecutive candidates per pass from one 16-byte (4) or 8-byte (2)
+                    // load, the next pass's loaded before these are tested.  (A wavefront
+                    // runs each segment to its lanes' longest count rounded up to the pass
+                    // width.)  The loads run up to seven codes past the segment (the next
+                    // cells' beads, or the padding after the last slot); none of those is
+                    // listed.  Slots are below 2^26, so byte offsets fit 32 bits and the loads
+                    // address from the array base.  Every candidate is stored at the list's
+                    // write pointer, which advances on a hit only (a miss is overwritten by
+                    // the next store).
+                    uint32_t e = e0;
+                    // one candidate: stored at the write pointer, which advances on a hit
+                    auto put = [&](uint32_t cj, uint32_t ej, bool in_seg) {
+                        const bool hit = ((uint32_t)code_dist2(cj, QXY, Qz) < thr) & in_seg;
+                        *(lds_u32*)(size_t)wp = ej + (((cj >> 8) & 63u) << kHitSlotBits);
+                        wp += hit ? 4 * THREADS : 0;
+                    };
+                    if constexpr (PASS == 4) {
+                        code4_t c4 = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j0 << 2));
+                        for (int j = j0; j < j1; j += 4, e += 4) {
+                            const code4_t cn = *reinterpret_cast<const code4_t*>(cbase + ((uint32_t)j << 2) + 16);
+                            // room for the pass: a list past CAP - 4 entries is evaluated first
+                            // (overflow inside a segment: this lane alone)
+                            if (wp > laddr + 4 * (CAP - 4) * THREADS) drain();
+                            put(c4.x, e, true);
+                            put(c4.y, e + 1u, j + 1 < j1);
+                            put(c4.z, e + 2u, j + 2 < j1);
+                            put(c4.w, e + 3u, j + 3 < j1);
+                            c4 = cn;
+                        }
+                    } else {
+                        code2_t c2 = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j0 << 2));
+                        for (int j = j0; j < j1; j += 2, e += 2) {
+                            const code2_t cn = *reinterpret_cast<const code2_t*>(cbase + ((uint32_t)j << 2) + 8);
+                            if (wp > laddr + 4 * (CAP - 2) * THREADS) drain();
+                            put(c2.x, e, true);
+                            put(c2.y, e + 1u, j + 1 < j1);
+                            c2 = cn;
+                        }
                     }
                     if (FLUSH < CAP && __ballot(wp >= laddr + 4 * FLUSH * THREADS) != 0) drain();
                 }
@@ -384,16 +429,20 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         if (OBS)
             for (int q = 0; q < 10; ++q) vir[q] = 0.5 * (double)sv[q];
     }
-    if (SPLIT == 3) {
-        // the bead's force: its z layers -1, 0, +1 added in that order (the whole
-        // wavefront takes part in the shuffles)
-        const int b = 3 * grp;
-        const float ax = __shfl(fx, b), ay = __shfl(fy, b), az = __shfl(fz, b);
-        const float bx = __shfl(fx, b + 1), by = __shfl(fy, b + 1), bz = __shfl(fz, b + 1);
-        const float cx = __shfl(fx, b + 2), cy = __shfl(fy, b + 2), cz = __shfl(fz, b + 2);
-        fx = (ax + bx) + cx;
-        fy = (ay + by) + cy;
-        fz = (az + bz) + cz;
+    if (SPLIT > 1) {
+        // the bead's force: its lanes' parts (z layers, or rows in z-major order) added in
+        // lane order (the whole wavefront takes part in the shuffles)
+        const int b = SPLIT * grp;
+        float sx = __shfl(fx, b), sy = __shfl(fy, b), sz = __shfl(fz, b);
+#pragma unroll
+        for (int q = 1; q < SPLIT; ++q) {
+            sx += __shfl(fx, b + q);
+            sy += __shfl(fy, b + q);
+            sz += __shfl(fz, b + q);
+        }
+        fx = sx;
+        fy = sy;
+        fz = sz;
         if (valid && part == 0) A.frc[i] = make_float4(fx, fy, fz, 0.f);
     }
     if (EPI) pair_epilogue(A, B, i, valid && part == 0, fx, fy, fz);
@@ -431,11 +480,11 @@ void k_pair_hitlist(PairArgs A, BoxDev B0, int nblocks, int per_xcd)
 struct Schedule {
     int kernel;    // DPD_PAIR_HITLIST or DPD_PAIR_FULLSHELL
     int threads, cap, flush;
-    int split;     // lanes per bead (hit list): 1, or 3 (one z layer of neighbour rows each)
+    int split;     // lanes per bead (hit list): 1, 3 (one z layer of neighbour rows each) or 9 (one row)
     const char* name;
     constexpr int beads() const { return hit_beads(threads, split); }   // per workgroup
 };
-constexpr int kSchedules = 6;
+constexpr int kSchedules = 7;
 constexpr Schedule kSchedule[kSchedules] = {
     {DPD_PAIR_HITLIST, 512, 24, 24, 1, "hitlist-t512-c24"},
     {DPD_PAIR_HITLIST, 512, 24, 16, 1, "hitlist-t512-c24-f16"},
@@ -443,6 +492,7 @@ constexpr Schedule kSchedule[kSchedules] = {
     {DPD_PAIR_HITLIST, 256, 32, 32, 1, "hitlist-t256-c32"},
     {DPD_PAIR_FULLSHELL, 256, 0, 0, 1, "fullshell-t256"},
     {DPD_PAIR_HITLIST, 256, 12, 12, 3, "hitlist-split3-t256-c12"},
+    {DPD_PAIR_HITLIST, 256, 8, 8, 9, "hitlist-split9-t256-c8"},
 };
 constexpr int kDefaultSchedule = 0;
 constexpr int kFullShellSchedule = 4;
@@ -465,6 +515,7 @@ inline void launch_hitlist(int sched, const PairArgs& a, const BoxDev& B0, int n
     case 2: launch_hitlist_s<2, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     case 3: launch_hitlist_s<3, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     case 5: launch_hitlist_s<5, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
+    case 6: launch_hitlist_s<6, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     default: launch_hitlist_s<0, OBS, MULTI, EPI>(a, B0, nblocks, s); break;
     }
 }
