@@ -59,8 +59,8 @@ struct Topo {
 
 struct Buffers {
     float4* pos;   // offset xyz, type | box << 8 | has bonds << 30
-    float4* vel;   // velocity xyz, id bits
-    float4* fold;  // previous force
+    float4* vel;   // velocity xyz, id bits: v at the state's time, or v + dt/2 F inside a sequence
+    float4* fold;  // the uploaded force (re-binning only)
     int* bead;     // bead index (upload order) of each slot
     int4* img;     // periodic image counts xyz by bead index (not sorted; one buffer)
     int* key;      // global cell key
@@ -288,11 +288,10 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
             frc[s] = f;
         }
         if (MODE == kEndBegin) {
-            const float4 fo = S.fold[s];
-            v.x = v.x + B.hdt * (f.x + fo.x);
-            v.y = v.y + B.hdt * (f.y + fo.y);
-            v.z = v.z + B.hdt * (f.z + fo.z);
-            S.vel[s] = v;
+            // UpdateMom: the stored velocity is v + dt/2 F_old; k_permute repeats this sum
+            v.x = kick(v.x, f.x, B.hdt);
+            v.y = kick(v.y, f.y, B.hdt);
+            v.z = kick(v.z, f.z, B.hdt);
             if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
                 atomicAdd(flags + 0, 1);
         }
@@ -302,9 +301,9 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
         const int img[3] = {0, 0, 0};   // the image change is applied by k_permute
         float d[3] = {0.f, 0.f, 0.f};
         if (MODE != kRebin) {
-            d[0] = B.dt * v.x + B.hdt2 * f.x;
-            d[1] = B.dt * v.y + B.hdt2 * f.y;
-            d[2] = B.dt * v.z + B.hdt2 * f.z;
+            d[0] = drift(v.x, f.x, B);
+            d[1] = drift(v.y, f.y, B);
+            d[2] = drift(v.z, f.z, B);
         }
         Moved m;
         advance(o, c, img, d, B, m);
@@ -466,8 +465,14 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     for (int e = c0; e < c1; ++e) dst += tmp[e].x < t0.x;
     const float4 p = S.pos[s];
     const BoxDev& B = box_of<MULTI>(boxes, B0, __float_as_int(p.w));
-    const float4 v = S.vel[s];
     const float4 f = (MODE == kRebin) ? S.fold[s] : frc[s];
+    float4 v = S.vel[s];
+    if (MODE == kEndBegin) {
+        // UpdateMom of the step that ends (the same sum as k_advance)
+        v.x = kick(v.x, f.x, B.hdt);
+        v.y = kick(v.y, f.y, B.hdt);
+        v.z = kick(v.z, f.z, B.hdt);
+    }
     const int q = t0.x;   // bead index of the source slot
     int c[3];
     key_cell(S.key[s] - B.koff, B, c);
@@ -475,15 +480,20 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     const int img[3] = {0, 0, 0};   // m.img is the change of image this step
     float d[3] = {0.f, 0.f, 0.f};
     if (MODE != kRebin) {
-        d[0] = B.dt * v.x + B.hdt2 * f.x;
-        d[1] = B.dt * v.y + B.hdt2 * f.y;
-        d[2] = B.dt * v.z + B.hdt2 * f.z;
+        d[0] = drift(v.x, f.x, B);
+        d[1] = drift(v.y, f.y, B);
+        d[2] = drift(v.z, f.z, B);
     }
     Moved m;
     advance(o, c, img, d, B, m);
     D.pos[dst] = make_float4(m.o[0], m.o[1], m.o[2], p.w);
-    D.vel[dst] = v;
-    D.fold[dst] = f;
+    if (MODE == kRebin) {
+        D.vel[dst] = v;
+        D.fold[dst] = f;
+    } else {
+        // the first half of the next UpdateMom: v + dt/2 F, so the force need not be kept
+        D.vel[dst] = make_float4(kick(v.x, f.x, B.hdt), kick(v.y, f.y, B.hdt), kick(v.z, f.z, B.hdt), v.w);
+    }
     D.bead[dst] = q;
     // the image counts live by bead index: only a bead that crosses the box changes them,
     // and only its own thread writes them
@@ -497,7 +507,7 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     if (MODE == kRebin) {
         vp[dst] = v;
     } else {
-        vp[dst] = make_float4(v.x + B.lamdt * f.x, v.y + B.lamdt * f.y, v.z + B.lamdt * f.z, v.w);
+        vp[dst] = make_float4(kick(v.x, f.x, B.lamdt), kick(v.y, f.y, B.lamdt), kick(v.z, f.z, B.lamdt), v.w);
     }
     // the slot of each bead is read for bonded beads only (bond and angle terms); a scattered
     // 4-byte store per bead costs a partial-line write each, so the others are skipped once
@@ -533,10 +543,10 @@ __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, const float4* 
         const float4 f = frc[s];
         float4 v = S.vel[s];
         if (UPDATE_V) {
-            const float4 fo = S.fold[s];
-            v.x = v.x + B.hdt * (f.x + fo.x);
-            v.y = v.y + B.hdt * (f.y + fo.y);
-            v.z = v.z + B.hdt * (f.z + fo.z);
+            // the second half of UpdateMom (the stored velocity is v + dt/2 F_old)
+            v.x = kick(v.x, f.x, B.hdt);
+            v.y = kick(v.y, f.y, B.hdt);
+            v.z = kick(v.z, f.z, B.hdt);
             S.vel[s] = v;
         }
         if (!(fabsf(f.x) + fabsf(f.y) + fabsf(f.z) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
@@ -889,7 +899,6 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr, bool epi = fals
     {
         const Buffers S = bufs(e, e->cur);
         a.vel = S.vel;
-        a.fold = S.fold;
         a.nkey = e->nkey;
         a.rank = e->rank;
         a.cnt = e->cnt;
@@ -1776,15 +1785,11 @@ int dpd_time_pair(dpd_engine* e, int32_t nlaunch, double* ms_per_launch)
     float4* keep = e->fold[e->cur ^ 1];
     HIPCHK(hipMemcpyAsync(keep, e->frc, f4, hipMemcpyDeviceToDevice, e->stream));
     // the launch a step that another follows uses: with the epilogue where the schedule has
-    // one, which also updates the velocities in place (kept in the inactive buffer), counts
-    // the counting sort's cells and may raise the flags (restored below)
+    // one, which also writes new cell keys and ranks, counts the counting sort's cells and
+    // may raise the flags (restored below)
     const bool epi = epilogue_ok(e);
-    float4* keepv = e->vel[e->cur ^ 1];
     int flags0[2] = {0, 0};
-    if (epi) {
-        HIPCHK(hipMemcpyAsync(keepv, e->vel[e->cur], f4, hipMemcpyDeviceToDevice, e->stream));
-        HIPCHK(hipMemcpyAsync(flags0, e->flags, sizeof(flags0), hipMemcpyDeviceToHost, e->stream));
-    }
+    if (epi) HIPCHK(hipMemcpyAsync(flags0, e->flags, sizeof(flags0), hipMemcpyDeviceToHost, e->stream));
     bool terms_done = false;
     if ((rc = launch_pair(e, false, &terms_done, epi))) return rc;   // untimed first launch
     HIPCHK(hipEventRecord(e->ev[0], e->stream));
@@ -1793,7 +1798,6 @@ int dpd_time_pair(dpd_engine* e, int32_t nlaunch, double* ms_per_launch)
     HIPCHK(hipEventRecord(e->ev[1], e->stream));
     HIPCHK(hipMemcpyAsync(e->frc, keep, f4, hipMemcpyDeviceToDevice, e->stream));
     if (epi) {
-        HIPCHK(hipMemcpyAsync(e->vel[e->cur], keepv, f4, hipMemcpyDeviceToDevice, e->stream));
         HIPCHK(hipMemcpyAsync(e->flags, flags0, sizeof(flags0), hipMemcpyHostToDevice, e->stream));
         HIPCHK(hipMemsetAsync(e->cnt, 0, sizeof(int) * (size_t)e->nkeys, e->stream));
         HIPCHK(hipMemsetAsync(e->gcnt, 0, sizeof(int) * (size_t)e->ngroups, e->stream));

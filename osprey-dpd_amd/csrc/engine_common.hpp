@@ -151,6 +151,13 @@ __device__ inline uint32_t scan_code(const float o[3], int cx, const BoxDev& B)
     return q[0] | ((uint32_t)(cx & 0xFF) << 8) | (q[1] << 16) | (q[2] << 24);
 }
 
+// The integration arithmetic with its rounding fixed by explicit fused multiply-adds, so
+// that kernels repeating a sum (k_advance and k_permute, the pair epilogue, the last step's
+// k_finish) get the same bits however the compiler contracts the rest: kick is v + h f (the
+// halves of UpdateMom, the predicted velocity), drift the displacement dt v + dt^2/2 f.
+__device__ inline float kick(float v, float f, float h) { return fmaf(h, f, v); }
+__device__ inline float drift(float v, float f, const BoxDev& B) { return fmaf(B.dt, v, B.hdt2 * f); }
+
 // Move a bead by displacement d from offset o in cell c (image img): new offset in
 // [0, w), cell and image; `jump` when it crossed more than one cell.
 struct Moved {

@@ -38,8 +38,7 @@ struct PairArgs {
     // the epilogue of a step that another follows (hit-list kernels, engines without bonded
     // terms): UpdateMom of this step, UpdatePos of the next, the new cell key and
     // counting-sort rank
-    float4* vel;           // velocity, updated in place
-    const float4* fold;    // force of the previous step
+    const float4* vel;     // v + dt/2 F of the previous step
     int* nkey;             // new cell key per slot
     int* rank;             // rank inside the new cell
     int* cnt;              // beads per cell key (counting sort)
@@ -191,19 +190,18 @@ __device__ inline void pair_epilogue(const PairArgs& A, const BoxDev& B, int i, 
         const float4 p = A.pos[i];
         act = !(__float_as_int(p.w) & kBondedBit);
         if (act) {
-            const float4 fo = A.fold[i];
+            // UpdateMom (k_permute repeats the sum and stores the next half step)
             float4 v = A.vel[i];
-            v.x = v.x + B.hdt * (Fx + fo.x);
-            v.y = v.y + B.hdt * (Fy + fo.y);
-            v.z = v.z + B.hdt * (Fz + fo.z);
-            A.vel[i] = v;
+            v.x = kick(v.x, Fx, B.hdt);
+            v.y = kick(v.y, Fy, B.hdt);
+            v.z = kick(v.z, Fz, B.hdt);
             if (!(fabsf(Fx) + fabsf(Fy) + fabsf(Fz) < 1e30f) || !(fabsf(v.x) + fabsf(v.y) + fabsf(v.z) < 1e30f))
                 atomicAdd(A.flags + 0, 1);
             int c[3];
             key_cell(A.key[i] - B.koff, B, c);
             const float o[3] = {p.x, p.y, p.z};
             const int img[3] = {0, 0, 0};   // the image change is applied by k_permute
-            const float d[3] = {B.dt * v.x + B.hdt2 * Fx, B.dt * v.y + B.hdt2 * Fy, B.dt * v.z + B.hdt2 * Fz};
+            const float d[3] = {drift(v.x, Fx, B), drift(v.y, Fy, B), drift(v.z, Fz, B)};
             Moved m;
             advance(o, c, img, d, B, m);
             k = B.koff + cell_key(m.c[0], m.c[1], m.c[2], B);
