@@ -1,8 +1,8 @@
 // mlp.hip -- surrogate network: BF16 matrix-core forward/backward and FP32 Adam on gfx950.
 //
-// Every matrix product runs as 16x16 output tiles, one wavefront per tile, on
-// v_mfma_f32_16x16x16_bf16 (BF16 operands, FP32 accumulation).  Operand fragment layout of
-// that instruction for lane l (0..63): A holds A[l%16][4(l/16) .. 4(l/16)+3], B holds
+// Every matrix product runs on v_mfma_f32_16x16x16_bf16 (BF16 operands, FP32 accumulation)
+// in LDS-staged workgroup tiles (k_gemm_tile).  Operand fragment layout of that instruction
+// for lane l (0..63): A holds A[l%16][4(l/16) .. 4(l/16)+3], B holds
 // B[4(l/16) .. 4(l/16)+3][l%16], and the accumulator holds C[4(l/16)+i][l%16], i = 0..3.
 // Strided operand addressing gives C = A B, A^T B and A B^T from one kernel.
 #include <hip/hip_runtime.h>
@@ -20,7 +20,6 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTile = 16;
 
 __host__ __device__ inline uint16_t f2bf(float f)
 {
@@ -31,27 +30,108 @@ __host__ __device__ inline uint16_t f2bf(float f)
     return (uint16_t)(u >> 16);
 }
 
-// C[m][n] (ldc) = sum_k A(m,k) B(k,n);  A(m,k) = A[m*am + k*ak], B(k,n) = B[k*bk + n*bn].
-// grid (N/16, M/16), one 64-lane wavefront per 16x16 tile; M, N, K multiples of 16.
-__global__ __launch_bounds__(64) void k_gemm_bf16(int K, const uint16_t* __restrict__ A, int am, int ak,
-                                                  const uint16_t* __restrict__ B, int bk, int bn,
-                                                  float* __restrict__ C, int ldc)
+// C[m][n] (ldc) = sum_k A(m,k) B(k,n);  A(m,k) = A[m*am + k*ak], B(k,n) = B[k*bk + n*bn],
+// with am or ak equal to 1 and bk or bn equal to 1 (each operand contiguous along one of its
+// dimensions).  A workgroup of GM x GN wavefronts computes a TM x TN tile, each wavefront
+// FM x FN fragments of 16 x 16.  K advances in chunks of kKC staged in LDS, both operands as
+// [row][k] rows (k contiguous, padded by 4 to spread the banks): the global loads are 8-byte
+// vectors along each operand's contiguous dimension (coalesced across the workgroup), and
+// each fragment operand is one 8-byte LDS read.  blockIdx.z takes the K range
+// [z kper, (z + 1) kper); with several, the tile goes to partial buffer z (C + z cstride) and
+// k_sum_partials adds them in order.  M, N, K, kper: multiples of 16.
+constexpr int kKC = 32;
+constexpr int kLDK = kKC + 4;
+template <int FM, int FN, int GM, int GN>
+__global__ __launch_bounds__(64 * GM * GN) void k_gemm_tile(int M, int N, int K, int kper,
+                                                            const uint16_t* __restrict__ A, int am, int ak,
+                                                            const uint16_t* __restrict__ B, int bk, int bn,
+                                                            float* __restrict__ C, int ldc, int64_t cstride)
 {
-    const int lane = threadIdx.x;
-    const int n0 = blockIdx.x * kTile, m0 = blockIdx.y * kTile;
+    constexpr int TM = 16 * FM * GM, TN = 16 * FN * GN, NT = 64 * GM * GN;
+    __shared__ __align__(16) uint16_t As[TM][kLDK];
+    __shared__ __align__(16) uint16_t Bs[TN][kLDK];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w / GN, wn = w % GN;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kz0 = blockIdx.z * kper, kz1 = min(K, kz0 + kper);
     const int r = lane & 15, q = (lane >> 4) * 4;
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += kTile) {
-        bf16x4 a, b;
+    floatx4 acc[FM][FN];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            a[i] = (short)A[(size_t)(m0 + r) * am + (size_t)(k0 + q + i) * ak];
-            b[i] = (short)B[(size_t)(k0 + q + i) * bk + (size_t)(n0 + r) * bn];
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = kz0; kc < kz1; kc += kKC) {
+        // stage A: TM x kKC, zero beyond M and kz1 (kz1 - kc is a multiple of 16)
+        for (int v = t; v < TM * kKC / 4; v += NT) {
+            int mm, kk;
+            if (ak == 1) { mm = v / (kKC / 4); kk = (v % (kKC / 4)) * 4; }
+            else { kk = v / (TM / 4); mm = (v % (TM / 4)) * 4; }
+            uint2 x = make_uint2(0u, 0u);
+            const int gm = m0 + mm, gk = kc + kk;
+            if (gm < M && gk < kz1) x = *reinterpret_cast<const uint2*>(A + (size_t)gm * am + (size_t)gk * ak);
+            if (ak == 1) {
+                *reinterpret_cast<uint2*>(&As[mm][kk]) = x;
+            } else {
+                As[mm][kk] = (uint16_t)(x.x & 0xFFFFu); As[mm + 1][kk] = (uint16_t)(x.x >> 16);
+                As[mm + 2][kk] = (uint16_t)(x.y & 0xFFFFu); As[mm + 3][kk] = (uint16_t)(x.y >> 16);
+            }
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
+        // stage B: kKC x TN as [n][k]
+        for (int v = t; v < TN * kKC / 4; v += NT) {
+            int nn, kk;
+            if (bk == 1) { nn = v / (kKC / 4); kk = (v % (kKC / 4)) * 4; }
+            else { kk = v / (TN / 4); nn = (v % (TN / 4)) * 4; }
+            uint2 x = make_uint2(0u, 0u);
+            const int gn = n0 + nn, gk = kc + kk;
+            if (gn < N && gk < kz1) x = *reinterpret_cast<const uint2*>(B + (size_t)gk * bk + (size_t)gn * bn);
+            if (bk == 1) {
+                *reinterpret_cast<uint2*>(&Bs[nn][kk]) = x;
+            } else {
+                Bs[nn][kk] = (uint16_t)(x.x & 0xFFFFu); Bs[nn + 1][kk] = (uint16_t)(x.x >> 16);
+                Bs[nn + 2][kk] = (uint16_t)(x.y & 0xFFFFu); Bs[nn + 3][kk] = (uint16_t)(x.y >> 16);
+            }
+        }
+        __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 4; ++i) C[(size_t)(m0 + q + i) * ldc + n0 + r] = acc[i];
+        for (int ks = 0; ks < kKC; ks += 16) {
+            bf16x4 a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+                a[i] = *reinterpret_cast<const bf16x4*>(&As[(wm * FM + i) * 16 + r][ks + q]);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                b[j] = *reinterpret_cast<const bf16x4*>(&Bs[(wn * FN + j) * 16 + r][ks + q]);
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float* Cz = C + (int64_t)blockIdx.z * cstride;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int gn = n0 + (wn * FN + j) * 16 + r;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int gm = m0 + (wm * FM + i) * 16 + q + e;
+                if (gm < M && gn < N) Cz[(size_t)gm * ldc + gn] = acc[i][j][e];
+            }
+        }
+}
+
+// C[m][n] (row stride ldc) = sum over z = 0 .. nz-1 of P[z][m][n] (M x N each), in that order
+__global__ void k_sum_partials(int M, int N, int nz, const float* __restrict__ P, int64_t pstride,
+                               float* __restrict__ C, int ldc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * N) return;
+    float s = 0.f;
+    for (int z = 0; z < nz; ++z) s += P[(int64_t)z * pstride + i];
+    C[(size_t)(i / N) * ldc + i % N] = s;
 }
 
 // H = tanh(Z + b) for rows < rows_valid (0 beyond), FP32 and BF16 copies
@@ -101,14 +181,24 @@ __global__ void k_tanh_grad(int total, const float* __restrict__ dH, const float
     dZb[i] = f2bf(g);
 }
 
-// db[n] = sum over rows of dZ[m][n]
-__global__ void k_colsum(int M, int N, const float* __restrict__ dZ, float* __restrict__ db)
+// db[n] = sum over rows of dZ[m][n]: workgroup b takes columns 64b .. 64b+63, wavefront w
+// of 16 the rows m = w (mod 16) (coalesced rows of 64 columns), the 16 partial sums added
+// in wavefront order through LDS
+__global__ __launch_bounds__(1024) void k_colsum(int M, int N, const float* __restrict__ dZ, float* __restrict__ db)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    __shared__ float part[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (int m = 0; m < M; ++m) s += dZ[(size_t)m * N + n];
-    db[n] = s;
+    if (n < N)
+        for (int m = w; m < M; m += 16) s += dZ[(size_t)m * N + n];
+    part[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && n < N) {
+        float t = 0.f;
+        for (int q = 0; q < 16; ++q) t += part[q][lane];
+        db[n] = t;
+    }
 }
 
 // Adam on a parameter block; entries with mask 0 (padding) stay zero
@@ -158,6 +248,8 @@ struct dpd_mlp {
     float *dY = nullptr, *dH2 = nullptr, *dZ2 = nullptr, *dH1 = nullptr, *dZ1 = nullptr;
     double* loss = nullptr;
     int loss_cap = 0;
+    float* part = nullptr;    // split-K partial tiles
+    int64_t part_cap = 0;
     std::vector<double> in_mean, in_scale, out_mean, out_scale;
     std::string err;
 };
@@ -221,11 +313,49 @@ int ensure_rows(dpd_mlp* m, int rows)
     return 0;
 }
 
-void gemm(dpd_mlp* m, int M, int N, int K, const uint16_t* A, int am, int ak, const uint16_t* B, int bk, int bn,
-          float* C, int ldc)
+// tile shapes: 64 x 64 (four wavefronts of 2 x 2 fragments), 256 x 16 for N = 16 outputs,
+// 16 x 256 for M = 16; K split over workgroups when the tiles alone would leave most CUs idle
+int gemm(dpd_mlp* m, int M, int N, int K, const uint16_t* A, int am, int ak, const uint16_t* B, int bk, int bn,
+         float* C, int ldc)
 {
-    hipLaunchKernelGGL(k_gemm_bf16, dim3(N / kTile, M / kTile), dim3(64), 0, m->stream, K, A, am, ak, B, bk, bn, C,
-                       ldc);
+    if ((am != 1 && ak != 1) || (bk != 1 && bn != 1) || M % 16 || N % 16 || K % 16)
+        return mfail(m, "gemm: operands must be contiguous along one dimension, sizes multiples of 16");
+    int TM = 64, TN = 64;
+    if (N < 64) { TM = 256; TN = 16; }
+    else if (M < 64) { TM = 16; TN = 256; }
+    const int gx = (N + TN - 1) / TN, gy = (M + TM - 1) / TM;
+    int nz = 1;
+    while (gx * gy * nz < 128 && K / (2 * nz) >= 4 * kKC) nz *= 2;
+    const int kper = (K / 16 + nz - 1) / nz * 16;
+    nz = (K + kper - 1) / kper;
+    float* out = C;
+    int ldo = ldc;
+    int64_t cstride = 0;
+    if (nz > 1) {
+        const int64_t need = (int64_t)nz * M * N;
+        if (need > m->part_cap) {
+            int rc;
+            if ((rc = malloc_dev(m, &m->part, need))) return rc;
+            m->part_cap = need;
+        }
+        out = m->part;
+        ldo = N;
+        cstride = (int64_t)M * N;
+    }
+    const dim3 grid(gx, gy, nz);
+    if (TN == 16)
+        hipLaunchKernelGGL((k_gemm_tile<4, 1, 4, 1>), grid, dim3(256), 0, m->stream, M, N, K, kper, A, am, ak, B, bk,
+                           bn, out, ldo, cstride);
+    else if (TM == 16)
+        hipLaunchKernelGGL((k_gemm_tile<1, 4, 1, 4>), grid, dim3(256), 0, m->stream, M, N, K, kper, A, am, ak, B, bk,
+                           bn, out, ldo, cstride);
+    else
+        hipLaunchKernelGGL((k_gemm_tile<2, 2, 2, 2>), grid, dim3(256), 0, m->stream, M, N, K, kper, A, am, ak, B, bk,
+                           bn, out, ldo, cstride);
+    if (nz > 1)
+        hipLaunchKernelGGL(k_sum_partials, dim3(blocks((int64_t)M * N)), dim3(256), 0, m->stream, M, N, nz, m->part,
+                           cstride, C, ldc);
+    return 0;
 }
 
 // standardised inputs (bf16, padded to kin) and targets (fp32, padded to kout)
@@ -246,40 +376,44 @@ int upload_batch(dpd_mlp* m, int64_t n, const double* x, const double* y, int Mp
     return 0;
 }
 
-void forward(dpd_mlp* m, int n, int Mp, double* loss)
+int forward(dpd_mlp* m, int n, int Mp, double* loss)
 {
     const int H = m->H, ko = m->kout;
-    gemm(m, Mp, H, m->kin, m->Xb, m->kin, 1, m->W1b, H, 1, m->Z1, H);
+    int rc;
+    if ((rc = gemm(m, Mp, H, m->kin, m->Xb, m->kin, 1, m->W1b, H, 1, m->Z1, H))) return rc;
     hipLaunchKernelGGL(k_bias_tanh, dim3(blocks((int64_t)Mp * H)), dim3(256), 0, m->stream, Mp, H, n, m->Z1, m->b1,
                        m->H1, m->H1b);
-    gemm(m, Mp, H, H, m->H1b, H, 1, m->W2b, H, 1, m->Z2, H);
+    if ((rc = gemm(m, Mp, H, H, m->H1b, H, 1, m->W2b, H, 1, m->Z2, H))) return rc;
     hipLaunchKernelGGL(k_bias_tanh, dim3(blocks((int64_t)Mp * H)), dim3(256), 0, m->stream, Mp, H, n, m->Z2, m->b2,
                        m->H2, m->H2b);
-    gemm(m, Mp, ko, H, m->H2b, H, 1, m->W3b, ko, 1, m->Z3, ko);
+    if ((rc = gemm(m, Mp, ko, H, m->H2b, H, 1, m->W3b, ko, 1, m->Z3, ko))) return rc;
     hipLaunchKernelGGL(k_output, dim3(blocks((int64_t)Mp * ko)), dim3(256), 0, m->stream, Mp, ko, n, m->nout, m->Z3,
                        m->b3, m->T, m->Y, m->dY, m->dYb, loss);
+    return 0;
 }
 
-void backward(dpd_mlp* m, int Mp)
+int backward(dpd_mlp* m, int Mp)
 {
     const int H = m->H, ko = m->kout, ki = m->kin;
+    int rc;
     // dW3 = H2^T dY, db3 = colsum dY
-    gemm(m, H, ko, Mp, m->H2b, 1, H, m->dYb, ko, 1, m->gW3, ko);
-    hipLaunchKernelGGL(k_colsum, dim3(blocks(ko)), dim3(256), 0, m->stream, Mp, ko, m->dY, m->gb3);
+    if ((rc = gemm(m, H, ko, Mp, m->H2b, 1, H, m->dYb, ko, 1, m->gW3, ko))) return rc;
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(ko, 64)), dim3(1024), 0, m->stream, Mp, ko, m->dY, m->gb3);
     // dH2 = dY W3^T
-    gemm(m, Mp, H, ko, m->dYb, ko, 1, m->W3b, 1, ko, m->dH2, H);
+    if ((rc = gemm(m, Mp, H, ko, m->dYb, ko, 1, m->W3b, 1, ko, m->dH2, H))) return rc;
     hipLaunchKernelGGL(k_tanh_grad, dim3(blocks((int64_t)Mp * H)), dim3(256), 0, m->stream, Mp * H, m->dH2, m->H2,
                        m->dZ2, m->dZ2b);
     // dW2 = H1^T dZ2, db2
-    gemm(m, H, H, Mp, m->H1b, 1, H, m->dZ2b, H, 1, m->gW2, H);
-    hipLaunchKernelGGL(k_colsum, dim3(blocks(H)), dim3(256), 0, m->stream, Mp, H, m->dZ2, m->gb2);
+    if ((rc = gemm(m, H, H, Mp, m->H1b, 1, H, m->dZ2b, H, 1, m->gW2, H))) return rc;
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(H, 64)), dim3(1024), 0, m->stream, Mp, H, m->dZ2, m->gb2);
     // dH1 = dZ2 W2^T
-    gemm(m, Mp, H, H, m->dZ2b, H, 1, m->W2b, 1, H, m->dH1, H);
+    if ((rc = gemm(m, Mp, H, H, m->dZ2b, H, 1, m->W2b, 1, H, m->dH1, H))) return rc;
     hipLaunchKernelGGL(k_tanh_grad, dim3(blocks((int64_t)Mp * H)), dim3(256), 0, m->stream, Mp * H, m->dH1, m->H1,
                        m->dZ1, m->dZ1b);
     // dW1 = X^T dZ1, db1
-    gemm(m, ki, H, Mp, m->Xb, 1, ki, m->dZ1b, H, 1, m->gW1, H);
-    hipLaunchKernelGGL(k_colsum, dim3(blocks(H)), dim3(256), 0, m->stream, Mp, H, m->dZ1, m->gb1);
+    if ((rc = gemm(m, ki, H, Mp, m->Xb, 1, ki, m->dZ1b, H, 1, m->gW1, H))) return rc;
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(H, 64)), dim3(1024), 0, m->stream, Mp, H, m->dZ1, m->gb1);
+    return 0;
 }
 
 void refresh_bf16(dpd_mlp* m)
@@ -372,7 +506,7 @@ void dpd_mlp_destroy(dpd_mlp* m)
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     void* ptrs[] = {m->W1, m->gW1, m->mom, m->var, m->mask, m->W1b, m->W2b, m->W3b, m->Xb, m->H1b, m->H2b, m->dYb,
                     m->dZ2b, m->dZ1b, m->Z1, m->H1, m->Z2, m->H2, m->Z3, m->Y, m->T, m->dY, m->dH2, m->dZ2,
-                    m->dH1, m->dZ1, m->loss};
+                    m->dH1, m->dZ1, m->loss, m->part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -439,8 +573,7 @@ int dpd_mlp_gradients(dpd_mlp* m, int64_t n, const double* x, const double* y, d
         m->loss_cap = 1;
     }
     MCHK(hipMemsetAsync(m->loss, 0, sizeof(double), m->stream));
-    forward(m, (int)n, Mp, m->loss);
-    backward(m, Mp);
+    if ((rc = forward(m, (int)n, Mp, m->loss)) || (rc = backward(m, Mp))) return rc;
     MCHK(hipGetLastError());
     double l = 0;
     MCHK(hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
@@ -470,8 +603,7 @@ int dpd_mlp_train(dpd_mlp* m, int64_t n, const double* x, const double* y, int32
     MCHK(hipMemsetAsync(m->loss, 0, sizeof(double) * std::max(iters, 1), m->stream));
     const int64_t tot = m->nparam_pad;
     for (int32_t it = 0; it < iters; ++it) {
-        forward(m, (int)n, Mp, m->loss + it);
-        backward(m, Mp);
+        if ((rc = forward(m, (int)n, Mp, m->loss + it)) || (rc = backward(m, Mp))) return rc;
         m->adam_t += 1;
         const float c1 = 1.0f - std::pow(0.9f, (float)m->adam_t);
         const float c2 = 1.0f - std::pow(0.999f, (float)m->adam_t);
@@ -493,7 +625,7 @@ int dpd_mlp_predict(dpd_mlp* m, int64_t n, const double* x, double* y)
     const int Mp = pad16(n);
     int rc;
     if ((rc = ensure_rows(m, Mp)) || (rc = upload_batch(m, n, x, nullptr, Mp))) return rc;
-    forward(m, (int)n, Mp, nullptr);
+    if ((rc = forward(m, (int)n, Mp, nullptr))) return rc;
     MCHK(hipGetLastError());
     std::vector<float> out((size_t)Mp * m->kout);
     MCHK(hipMemcpyAsync(out.data(), m->Y, sizeof(float) * out.size(), hipMemcpyDeviceToHost, m->stream));

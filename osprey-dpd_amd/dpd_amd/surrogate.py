@@ -1,11 +1,15 @@
 """Batched replica runs and the surrogate that screens parameter sweeps.
 
 A replica is a cubic water/lipid box (H3(T4)2 lipids started as a bilayer at the
-mid-plane, State lamella) whose run parameters are the six surrogate features:
-a_HW, a_TW, a_HT, a_TT (a_HH = a_WW = 25), the lipid bead fraction and the Hookean bond
-stiffness.  Hundreds of replicas of uneven size advance together in one batched engine;
-the time-averaged pressure and the mesh area per lipid of each replica are the targets.
-The surrogate (dpd_amd.mlp.Surrogate, BF16 matrix cores) is trained on those runs and
+mid-plane, State lamella) whose run parameters are six interaction and composition
+features -- a_HW, a_TW, a_HT, a_TT (a_HH = a_WW = 25), the lipid bead fraction and the
+Hookean bond stiffness -- and the box side L ~ U{8..24}.  Hundreds of replicas of uneven
+size advance together in one batched engine; the time-averaged pressure and the mesh area
+per lipid of each replica are the targets.  The surrogate (dpd_amd.mlp.Surrogate, BF16
+matrix cores) takes the six parameters and the box side: in a periodic box of fixed size
+the bilayer's area per lipid is set by the box (about 2 L^2 / lipids, lipids ~ L^3) and the
+pressure carries the bilayer's tension over the box height, so a map from the six
+parameters alone leaves scatter no training removes.  It is trained on those runs and
 screens candidate parameter sets; the simulation then checks every accepted candidate.
 """
 import time
@@ -17,7 +21,8 @@ from . import analysis as A
 from .host import Input, replica_batch
 from .mlp import Surrogate
 
-FEATURES = ("a_HW", "a_TW", "a_HT", "a_TT", "lipid_fraction", "k_bond")
+PARAMS = ("a_HW", "a_TW", "a_HT", "a_TT", "lipid_fraction", "k_bond")
+FEATURES = PARAMS + ("box_side",)
 LOW = np.array([20.0, 20.0, 20.0, 20.0, 0.10, 64.0])
 HIGH = np.array([90.0, 90.0, 90.0, 90.0, 0.35, 256.0])
 
@@ -81,6 +86,16 @@ def sample_params(n, rng):
     return LOW + (HIGH - LOW) * rng.uniform(size=(n, len(LOW)))
 
 
+def sample_sizes(n, rng, lo=8, hi=24):
+    """Box sides L ~ U{lo..hi}"""
+    return rng.integers(lo, hi + 1, size=n)
+
+
+def features(params, sizes):
+    """The surrogate's inputs: the six run parameters and the box side"""
+    return np.column_stack([np.asarray(params, float), np.asarray(sizes, float)])
+
+
 def replica_text(p, L, seed, steps=5000, dt=0.02):
     aHW, aTW, aHT, aTT, c, k = p
     fl = c / (11.0 - 10.0 * c)          # lipid number fraction giving bead fraction c
@@ -136,7 +151,8 @@ def train(X, Y, iters=3000, lr=2e-3, seed=1, hidden=256):
 
 
 def screen(net, candidates, target, tol):
-    """Candidates whose predicted pressure lies within `tol` (relative) of `target`."""
+    """Candidates (rows of features()) whose predicted pressure lies within `tol`
+    (relative) of `target`."""
     pred = net.predict(candidates)
     keep = np.abs(pred[:, 0] - target) <= tol * target
     return np.where(keep)[0], pred
