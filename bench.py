@@ -166,6 +166,30 @@ def bench_configs(steps=100, warmup=20):
     return out
 
 
+def bench_mlp(nrows=1638, iters=1000, warm=50, seed=3):
+    """The surrogate network's training iteration (7 -> 256 -> 256 -> 2: BF16 matrix-core
+    forward and backward, FP32 Adam) on a synthetic full batch the size of the replica
+    dataset's training split: ms per iteration and the rate of its eight matrix products
+    (FLOPs of the padded shapes: inputs and outputs padded to 16)."""
+    from dpd_amd.mlp import Surrogate
+    rng = np.random.default_rng(seed)
+    nin, H, nout = 7, 256, 2
+    X = rng.normal(size=(nrows, nin))
+    Y = rng.normal(size=(nrows, nout))
+    net = Surrogate(nin, H, nout, seed=1)
+    net.set_scaling(np.zeros(nin), np.ones(nin), np.zeros(nout), np.ones(nout))
+    net.train(X, Y, iters=warm, lr=1e-3)
+    t0 = time.perf_counter()
+    net.train(X, Y, iters=iters, lr=1e-3)
+    wall = time.perf_counter() - t0
+    net.close()
+    mp, ki, ko = (nrows + 15) // 16 * 16, 16, 16
+    flops = 2 * mp * (2 * ki * H + 3 * H * H + 3 * H * ko)
+    ms = wall / iters * 1e3
+    return {"rows": nrows, "iters": iters, "ms_per_iter": ms, "gemm_flops_per_iter": flops,
+            "gemm_tflops": flops / (ms * 1e-3) / 1e12, "net": f"{nin}-{H}-{H}-{nout}"}
+
+
 def cpu_baseline_reference(t1=2, t2=12):
     """Time the reference binary on the condensate configuration: run lengths t1 and t2
     concurrently (one core each); per-step time = (wall2 - wall1)/(t2 - t1)."""
@@ -364,12 +388,17 @@ def main():
                     help="only run one 256 MiB device-to-device copy (calibrates the PMC byte counters)")
     ap.add_argument("--configs-only", action="store_true",
                     help="only the water, binary and bilayer boxes (other_configs), one JSON line")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="only the surrogate network's training iteration, one JSON line")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
         return
     if args.configs_only:
         print(json.dumps({"other_configs": bench_configs()}), flush=True)
+        return
+    if args.mlp_only:
+        print(json.dumps({"mlp": bench_mlp()}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (multi-GPU "

@@ -181,39 +181,47 @@ __global__ void k_tanh_grad(int total, const float* __restrict__ dH, const float
     dZb[i] = f2bf(g);
 }
 
-// db[n] = sum over rows of dZ[m][n]: workgroup b takes columns 64b .. 64b+63, wavefront w
-// of 16 the rows m = w (mod 16) (coalesced rows of 64 columns), the 16 partial sums added
-// in wavefront order through LDS
-__global__ __launch_bounds__(1024) void k_colsum(int M, int N, const float* __restrict__ dZ, float* __restrict__ db)
+// Column sums of dZ (M x N) for the bias gradients, in two ordered passes: workgroup (b, z)
+// takes columns 64b .. 64b+63 and the rows of slab z (kColSlab of them), wavefront w of 16
+// the slab's rows m = w (mod 16) (coalesced rows of 64 columns), and writes the slab's sum
+// P[z][n] (its 16 wavefront sums added in order through LDS); k_sum_partials then adds the
+// slabs in order.
+constexpr int kColSlab = 128;
+__global__ __launch_bounds__(1024) void k_colsum(int M, int N, const float* __restrict__ dZ, float* __restrict__ P)
 {
     __shared__ float part[16][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = blockIdx.x * 64 + lane;
+    const int m1 = min(M, (int)(blockIdx.y + 1) * kColSlab);
     float s = 0.f;
     if (n < N)
-        for (int m = w; m < M; m += 16) s += dZ[(size_t)m * N + n];
+        for (int m = blockIdx.y * kColSlab + w; m < m1; m += 16) s += dZ[(size_t)m * N + n];
     part[w][lane] = s;
     __syncthreads();
     if (w == 0 && n < N) {
         float t = 0.f;
         for (int q = 0; q < 16; ++q) t += part[q][lane];
-        db[n] = t;
+        P[(size_t)blockIdx.y * N + n] = t;
     }
 }
 
-// Adam on a parameter block; entries with mask 0 (padding) stay zero
+// Adam on the parameter array; entries with mask 0 (padding) stay zero.  Also refreshes the
+// BF16 copy (same layout) the matrix products read.
 __global__ void k_adam(int total, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ mo,
-                       float* __restrict__ ve, const uint8_t* __restrict__ mask, float lr, float c1, float c2)
+                       float* __restrict__ ve, const uint8_t* __restrict__ mask, float lr, float c1, float c2,
+                       uint16_t* __restrict__ wb)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    if (mask && !mask[i]) { w[i] = 0.f; return; }
+    if (mask && !mask[i]) { w[i] = 0.f; wb[i] = 0; return; }
     const float gi = g[i];
     const float m = 0.9f * mo[i] + 0.1f * gi;
     const float v = 0.999f * ve[i] + 0.001f * gi * gi;
     mo[i] = m;
     ve[i] = v;
-    w[i] -= lr * (m / c1) / (sqrtf(v / c2) + 1e-8f);
+    const float wi = w[i] - lr * (m / c1) / (sqrtf(v / c2) + 1e-8f);
+    w[i] = wi;
+    wb[i] = f2bf(wi);
 }
 
 __global__ void k_to_bf16(int total, const float* __restrict__ w, uint16_t* __restrict__ wb)
@@ -233,7 +241,8 @@ struct dpd_mlp {
     hipStream_t stream = nullptr;
     // FP32 parameters (padded): W1[kin][H], b1[H], W2[H][H], b2[H], W3[H][kout], b3[kout]
     float *W1 = nullptr, *b1 = nullptr, *W2 = nullptr, *b2 = nullptr, *W3 = nullptr, *b3 = nullptr;
-    uint16_t *W1b = nullptr, *W2b = nullptr, *W3b = nullptr;
+    uint16_t* Wb = nullptr;   // BF16 copy of the parameter array (same layout)
+    uint16_t *W1b = nullptr, *W2b = nullptr, *W3b = nullptr;   // its weight blocks
     // gradients and Adam moments, same shapes
     float *gW1 = nullptr, *gb1 = nullptr, *gW2 = nullptr, *gb2 = nullptr, *gW3 = nullptr, *gb3 = nullptr;
     float* mom = nullptr;     // all parameters: first moment
@@ -313,41 +322,60 @@ int ensure_rows(dpd_mlp* m, int rows)
     return 0;
 }
 
-// tile shapes: 64 x 64 (four wavefronts of 2 x 2 fragments), 256 x 16 for N = 16 outputs,
-// 16 x 256 for M = 16; K split over workgroups when the tiles alone would leave most CUs idle
+// the partial-sum buffer (split-K tiles, column-sum slabs), grown to `need` floats
+int ensure_part(dpd_mlp* m, int64_t need)
+{
+    if (need <= m->part_cap) return 0;
+    int rc;
+    if ((rc = malloc_dev(m, &m->part, need))) return rc;
+    m->part_cap = need;
+    return 0;
+}
+
+// db[n] = sum over the M rows of dZ[m][n] (N columns), in slabs of kColSlab rows
+int colsum(dpd_mlp* m, int M, int N, const float* dZ, float* db)
+{
+    const int nslab = (M + kColSlab - 1) / kColSlab;
+    int rc;
+    if ((rc = ensure_part(m, (int64_t)nslab * N))) return rc;
+    hipLaunchKernelGGL(k_colsum, dim3(blocks(N, 64), nslab), dim3(1024), 0, m->stream, M, N, dZ, m->part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(blocks(N)), dim3(256), 0, m->stream, 1, N, nslab, m->part, (int64_t)N,
+                       db, N);
+    return 0;
+}
+
+// tile shapes: 64 x 64 (four wavefronts of 2 x 2 fragments), 64 x 16 for N = 16 outputs,
+// 16 x 64 for M = 16 (a fragment per wavefront); K split over workgroups (at least 64 each)
+// when the tiles alone would leave CUs idle
 int gemm(dpd_mlp* m, int M, int N, int K, const uint16_t* A, int am, int ak, const uint16_t* B, int bk, int bn,
          float* C, int ldc)
 {
     if ((am != 1 && ak != 1) || (bk != 1 && bn != 1) || M % 16 || N % 16 || K % 16)
         return mfail(m, "gemm: operands must be contiguous along one dimension, sizes multiples of 16");
     int TM = 64, TN = 64;
-    if (N < 64) { TM = 256; TN = 16; }
-    else if (M < 64) { TM = 16; TN = 256; }
+    if (N < 64) TN = 16;
+    else if (M < 64) TM = 16;
     const int gx = (N + TN - 1) / TN, gy = (M + TM - 1) / TM;
     int nz = 1;
-    while (gx * gy * nz < 128 && K / (2 * nz) >= 4 * kKC) nz *= 2;
+    while (gx * gy * nz < 256 && K / (2 * nz) >= 2 * kKC) nz *= 2;
     const int kper = (K / 16 + nz - 1) / nz * 16;
     nz = (K + kper - 1) / kper;
     float* out = C;
     int ldo = ldc;
     int64_t cstride = 0;
     if (nz > 1) {
-        const int64_t need = (int64_t)nz * M * N;
-        if (need > m->part_cap) {
-            int rc;
-            if ((rc = malloc_dev(m, &m->part, need))) return rc;
-            m->part_cap = need;
-        }
+        int rc;
+        if ((rc = ensure_part(m, (int64_t)nz * M * N))) return rc;
         out = m->part;
         ldo = N;
         cstride = (int64_t)M * N;
     }
     const dim3 grid(gx, gy, nz);
     if (TN == 16)
-        hipLaunchKernelGGL((k_gemm_tile<4, 1, 4, 1>), grid, dim3(256), 0, m->stream, M, N, K, kper, A, am, ak, B, bk,
+        hipLaunchKernelGGL((k_gemm_tile<1, 1, 4, 1>), grid, dim3(256), 0, m->stream, M, N, K, kper, A, am, ak, B, bk,
                            bn, out, ldo, cstride);
     else if (TM == 16)
-        hipLaunchKernelGGL((k_gemm_tile<1, 4, 1, 4>), grid, dim3(256), 0, m->stream, M, N, K, kper, A, am, ak, B, bk,
+        hipLaunchKernelGGL((k_gemm_tile<1, 1, 1, 4>), grid, dim3(256), 0, m->stream, M, N, K, kper, A, am, ak, B, bk,
                            bn, out, ldo, cstride);
     else
         hipLaunchKernelGGL((k_gemm_tile<2, 2, 2, 2>), grid, dim3(256), 0, m->stream, M, N, K, kper, A, am, ak, B, bk,
@@ -398,30 +426,28 @@ int backward(dpd_mlp* m, int Mp)
     int rc;
     // dW3 = H2^T dY, db3 = colsum dY
     if ((rc = gemm(m, H, ko, Mp, m->H2b, 1, H, m->dYb, ko, 1, m->gW3, ko))) return rc;
-    hipLaunchKernelGGL(k_colsum, dim3(blocks(ko, 64)), dim3(1024), 0, m->stream, Mp, ko, m->dY, m->gb3);
+    if ((rc = colsum(m, Mp, ko, m->dY, m->gb3))) return rc;
     // dH2 = dY W3^T
     if ((rc = gemm(m, Mp, H, ko, m->dYb, ko, 1, m->W3b, 1, ko, m->dH2, H))) return rc;
     hipLaunchKernelGGL(k_tanh_grad, dim3(blocks((int64_t)Mp * H)), dim3(256), 0, m->stream, Mp * H, m->dH2, m->H2,
                        m->dZ2, m->dZ2b);
     // dW2 = H1^T dZ2, db2
     if ((rc = gemm(m, H, H, Mp, m->H1b, 1, H, m->dZ2b, H, 1, m->gW2, H))) return rc;
-    hipLaunchKernelGGL(k_colsum, dim3(blocks(H, 64)), dim3(1024), 0, m->stream, Mp, H, m->dZ2, m->gb2);
+    if ((rc = colsum(m, Mp, H, m->dZ2, m->gb2))) return rc;
     // dH1 = dZ2 W2^T
     if ((rc = gemm(m, Mp, H, H, m->dZ2b, H, 1, m->W2b, 1, H, m->dH1, H))) return rc;
     hipLaunchKernelGGL(k_tanh_grad, dim3(blocks((int64_t)Mp * H)), dim3(256), 0, m->stream, Mp * H, m->dH1, m->H1,
                        m->dZ1, m->dZ1b);
     // dW1 = X^T dZ1, db1
     if ((rc = gemm(m, ki, H, Mp, m->Xb, 1, ki, m->dZ1b, H, 1, m->gW1, H))) return rc;
-    hipLaunchKernelGGL(k_colsum, dim3(blocks(H, 64)), dim3(1024), 0, m->stream, Mp, H, m->dZ1, m->gb1);
+    if ((rc = colsum(m, Mp, H, m->dZ1, m->gb1))) return rc;
     return 0;
 }
 
 void refresh_bf16(dpd_mlp* m)
 {
     const Layout L = layout(m);
-    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(L.b1 - L.w1)), dim3(256), 0, m->stream, (int)(L.b1 - L.w1), m->W1, m->W1b);
-    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(L.b2 - L.w2)), dim3(256), 0, m->stream, (int)(L.b2 - L.w2), m->W2, m->W2b);
-    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(L.b3 - L.w3)), dim3(256), 0, m->stream, (int)(L.b3 - L.w3), m->W3, m->W3b);
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(L.total)), dim3(256), 0, m->stream, (int)L.total, m->W1, m->Wb);
 }
 
 // copy between the padded device layout and the compact parameter order of the ABI
@@ -469,9 +495,8 @@ int dpd_mlp_create(int32_t nin, int32_t hidden, int32_t nout, uint64_t seed, int
     if ((rc = malloc_dev(m, &grads, L.total))) return rc;
     m->gW1 = grads + L.w1; m->gb1 = grads + L.b1; m->gW2 = grads + L.w2;
     m->gb2 = grads + L.b2; m->gW3 = grads + L.w3; m->gb3 = grads + L.b3;
-    if ((rc = malloc_dev(m, &m->W1b, L.b1 - L.w1)) || (rc = malloc_dev(m, &m->W2b, L.b2 - L.w2)) ||
-        (rc = malloc_dev(m, &m->W3b, L.b3 - L.w3)))
-        return rc;
+    if ((rc = malloc_dev(m, &m->Wb, L.total))) return rc;
+    m->W1b = m->Wb + L.w1; m->W2b = m->Wb + L.w2; m->W3b = m->Wb + L.w3;
     // Glorot-uniform weights, zero biases, padding masked out
     std::mt19937_64 rng(seed);
     std::vector<float> w(L.total, 0.f);
@@ -504,7 +529,7 @@ void dpd_mlp_destroy(dpd_mlp* m)
 {
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    void* ptrs[] = {m->W1, m->gW1, m->mom, m->var, m->mask, m->W1b, m->W2b, m->W3b, m->Xb, m->H1b, m->H2b, m->dYb,
+    void* ptrs[] = {m->W1, m->gW1, m->mom, m->var, m->mask, m->Wb, m->Xb, m->H1b, m->H2b, m->dYb,
                     m->dZ2b, m->dZ1b, m->Z1, m->H1, m->Z2, m->H2, m->Z3, m->Y, m->T, m->dY, m->dH2, m->dZ2,
                     m->dH1, m->dZ1, m->loss, m->part};
     for (void* p : ptrs)
@@ -608,8 +633,7 @@ int dpd_mlp_train(dpd_mlp* m, int64_t n, const double* x, const double* y, int32
         const float c1 = 1.0f - std::pow(0.9f, (float)m->adam_t);
         const float c2 = 1.0f - std::pow(0.999f, (float)m->adam_t);
         hipLaunchKernelGGL(k_adam, dim3(blocks(tot)), dim3(256), 0, m->stream, (int)tot, m->W1, m->gW1, m->mom,
-                           m->var, m->mask, (float)lr, c1, c2);
-        refresh_bf16(m);
+                           m->var, m->mask, (float)lr, c1, c2, m->Wb);
     }
     MCHK(hipGetLastError());
     if (loss_out && iters > 0)
