@@ -329,8 +329,8 @@ __global__ __launch_bounds__(256) void k_advance(int n, Buffers S, float4* __res
 // Exclusive scan of in[0, m) into out, in[] cleared, by one workgroup of 1024 threads: in
 // rounds of kScanRound entries, wavefront w takes up to kScanChunks consecutive chunks of 64
 // entries, loads all of them at once (one coalesced load per chunk, all in flight), scans each
-// chunk with shuffles in order into LDS, and the 16 wavefront totals are added in order through
-// LDS.  ex: LDS of kScanRound ints.
+// chunk with DPP (wave_incl_scan) in order into LDS, and the 16 wavefront totals are added in
+// order through LDS.  ex: LDS of kScanRound ints.
 constexpr int kScanChunks = 8;
 constexpr int kScanRound = 16 * 64 * kScanChunks;
 __device__ inline void scan_block(int* in, int* __restrict__ out, int m, int* ex, int* wsum, int* carry)
@@ -351,14 +351,9 @@ __device__ inline void scan_block(int* in, int* __restrict__ out, int m, int* ex
         int run = 0;
 #pragma unroll
         for (int j = 0; j < kScanChunks; ++j) {
-            int incl = v[j];
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int y = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += y;
-            }
+            const int incl = wave_incl_scan(v[j]);
             if (j < per) ex[((ch0 + j) << 6) + lane] = run + incl - v[j];
-            run += __shfl(incl, 63, 64);
+            run += __builtin_amdgcn_readlane(incl, 63);
         }
         if (lane == 0) wsum[wave] = run;
         __syncthreads();
@@ -420,12 +415,7 @@ __global__ __launch_bounds__(256) void k_cell_start(int ngroups, int n, const in
     if (g >= ngroups) return;
     const int k = g * 64 + lane;
     const int c = cnt[k];
-    int incl = c;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
-    }
+    const int incl = wave_incl_scan(c);
     cstart[k] = gstart[g] + incl - c;
     cnt[k] = 0;
     if (g == ngroups - 1 && lane == 63) cstart[ngroups * 64] = n;

@@ -250,6 +250,21 @@ __device__ inline int lane_prefix(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
 }
 
+// Inclusive prefix sum over the 64 lanes of a wavefront with DPP moves: row_shr 1, 2, 4, 8
+// scan each row of 16 lanes (lanes shifted in from outside the row read 0), row_bcast 15
+// adds row 0's total to row 1 and row 2's to row 3, row_bcast 31 adds the first half's total
+// to rows 2 and 3.  Six dependent adds instead of six LDS permutes.
+__device__ inline int wave_incl_scan(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    return x;
+}
+
 // Active lanes of a wavefront holding equal keys form a group: this lane's rank in its
 // group, the group's size and its lowest lane.  Must be reached by the whole wavefront.
 // One pass per distinct key, with wave-uniform bookkeeping; a wavefront of cell-sorted
