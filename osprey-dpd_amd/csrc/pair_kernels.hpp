@@ -339,6 +339,9 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
         const float tu = (1.0f + m) * fmaxf(inv_ux, fmaxf(inv_uy, inv_uz));
         const uint32_t thr = (uint32_t)(tu * tu) + 1u;
         const char* cbase = reinterpret_cast<const char*>(A.code);
+        // a row wraps through the periodic x face only for beads in the first or last cell of
+        // their row: wavefronts without such a bead scan one segment per row
+        const int nseg = __ballot(c[0] == 0 || c[0] == B.n[0] - 1) != 0 ? 2 : 1;
         // the neighbour rows of this lane: all nine, one z layer (SPLIT 3) or one row (SPLIT 9)
         const int dz0 = SPLIT == 1 ? -1 : (SPLIT == 3 ? part - 1 : part / 3 - 1);
         const int dz1 = SPLIT == 1 ? 1 : dz0;
@@ -361,7 +364,7 @@ __device__ __attribute__((always_inline)) inline void hitlist_body(const PairArg
                 const int Qy = (int)floorf((pi.y - (float)dy * B.w[1]) * inv_uy);
                 const int Qz = (int)floorf((pi.z - (float)dz * B.w[2]) * inv_uz);
                 const int cx = c[0], nx = B.n[0];
-                for (int sgm = 0; sgm < 2; ++sgm) {
+                for (int sgm = 0; sgm < nseg; ++sgm) {
                     int fx, lx, fu;
                     if (cx >= 1 && cx <= nx - 2) { fx = cx - 1; lx = sgm ? cx - 2 : cx + 1; fu = -1; }
                     else if (cx == 0) { fx = sgm ? 0 : nx - 1; lx = sgm ? 1 : nx - 1; fu = sgm ? 0 : -1; }
