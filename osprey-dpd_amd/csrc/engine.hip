@@ -43,15 +43,15 @@ static_assert(O_VIR == kObsVir && O_PEP == kObsPEP, "accumulator layout");
 
 struct Topo {
     const int* bond_ptr;      // [n+1] bond entries of each bead index (all boxes)
-    const int2* bond_ent;     // (bond, 1 if the bead is the bond's head)
+    const int2* bond_csr;     // per bond: the entries of its head and of its tail
     const int2* bond_bead;    // (head, tail) bead index per bond
     const double2* bond_par;  // (k, l0) per bond
     const int* ang_ptr;       // [n+1] angle entries of each bead index
-    const int2* ang_ent;      // (angle, role)
+    const int4* ang_csr;      // per angle: the entries of roles 0 (c), 1 (b), 2 (a)
     const int4* ang_bead;     // (a, b, c, 0) per angle
     const double4* ang_par;   // (k, phi0, cos phi0, sin phi0)
-    double4* fbond;           // per bond: force on the head (the tail takes minus)
-    double4* fang;            // per angle: force on roles 0 (c), 1 (b), 2 (a)
+    double4* fbond;           // per bond entry: the bond's force on that entry's bead
+    double4* fang;            // per angle entry: the angle's force on that entry's bead
     const int* bonded;        // bead indices with at least one bond or angle
     int nbonds, nangles, nbonded;
     int has_bonds, has_angles;
@@ -91,16 +91,13 @@ __device__ inline void bonded_gather(int q, const Topo& T, double f[3])
     f[0] = f[1] = f[2] = 0.0;
     if (T.has_bonds) {
         for (int e = T.bond_ptr[q], e1 = T.bond_ptr[q + 1]; e < e1; ++e) {
-            const int2 ent = T.bond_ent[e];
-            const double4 fb = T.fbond[ent.x];
-            const double sg = ent.y ? 1.0 : -1.0;
-            f[0] += sg * fb.x; f[1] += sg * fb.y; f[2] += sg * fb.z;
+            const double4 fb = T.fbond[e];
+            f[0] += fb.x; f[1] += fb.y; f[2] += fb.z;
         }
     }
     if (T.has_angles) {
         for (int e = T.ang_ptr[q], e1 = T.ang_ptr[q + 1]; e < e1; ++e) {
-            const int2 ent = T.ang_ent[e];
-            const double4 fa = T.fang[3 * ent.x + ent.y];
+            const double4 fa = T.fang[e];
             f[0] += fa.x; f[1] += fa.y; f[2] += fa.z;
         }
     }
@@ -138,7 +135,11 @@ __device__ __attribute__((always_inline)) inline void bonded_body(const Buffers&
         const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         const double s = par.x * (par.y - L) / L;
         const double fb[3] = {s * d[0], s * d[1], s * d[2]};
-        T.fbond[q] = make_double4(fb[0], fb[1], fb[2], 0.0);
+        // straight into the two beads' entries (negation is exact: the sums are those of
+        // one force per bond added with its sign)
+        const int2 ent = T.bond_csr[q];
+        T.fbond[ent.x] = make_double4(fb[0], fb[1], fb[2], 0.0);
+        T.fbond[ent.y] = make_double4(-fb[0], -fb[1], -fb[2], 0.0);
         if (OBS) {
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) ob[3 * a + b] = d[b] * fb[a];
@@ -156,7 +157,10 @@ __device__ __attribute__((always_inline)) inline void bonded_body(const Buffers&
         unwrapped_pos(sb, abc.y, S, B, ub);
         unwrapped_pos(sc, abc.z, S, B, uc);
         angle_forces(ua, ub, uc, T.ang_par[g], f, OBS ? &pe : nullptr);
-        for (int r = 0; r < 3; ++r) T.fang[3 * g + r] = make_double4(f[r][0], f[r][1], f[r][2], 0.0);
+        const int4 ent = T.ang_csr[g];
+        T.fang[ent.x] = make_double4(f[0][0], f[0][1], f[0][2], 0.0);
+        T.fang[ent.y] = make_double4(f[1][0], f[1][1], f[1][2], 0.0);
+        T.fang[ent.z] = make_double4(f[2][0], f[2][1], f[2][2], 0.0);
         if (OBS) ob[11] = pe;
     }
     if (OBS) {
@@ -687,7 +691,8 @@ struct dpd_engine {
     size_t slice_cap = 0;
     // topology
     int *bond_ptr = nullptr, *ang_ptr = nullptr;
-    int2 *bond_ent = nullptr, *bond_bead = nullptr, *ang_ent = nullptr;
+    int2 *bond_csr = nullptr, *bond_bead = nullptr;
+    int4* ang_csr = nullptr;
     double2* bond_par = nullptr;
     int4* ang_bead = nullptr;
     double4 *ang_par = nullptr, *fbond = nullptr, *fang = nullptr;
@@ -827,7 +832,7 @@ Buffers bufs(dpd_engine* e, int i)
 
 Topo topo(dpd_engine* e)
 {
-    return Topo{e->bond_ptr, e->bond_ent, e->bond_bead, e->bond_par, e->ang_ptr, e->ang_ent, e->ang_bead,
+    return Topo{e->bond_ptr, e->bond_csr, e->bond_bead, e->bond_par, e->ang_ptr, e->ang_csr, e->ang_bead,
                 e->ang_par, e->fbond, e->fang, e->bonded, e->nbonds, e->nangles, e->nbonded, e->has_bonds,
                 e->has_angles};
 }
@@ -1203,16 +1208,19 @@ int commit_topology(dpd_engine* e)
         ap[a + 1]++; ap[bm + 1]++; ap[c + 1]++;
     }
     for (int64_t i = 0; i < n; ++i) { bp[i + 1] += bp[i]; ap[i + 1] += ap[i]; }
-    std::vector<int2> bent(std::max<int>(bp[n], 1)), bbead(std::max<int64_t>(nbonds, 1));
+    // each bead's entries in bond order (then angle order): the bonded kernels write every
+    // bond's and angle's forces into its beads' entries, and a bead's sum reads its entries
+    std::vector<int2> bcsr(std::max<int64_t>(nbonds, 1)), bbead(std::max<int64_t>(nbonds, 1));
     std::vector<double2> bpar(std::max<int64_t>(nbonds, 1));
-    std::vector<int2> aent(std::max<int>(ap[n], 1));
+    std::vector<int4> acsr(std::max<int64_t>(npairs, 1));
     std::vector<int> fill(bp.begin(), bp.end() - 1);
     for (int64_t b = 0; b < nbonds; ++b) {
         const int h = H[b], t = Tl[b];
         bbead[b] = make_int2(h, t);
         bpar[b] = make_double2(K[b], L0[b]);
-        bent[fill[h]++] = make_int2((int)b, 1);
-        bent[fill[t]++] = make_int2((int)b, 0);
+        const int eh = fill[h]++;
+        const int et = fill[t]++;
+        bcsr[b] = make_int2(eh, et);
     }
     std::vector<int> bonded;
     for (int64_t i = 0; i < n; ++i)
@@ -1220,27 +1228,29 @@ int commit_topology(dpd_engine* e)
     std::vector<int> afill(ap.begin(), ap.end() - 1);
     for (int64_t q = 0; q < npairs; ++q) {
         const int4 abc = abead[q];
-        aent[afill[abc.z]++] = make_int2((int)q, 0);
-        aent[afill[abc.y]++] = make_int2((int)q, 1);
-        aent[afill[abc.x]++] = make_int2((int)q, 2);
+        const int ec = afill[abc.z]++;
+        const int eb = afill[abc.y]++;
+        const int ea = afill[abc.x]++;
+        acsr[q] = make_int4(ec, eb, ea, 0);
     }
     drop_graphs(e);
     int rc;
     if ((rc = dalloc(e, &e->bond_ptr, n + 1)) || (rc = dalloc(e, &e->ang_ptr, n + 1)) ||
-        (rc = dalloc(e, &e->bond_ent, bent.size())) || (rc = dalloc(e, &e->bond_par, bpar.size())) ||
-        (rc = dalloc(e, &e->bond_bead, bbead.size())) || (rc = dalloc(e, &e->fbond, bbead.size())) ||
-        (rc = dalloc(e, &e->fang, 3 * abead.size())) || (rc = dalloc(e, &e->bonded, bonded.size())) ||
-        (rc = dalloc(e, &e->ang_ent, aent.size())) || (rc = dalloc(e, &e->ang_bead, abead.size())) ||
+        (rc = dalloc(e, &e->bond_csr, bcsr.size())) || (rc = dalloc(e, &e->bond_par, bpar.size())) ||
+        (rc = dalloc(e, &e->bond_bead, bbead.size())) ||
+        (rc = dalloc(e, &e->fbond, std::max<int64_t>(bp[n], 1))) ||
+        (rc = dalloc(e, &e->fang, std::max<int64_t>(ap[n], 1))) || (rc = dalloc(e, &e->bonded, bonded.size())) ||
+        (rc = dalloc(e, &e->ang_csr, acsr.size())) || (rc = dalloc(e, &e->ang_bead, abead.size())) ||
         (rc = dalloc(e, &e->ang_par, apar.size())))
         return rc;
     HIPCHK(hipMemcpyAsync(e->bond_ptr, bp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_ptr, ap.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemcpyAsync(e->bond_ent, bent.data(), sizeof(int2) * bent.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->bond_csr, bcsr.data(), sizeof(int2) * bcsr.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->bond_bead, bbead.data(), sizeof(int2) * bbead.size(), hipMemcpyHostToDevice, e->stream));
     if (!bonded.empty())
         HIPCHK(hipMemcpyAsync(e->bonded, bonded.data(), sizeof(int) * bonded.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->bond_par, bpar.data(), sizeof(double2) * bpar.size(), hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemcpyAsync(e->ang_ent, aent.data(), sizeof(int2) * aent.size(), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->ang_csr, acsr.data(), sizeof(int4) * acsr.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_bead, abead.data(), sizeof(int4) * abead.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->ang_par, apar.data(), sizeof(double4) * apar.size(), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -1454,7 +1464,7 @@ void dpd_destroy(dpd_engine* e)
                     e->key[0], e->key[1], e->code[0], e->code[1], e->frc, e->vp, e->slot_of, e->nkey, e->rank,
                     e->place, e->cnt, e->gcnt,
                     e->gstart, e->cstart, e->flags, e->dstep, e->acc, e->table, e->gather, e->bond_ptr,
-                    e->ang_ptr, e->bond_ent, e->bond_par, e->ang_ent, e->ang_bead, e->ang_par, e->bond_bead,
+                    e->ang_ptr, e->bond_csr, e->bond_par, e->ang_csr, e->ang_bead, e->ang_par, e->bond_bead,
                     e->fbond, e->fang, e->bonded, e->dboxes, e->slice, e->madj};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
