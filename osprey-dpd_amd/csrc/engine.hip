@@ -447,7 +447,8 @@ template <int MODE, bool MULTI>
 __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, float4* __restrict__ frc,
                                                  float4* __restrict__ vp, const BoxDev* boxes, BoxDev B0,
                                                  const int* __restrict__ nkey, const int2* __restrict__ tmp,
-                                                 const int* __restrict__ cstart, int* __restrict__ slot_of)
+                                                 const int* __restrict__ cstart, int* __restrict__ slot_of,
+                                                 int write_vp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -501,7 +502,9 @@ __global__ __launch_bounds__(256) void k_permute(int n, Buffers S, Buffers D, fl
     if (MODE == kRebin) {
         vp[dst] = v;
     } else {
-        vp[dst] = make_float4(kick(v.x, f.x, B.lamdt), kick(v.y, f.y, B.lamdt), kick(v.z, f.z, B.lamdt), v.w);
+        // (equal to the new velocity's bits when lambda dt = dt/2: the steps then read that)
+        if (write_vp)
+            vp[dst] = make_float4(kick(v.x, f.x, B.lamdt), kick(v.y, f.y, B.lamdt), kick(v.z, f.z, B.lamdt), v.w);
     }
     // the slot of each bead is read for bonded beads only (bond and angle terms); a scattered
     // 4-byte store per bead costs a partial-line write each, so the others are skipped once
@@ -522,7 +525,7 @@ __global__ void k_copy_force(int n, const float4* __restrict__ fold, float4* __r
 template <bool UPDATE_V, bool OBS, bool MULTI>
 __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, const float4* __restrict__ frc,
                                                 const BoxDev* boxes, BoxDev B0, int* __restrict__ flags,
-                                                double* __restrict__ acc)
+                                                double* __restrict__ acc, float4* __restrict__ vp_keep)
 {
     static_assert(O_V2 == 0 && O_MOM == 3, "kinetic slots first");
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -537,6 +540,8 @@ __global__ __launch_bounds__(256) void k_finish(int n, Buffers S, const float4* 
         const float4 f = frc[s];
         float4 v = S.vel[s];
         if (UPDATE_V) {
+            // the last step's predicted velocity, which the steps read from vel (vp_is_vel)
+            if (vp_keep) vp_keep[s] = v;
             // the second half of UpdateMom (the stored velocity is v + dt/2 F_old)
             v.x = kick(v.x, f.x, B.hdt);
             v.y = kick(v.y, f.y, B.hdt);
@@ -739,6 +744,17 @@ int dalloc(dpd_engine* e, T** p, size_t count)
     return DPD_OK;
 }
 
+// With lambda = 1/2 and the forces on, the predicted velocity v + lambda dt f and the stored
+// half step v + dt/2 f are the same FP32 expression (lambda dt and dt/2 round to the same
+// float): steps then read the predicted velocities from vel, k_permute does not write vp, and
+// k_finish keeps the last step's in vp for the calls between steps.
+inline bool vp_is_vel(const dpd_engine* e)
+{
+    for (const auto& b : e->boxes)
+        if (!(b.B.hdt == b.B.lamdt)) return false;
+    return true;
+}
+
 void drop_graphs(dpd_engine* e)
 {
     for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.second);
@@ -864,7 +880,8 @@ void sort_step_t(dpd_engine* e, bool advanced)
     hipLaunchKernelGGL(k_place, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S.bead, e->nkey, e->rank,
                        e->cstart, e->place);
     hipLaunchKernelGGL((k_permute<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, D, e->frc,
-                       e->vp, e->dboxes, B0, e->nkey, e->place, e->cstart, e->slot_of);
+                       e->vp, e->dboxes, B0, e->nkey, e->place, e->cstart, e->slot_of,
+                       (MODE == kRebin || !vp_is_vel(e)) ? 1 : 0);
     e->cur ^= 1;
 }
 
@@ -887,7 +904,7 @@ inline bool epilogue_ok(const dpd_engine* e) { return e->nbonded == 0 && epilogu
 // Pair forces of this step.  With the hit-list kernel the bond and angle terms run in the
 // same launch (k_pair_hitlist_bonded); `bonded` then tells launch_bonded to add only the sums.
 // `epi` (epilogue_ok): the launch also closes the step and starts the next one (pair_epilogue).
-int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr, bool epi = false)
+int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr, bool epi = false, bool in_seq = false)
 {
     if (bonded) *bonded = false;
     PairArgs a;
@@ -905,7 +922,9 @@ int launch_pair(dpd_engine* e, bool obs, bool* bonded = nullptr, bool epi = fals
     a.ngroups = e->ngroups;
     a.boxes = e->dboxes;
     a.pos = e->pos[e->cur];
-    a.vp = e->vp;
+    // inside a sequence of steps with lambda dt = dt/2 the predicted velocities are the
+    // stored half steps (k_permute did not write vp)
+    a.vp = (in_seq && vp_is_vel(e)) ? e->vel[e->cur] : e->vp;
     a.key = e->key[e->cur];
     a.code = e->code[e->cur];
     a.cstart = e->cstart;
@@ -938,12 +957,13 @@ void launch_finish(dpd_engine* e)
 {
     const int n = (int)e->n;
     const BoxDev B0 = e->boxes[0].B;
+    float4* keep = (UPDATE_V && vp_is_vel(e)) ? e->vp : nullptr;
     if (multi(e))
         hipLaunchKernelGGL((k_finish<UPDATE_V, OBS, true>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n,
-                           bufs(e, e->cur), e->frc, e->dboxes, B0, e->flags, e->acc);
+                           bufs(e, e->cur), e->frc, e->dboxes, B0, e->flags, e->acc, keep);
     else
         hipLaunchKernelGGL((k_finish<UPDATE_V, OBS, false>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n,
-                           bufs(e, e->cur), e->frc, e->dboxes, B0, e->flags, e->acc);
+                           bufs(e, e->cur), e->frc, e->dboxes, B0, e->flags, e->acc, keep);
 }
 
 // bonds and angles of this step (unless the pair launch ran them) and each bonded bead's
@@ -1044,7 +1064,7 @@ int enqueue_steps(dpd_engine* e, int64_t nsteps, bool obs, int timing)
         const bool timed = timed_step(t, timing);
         if (timed) HIPCHK(hipEventRecord(e->pev[2 * ti], e->stream));
         bool terms_done = false;
-        if ((rc = launch_pair(e, obs && last, &terms_done, epi))) return rc;
+        if ((rc = launch_pair(e, obs && last, &terms_done, epi, true))) return rc;
         if (timed) HIPCHK(hipEventRecord(e->pev[2 * ti++ + 1], e->stream));
         // the bonded sum of every step but the last is added by the next step's k_advance
         if (obs && last) launch_bonded<true>(e, terms_done);
