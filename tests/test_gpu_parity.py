@@ -159,17 +159,20 @@ def test_golden_noise_free_step(kernel, case):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
-def test_ten_steps_track_oracle(kernel):
+@pytest.mark.parametrize("lam", [0.5, 0.65])
+def test_ten_steps_track_oracle(kernel, lam):
+    """Ten steps with lambda = 1/2 (the steps read the predicted velocities from the stored
+    half steps) and with another lambda (the predicted velocities stored apart)."""
     cells = (10, 10, 10)
     n = 3000
     ids, t, x, v = random_state(cells, n, 2, seed=21)
-    box = O.Box(cells, BINARY_A, BINARY_G, seed=5)
+    box = O.Box(cells, BINARY_A, BINARY_G, seed=5, lam=lam)
     # relax the random start so forces are moderate
     s = O.System(box, ids, t, x, v)
     s.run(30)
     x, u, v, f = s.state()
-    say("ten steps", kernel)
-    e = D.Engine(cells, BINARY_A, BINARY_G, seed=5, pair_kernel=kernel)
+    say("ten steps", kernel, "lambda", lam)
+    e = D.Engine(cells, BINARY_A, BINARY_G, seed=5, lam=lam, pair_kernel=kernel)
     e.upload(ids, t, x, v, f=f, unpbc=u, step0=30)
     e.step(10, observe=True)
     X, U, V, F = e.download()
