@@ -425,14 +425,50 @@ __global__ __launch_bounds__(256) void k_cell_start(int ngroups, int n, const in
     if (g == ngroups - 1 && lane == 63) cstart[ngroups * 64] = n;
 }
 
+// Engines of up to kDirectGroups scan groups: the cell starts in one launch without a scan of
+// the group counts.  Each workgroup (four groups, a wavefront each) sums the counts of every
+// group before its first itself, from L2 (at most kDirectGroups ints per workgroup, 8 MB
+// in all for 4096 groups), then each wavefront scans its 64 cell counts as k_cell_start does.
+// Clears the cell counts; the group counts stay for the other workgroups and k_place clears
+// them.  `step` (may be null): as for k_group_scan.
+constexpr int kDirectGroups = 8192;
+__global__ __launch_bounds__(256) void k_cell_start_direct(int ngroups, int n, const int* __restrict__ gcnt,
+                                                           int* __restrict__ cnt, int* __restrict__ cstart,
+                                                           int64_t* __restrict__ step)
+{
+    __shared__ int wsum[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int g0 = blockIdx.x * 4;
+    if (step && blockIdx.x == 0 && t == 0) *step += 1;
+    int p = 0;
+#pragma unroll 4
+    for (int i = t; i < g0; i += 256) p += gcnt[i];
+    p = __builtin_amdgcn_readlane(wave_incl_scan(p), 63);
+    if (lane == 0) wsum[w] = p;
+    __syncthreads();
+    const int g = g0 + w;
+    if (g >= ngroups) return;
+    int start = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    for (int j = g0; j < g; ++j) start += gcnt[j];
+    const int k = g * 64 + lane;
+    const int c = cnt[k];
+    const int incl = wave_incl_scan(c);
+    cstart[k] = start + incl - c;
+    cnt[k] = 0;
+    if (g == ngroups - 1 && lane == 63) cstart[ngroups * 64] = n;
+}
+
 // Counting-sort placement, first half: every source slot s writes (bead index, s) to the
 // position its rank gives it inside its new cell.  Ranks come from atomics, so the order
-// inside a cell is arbitrary here; k_permute restores a deterministic one.
+// inside a cell is arbitrary here; k_permute restores a deterministic one.  gclear (may be
+// null): the group counts k_cell_start_direct left, cleared by threads s < ngroups (the grid
+// covers max(n, ngroups) threads).
 __global__ __launch_bounds__(256) void k_place(int n, const int* __restrict__ bead, const int* __restrict__ nkey,
                                                const int* __restrict__ rank, const int* __restrict__ cstart,
-                                               int2* __restrict__ tmp)
+                                               int2* __restrict__ tmp, int* __restrict__ gclear, int ngroups)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gclear && s < ngroups) gclear[s] = 0;
     if (s >= n) return;
     tmp[cstart[nkey[s]] + rank[s]] = make_int2(bead[s], s);
 }
@@ -869,16 +905,21 @@ void sort_step_t(dpd_engine* e, bool advanced)
         hipLaunchKernelGGL((k_advance<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, e->frc,
                            topo(e), e->dboxes, B0, e->cnt, e->gcnt, e->nkey, e->rank, e->flags, e->dstep);
     int64_t* bump = advanced ? e->dstep : nullptr;
+    int* gclear = nullptr;
     if (e->nkeys <= kSmallScanKeys) {
         hipLaunchKernelGGL(k_cell_scan_small, dim3(1), dim3(1024), 0, e->stream, e->nkeys, e->ngroups, n, e->cnt,
                            e->gcnt, e->cstart, bump);
+    } else if (e->ngroups <= kDirectGroups) {
+        hipLaunchKernelGGL(k_cell_start_direct, dim3(grid(e->ngroups, 4)), dim3(256), 0, e->stream, e->ngroups, n,
+                           e->gcnt, e->cnt, e->cstart, bump);
+        gclear = e->gcnt;
     } else {
         hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e->stream, e->ngroups, e->gcnt, e->gstart, bump);
         hipLaunchKernelGGL(k_cell_start, dim3(grid(e->ngroups, 4)), dim3(256), 0, e->stream, e->ngroups, n, e->gstart,
                            e->cnt, e->cstart);
     }
-    hipLaunchKernelGGL(k_place, dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S.bead, e->nkey, e->rank,
-                       e->cstart, e->place);
+    hipLaunchKernelGGL(k_place, dim3(grid(std::max<int64_t>(n, gclear ? e->ngroups : 0), 256)), dim3(256), 0,
+                       e->stream, n, S.bead, e->nkey, e->rank, e->cstart, e->place, gclear, (int)e->ngroups);
     hipLaunchKernelGGL((k_permute<MODE, MULTI>), dim3(grid(n, 256)), dim3(256), 0, e->stream, n, S, D, e->frc,
                        e->vp, e->dboxes, B0, e->nkey, e->place, e->cstart, e->slot_of,
                        (MODE == kRebin || !vp_is_vel(e)) ? 1 : 0);

@@ -69,6 +69,30 @@ def test_forces_condensate_box_full_size(kernel):
     assert np.abs(f.sum(axis=0)).max() / np.abs(f).sum() < 1e-6
 
 
+def test_forces_after_steps_large_box():
+    """84^3 cells (1,778,112 beads, 9,261 scan groups of 64 cell keys): above 8,192 groups the
+    counting sort scans the group counts in a launch of its own before the cell starts.  Three
+    steps, then the conservative forces of the state reached against the oracle's on the same
+    positions (after steps the engine's dissipative term reads the last predicted velocities,
+    which the download does not return)."""
+    cells = (84, 84, 84)
+    n = 3 * 84 ** 3
+    say("large box", cells)
+    ids, t, x, v = random_state(cells, n, 2, seed=13)
+    e = engine(cells, BINARY_A, BINARY_G, D.PAIR_HITLIST)
+    e.upload(ids, t, x, v)
+    e.step(3)
+    X, U, V, _ = e.download()
+    L = np.array(cells, dtype=float)
+    assert (X >= 0).all() and (X < L).all()
+    f, vir, _ = e.compute_forces(step=9, thermostat=False)
+    fr, virr, _ = O.forces(O.Box(cells, BINARY_A, BINARY_G, seed=1234), ids, t, X, V, step=9, thermostat=False)
+    norm, per_bead = force_errors(f, fr)
+    say(f"  normwise {norm:.2e} per-bead {per_bead:.2e}")
+    assert norm < FORCE_TOL
+    assert rel_norm(vir, virr) < FORCE_TOL
+
+
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_bonded_forces_match_oracle(kernel):
     cells = (12, 12, 12)
