@@ -69,15 +69,17 @@ def test_forces_condensate_box_full_size(kernel):
     assert np.abs(f.sum(axis=0)).max() / np.abs(f).sum() < 1e-6
 
 
-def test_forces_after_steps_large_box():
-    """84^3 cells (1,778,112 beads, 9,261 scan groups of 64 cell keys): above 8,192 groups the
-    counting sort scans the group counts in a launch of its own before the cell starts.  Three
-    steps, then the conservative forces of the state reached against the oracle's on the same
-    positions (after steps the engine's dissipative term reads the last predicted velocities,
-    which the download does not return)."""
-    cells = (84, 84, 84)
-    n = 3 * 84 ** 3
-    say("large box", cells)
+@pytest.mark.parametrize("cells", [(26, 26, 26), (84, 84, 84)], ids=["groups275", "groups9261"])
+def test_forces_after_steps_sort_paths(cells):
+    """The counting sort's two cell-start paths above 16,384 cell keys: 26^3 cells (52,728
+    beads, 275 scan groups of 64 keys, so the last workgroup of the one-launch path holds
+    three groups) and 84^3 cells (1,778,112 beads, 9,261 groups: above 8,192 groups the group
+    counts are scanned in a launch of their own).  Three steps, then the conservative forces
+    of the state reached against the oracle's on the same positions (after steps the engine's
+    dissipative term reads the last predicted velocities, which the download does not
+    return)."""
+    n = 3 * int(np.prod(cells))
+    say("sort paths", cells)
     ids, t, x, v = random_state(cells, n, 2, seed=13)
     e = engine(cells, BINARY_A, BINARY_G, D.PAIR_HITLIST)
     e.upload(ids, t, x, v)
