@@ -6,7 +6,8 @@
 //   k_advance   UpdateMom of the previous step fused with UpdatePos of this one
 //               (src/CNTCell.cpp:2280-2310, :1704-1757); computes each bead's new cell
 //               key and its rank in that cell
-//   k_group_scan / k_cell_start   counting-sort prefix sums (groups of 64 keys, then cells)
+//   cell starts  counting-sort prefix sums, one launch up to 8192 groups of 64 keys
+//               (k_cell_scan_small, k_cell_start_direct), else k_group_scan + k_cell_start
 //   k_place / k_permute   counting-sort placement into the new cell-sorted SoA buffers,
 //               beads of a cell in ascending bead index (deterministic layout)
 //   pair kernel DPD pair forces (src/CNTCell.cpp:1144-1555)
