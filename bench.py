@@ -401,8 +401,8 @@ def main():
         print(json.dumps({"mlp": bench_mlp()}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (multi-GPU "
-                                   "sharding is designed, not built)"}))
+        print(json.dumps({"error": "the engine runs on one GPU: --gpus must be 1 (sharding across "
+                                   "GPUs is designed, not built)"}))
         sys.exit(2)
 
     wl = W.condensate()

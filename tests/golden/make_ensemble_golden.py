@@ -21,14 +21,14 @@ Cases:
             while the mixture demixes.
   condensate20  the condensate composition on a 20^3 box (24,000 beads): 16-mers
             (S P P P)x4 at 20% of the beads in water, a_SS = 10, other a = 25, Hookean bonds
-            k 128 l0 0.5, 4000 steps from this build's random states (its restatement of
+            k 128 l0 0.5, 4000 steps from the package's random states (its restatement of
             the reference's random builder, src/RandomBuilder.cpp, handed over as State
             restart, as for the bilayer): kT, P, the sticker-sticker g(r) and the mean bond
             length from snapshots every 250 steps after step 2000.
 Each case runs NRUNS times from independent starting states (RNGSeed differs per run).
-The condensate runs start from this build's random states handed to the reference as
+The condensate runs start from the package's random states handed to the reference as
 restart snapshots (State restart); the water, binary and bilayer runs from the reference's
-own builders, whose states this build reproduces from the same seeds.
+own builders, whose states the package reproduces from the same seeds.
 Per run, kT and P are block-averaged over the second part of the run (water: after step
 1000, bilayer: after step 2000); the fixtures keep every run's mean so that comparisons
 use the run-to-run spread, which for a bilayer relaxing over a few thousand steps is
@@ -355,7 +355,7 @@ def restart_text(text, run_id):
 
 
 def own_start(work, run_id, text, seed):
-    """Write this build's starting state for `text` (seed) as the restart snapshot of
+    """Write the package's starting state for `text` (seed) as the restart snapshot of
     run_id at step 0, next to the control file the reference checks a restart against."""
     inp = H.Input(text=text)
     ids, types, pid, pt = inp.beads()

@@ -2,7 +2,7 @@
 stream) against several independent runs of the reference (serial LCG stream) on the
 same control files.  The GPU side runs 8 replicas from independent starting states in
 one batched engine.  The bilayer runs of the reference start from its own lamella builder
-and the GPU replicas from this build's restatement of it, which gives the same states for
+and the GPU replicas from the package's restatement of it, which gives the same states for
 the same seeds (the starting structure alone moves 4000-step bilayer averages by ~0.2% in
 pressure, so the starts must agree).  Criteria: kT within 0.5% and
 pressure within 1% of the reference ensemble mean, and both consistent with it within 4
@@ -120,7 +120,7 @@ def test_binary_ensemble():
 
 def test_condensate_ensemble():
     """The condensate composition (16-mers (S P P P)x4 at 20%, a_SS = 10, Hookean bonds) on a
-    20^3 box, both sides from this build's random states: kT, P, mean bond length and the
+    20^3 box, both sides from the package's random states: kT, P, mean bond length and the
     sticker-sticker g(r)."""
     ref = json.load(open(os.path.join(ENS, "condensate20.json")))
     seeds = list(ref["seeds"]) + [6161 + 104729 * r for r in range(REPLICAS - len(ref["seeds"]))]
